@@ -1,0 +1,194 @@
+"""Benchmark: triangle counting on R-MAT scale 21, edge factor 44 (92,274,688 samples,
+initiator 0.57/0.19/0.19/0.05, seed 1) on one MI355X.
+
+A step is one pass of the batched, degree-bucketed counting launch (captured in a hipGraph)
+over the device-resident oriented CSR. The graph is generated and preprocessed on the device
+before timing; the timed region holds only the counting kernels, the scope of the reference's
+counting time (tc/bs_tc/bs_tc.cu:313-322). Prints one JSON line on stdout; progress goes to
+stderr.
+
+value = oriented edges / second (m+ / t_count), whole job. roofline.achieved = algorithmic
+bytes (sum over edges of 24 + 4*min(s+l, s*(1+ceil(log2(l+1))))) / average launch time.
+cpu_baseline: the oracle's OpenMP restatement of ListIntersectionKernel (1A2) on a bounded
+sample of the same edges, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+METRIC = "TC oriented edges/sec + % HBM roofline, batched vs plain on 1 GPU; query p99"
+HBM_PEAK_GBS = 8000.0
+WORKLOADS = {
+    # name: (scale, edge_factor)
+    "rmat_s21_ef44": (21, 44),
+    "rmat_s18_ef44": (18, 44),
+}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_pkg():
+    import importlib.util
+    if "rtgraph_amd" in sys.modules:
+        return sys.modules["rtgraph_amd"]
+    pkg = ROOT / "rt-graph_amd"
+    spec = importlib.util.spec_from_file_location("rtgraph_amd", pkg / "__init__.py",
+                                                  submodule_search_locations=[str(pkg)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rtgraph_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def golden_triangles(workload: str):
+    try:
+        g = json.loads((ROOT / "tests" / "golden" / "golden.json").read_text())
+        return g.get(f"{workload}_seed1", {}).get("triangles")
+    except Exception:
+        return None
+
+
+def cpu_baseline(off: np.ndarray, adj: np.ndarray, target_s: float) -> dict:
+    """Times the oracle's 1A2 count on a sample of edge ranges spread over the CSR."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc  # checker / CPU baseline only
+    m = adj.size
+    threads = orc.max_threads()
+    nr = 64
+    # calibration on a 0.05% sample
+    def run(frac: float):
+        per = max(1, int(m * frac / nr))
+        t0 = time.perf_counter()
+        tot = 0
+        for r in range(nr):
+            b = (m * r) // nr
+            tot += orc.tc_count_1a2(off, adj, b, min(m, b + per), threads)
+        return per * nr, time.perf_counter() - t0
+    e0, t0 = run(0.0005)
+    rate = e0 / max(t0, 1e-6)
+    frac = min(1.0, max(0.0005, target_s * rate / m))
+    edges, secs = run(frac)
+    return {"value": edges / secs, "unit": "edges/s", "cores": threads, "kind": "port",
+            "sample": f"1A2 (ListIntersectionKernel restated, OpenMP) over {nr} edge ranges spread across the "
+                      f"CSR, {edges} of {m} oriented edges, {secs:.1f} s"}
+
+
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    p = ROOT / "profiles" / f"{kernel}_{workload}_pmc.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("hbm_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="rmat_s21_ef44", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path shards by "
+              "oriented-edge ranges, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    rtg = load_pkg()
+    scale, ef = WORKLOADS[args.workload]
+    nsamples = ef << scale
+    ctx = rtg.Context(0)
+    info = ctx.info()
+    log(f"device {info['name']} {info['arch']} CUs={info['cu_count']}")
+    t0 = time.perf_counter()
+    dev_pairs = rtg.gen_rmat_device(ctx, scale, nsamples, seed=1)
+    log(f"generated {nsamples} R-MAT samples on device in {time.perf_counter() - t0:.2f}s")
+    g = rtg.DeviceGraph.from_pairs(ctx, dev_pairs)
+    bst = g.build_stats
+    dev_pairs.free()
+    log(f"CSR build: n={g.n} m={g.m} max_out={g.max_row} device {bst.t_total_ms:.2f} ms "
+        f"(relabel {bst.t_relabel_ms:.2f}, orient {bst.t_orient_ms:.2f}, sort {bst.t_sort_ms:.2f}, csr {bst.t_csr_ms:.2f})")
+
+    plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=True)
+    log(f"batched plan: {plan.plan_stats['t_plan_ms']:.1f} ms, items={plan.plan_stats['bucket_items']}")
+    for _ in range(args.warmup):
+        cnt, st = plan.run()
+    ctx.synchronize()
+    times = []
+    wall0 = time.perf_counter()
+    for _ in range(args.steps):
+        cnt, st = plan.run()
+        times.append(st["t_count_ms"])
+    ctx.synchronize()
+    wall = time.perf_counter() - wall0
+    t_ms = float(np.mean(times))
+    gold = golden_triangles(args.workload)
+    if gold is not None and cnt != gold:
+        log(f"COUNT MISMATCH: {cnt} != golden {gold}")
+        return 1
+    value = g.m / (t_ms / 1e3)
+    alg = st["alg_bytes"]
+    achieved = alg / (t_ms / 1e3) / 1e9
+    log(f"batched: {t_ms:.3f} ms/step (min {min(times):.3f}), {value:.4g} edges/s, count={cnt}, "
+        f"alg {alg / 1e9:.1f} GB -> {achieved:.0f} GB/s")
+
+    plain = {}
+    if not args.no_plain:
+        for name, method in (("plain_wave", rtg.TC_PLAIN_WAVE), ("plain_thread", rtg.TC_PLAIN_THREAD)):
+            p = g.tc_plan(method=method, use_hip_graph=False)
+            c, s = p.run()
+            assert c == cnt, (name, c, cnt)
+            plain[name] = s["t_count_ms"]
+            log(f"{name}: {s['t_count_ms']:.2f} ms ({g.m / (s['t_count_ms'] / 1e3):.4g} edges/s)")
+            p.close()
+    plain_ms = min(plain.values()) if plain else None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        off, adj = g.download()
+        log("timing the CPU baseline")
+        cpu = cpu_baseline(off, adj, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.4g} edges/s on {cpu['cores']} threads")
+
+    traffic = read_traffic("tc_batched", args.workload)
+    out = {
+        "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": t_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": args.workload, "generator": f"R-MAT scale {scale}, edge factor {ef}, "
+                   "(0.57,0.19,0.19,0.05), seed 1", "n": g.n, "m_oriented": g.m, "triangles": cnt,
+                   "method": "batched (owner-grouped LDS hash, persistent queue, hipGraph)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "plain_ms": plain, "batched_over_plain": (plain_ms / t_ms) if plain_ms else None,
+        "t_build_ms": bst.t_total_ms, "t_plan_ms": plan.plan_stats["t_plan_ms"],
+        "wall_s_timed": wall, "alg_bytes": alg, "merge_bytes": st["merge_bytes"],
+    }
+    print(json.dumps(out), flush=True)
+    plan.close()
+    g.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

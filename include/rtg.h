@@ -1,0 +1,226 @@
+/*
+ * rtg.h — C ABI of librtg, the MI355X (gfx950) triangle-counting, sorted-set-intersection and
+ * sorted-lookup engine.
+ *
+ * Conventions: every entry point returns an int status (RTG_OK = 0) and never throws; on error
+ * rtg_last_error() returns a message for the calling thread. Pointers named d_* are device
+ * (HBM) pointers, obtained from rtg_malloc or from another HIP allocator on the context's
+ * device; all other pointers are host pointers. Compute entry points run on the context's
+ * stream and return after the result is complete.
+ *
+ * The reference exposes no C ABI; its boundary for this path is the C++ function
+ *   void CountOnGPU(int *adjs, int *offsets, int nodes, int edges, int device_id = 0)
+ * (tc/bs_tc/bs_tc.h:5, defined tc/bs_tc/bs_tc.cu:232-354) with its siblings
+ * BinarySearchBasedIntersection (set_intersection/baseline/bs.cu:21-22) and
+ * BSAccessCountFunction (rt_search/simple_bs.h:5). Drop-in replacements with exactly those
+ * signatures are declared in rtg_compat.h; this header is the engine underneath them.
+ */
+#ifndef RTG_H
+#define RTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------------------- */
+#define RTG_OK 0
+#define RTG_EINVAL 1      /* bad argument or precondition (replaces the reference's exit())  */
+#define RTG_EHIP 2        /* HIP runtime error (the reference's CUDA_CHECK exit(2) case,      */
+                          /* tc/utils/common.h:13-21)                                         */
+#define RTG_ENOMEM 3      /* device allocation failed or request exceeds a memory budget      */
+#define RTG_EIO 4         /* file could not be opened or parsed (tc/graph/graph.cpp:33-36)    */
+#define RTG_EUNSUPPORTED 5
+#define RTG_EAGAIN 6      /* server: no free slot / not finished yet                          */
+#define RTG_ECANCELED 7   /* server: request cancelled or timed out                           */
+
+const char *rtg_last_error(void);
+int rtg_version(void);
+
+/* ---- context: device, stream, scratch -------------------------------------------------- */
+typedef struct rtg_ctx rtg_ctx;
+
+typedef struct rtg_device_info {
+    char name[64];
+    char arch[32];      /* gcnArchName, e.g. "gfx950:sramecc+:xnack-"                        */
+    int32_t cu_count;
+    int32_t xcd_count;  /* 8 on MI355X                                                       */
+    int64_t lds_per_block;
+    int64_t hbm_bytes;
+    int32_t clock_khz;
+    int32_t mem_clock_khz;
+} rtg_device_info;
+
+/* Replaces the reference's cudaSetDevice(device_id) inside CountOnGPU (bs_tc.cu:233). */
+int rtg_create(int device, rtg_ctx **out);
+int rtg_destroy(rtg_ctx *ctx);
+int rtg_device_query(rtg_ctx *ctx, rtg_device_info *out);
+int rtg_synchronize(rtg_ctx *ctx);
+
+/* Device memory helpers, so hosts without a HIP toolchain (ctypes, cgo) can stage data. */
+int rtg_malloc(rtg_ctx *ctx, size_t bytes, void **d_ptr);
+int rtg_free(rtg_ctx *ctx, void *d_ptr);
+int rtg_memcpy_h2d(rtg_ctx *ctx, void *d_dst, const void *src, size_t bytes);
+int rtg_memcpy_d2h(rtg_ctx *ctx, void *dst, const void *d_src, size_t bytes);
+int rtg_memcpy_d2d(rtg_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+int rtg_memset(rtg_ctx *ctx, void *d_dst, int value, size_t bytes);
+
+/* ---- graphs: device-resident CSR --------------------------------------------------------- */
+typedef struct rtg_graph rtg_graph;
+
+/* Flags for rtg_graph_from_pairs. */
+#define RTG_PAIRS_RELABEL 0x1u  /* dense order-preserving relabel of the ids present, with 0- or
+                                   1-based detection (Graph::ReadBinaryEdges,
+                                   tc/graph/graph.cpp:40-62, intended semantics)            */
+#define RTG_PAIRS_ORIENT 0x2u   /* Graph::TCPreprocessing + ConvertToCSR
+                                   (tc/graph/graph.cpp:104-177, 71-102): raw-degree
+                                   orientation, degree<=1 removal, sort, dedup, CSR.
+                                   Without it: symmetric simple CSR (both directions,
+                                   sorted, deduplicated, self-loops dropped), for queries.  */
+#define RTG_PAIRS_DEVICE 0x4u   /* `pairs` is a device pointer (else host)                  */
+
+typedef struct rtg_build_stats {
+    double t_total_ms;      /* device time of the whole build                                */
+    double t_relabel_ms;
+    double t_orient_ms;     /* degree, orientation, removal                                  */
+    double t_sort_ms;       /* radix sort of the packed (src,dst) keys                       */
+    double t_csr_ms;        /* dedup, self-loop removal, offsets                             */
+    int64_t raw_pairs;      /* input pairs                                                   */
+    int64_t valid_pairs;    /* after degree<=1 removal (before dedup)                        */
+    int32_t nodes_in;       /* after relabel                                                 */
+    int32_t sort_passes;
+} rtg_build_stats;
+
+int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npairs, uint32_t flags,
+                         rtg_graph **out, rtg_build_stats *stats /* may be NULL */);
+/* Copies a host CSR (offsets[n+1], adjs[m]); the caller keeps ownership. Rows must be sorted,
+   unique and free of self-loops (the CountOnGPU precondition). Validated on the device.   */
+int rtg_graph_from_csr(rtg_ctx *ctx, const int32_t *offsets, const int32_t *adjs, int32_t n,
+                       int64_t m, rtg_graph **out);
+int rtg_graph_info(const rtg_graph *g, int32_t *n, int64_t *m, int32_t *max_row);
+int rtg_graph_download(const rtg_graph *g, int32_t *offsets /* n+1 */, int32_t *adjs /* m */);
+int rtg_graph_device_ptrs(const rtg_graph *g, const int32_t **d_offsets, const int32_t **d_adjs);
+int rtg_graph_destroy(rtg_graph *g);
+
+/* Device-side synthetic inputs (same values as the host generators below). */
+int rtg_gen_rmat_device(rtg_ctx *ctx, uint32_t scale, int64_t nsamples, double a, double b,
+                        double c, uint64_t seed, int32_t *d_pairs /* 2*nsamples */);
+int rtg_gen_uniform_u32_device(rtg_ctx *ctx, int64_t n, uint64_t seed, uint64_t stream,
+                               uint32_t *d_out);
+
+/* ---- triangle counting ----------------------------------------------------------------- */
+#define RTG_TC_PLAIN_THREAD 0  /* thread per oriented edge, 1A2 form of ListIntersectionKernel
+                                  (tc/bs_tc/bs_tc.cu:120-148): for w in N+(u), v in N+(w)?  */
+#define RTG_TC_PLAIN_WAVE 1    /* wave64 per oriented edge: |N+(u) & N+(v)| by binary search
+                                  of the shorter list in the longer                        */
+#define RTG_TC_BATCHED 2       /* degree-bucketed persistent launch, per-bucket strategy      */
+#define RTG_TC_2A1 3           /* two-hop multiset + membership (bs_tc.cu:153-229), memory
+                                  bounded; for graphs whose wedge count fits the budget     */
+
+typedef struct rtg_tc_opts {
+    int32_t method;          /* RTG_TC_*                                                      */
+    int32_t use_hip_graph;   /* batched: replay the captured launch sequence                  */
+    int64_t e_begin;         /* oriented-edge range [e_begin, e_end) (e_end <= 0: all)         */
+    int64_t e_end;
+    const char *tune_cache;  /* batched: JSON tuning cache path (NULL: built-in defaults)      */
+    int32_t autotune;        /* batched: time candidate strategies when no cache entry exists */
+    int32_t reserved;
+} rtg_tc_opts;
+
+#define RTG_TC_MAX_BUCKETS 8
+typedef struct rtg_tc_stats {
+    double t_count_ms;       /* device time of the counting kernels only (bs_tc.cu:313-322)   */
+    double t_plan_ms;        /* planning / bucketing (device), reported separately           */
+    uint64_t alg_bytes;      /* sum over edges of 24 + list_bytes(min,max) (algorithmic bytes)*/
+    uint64_t merge_bytes;    /* 24*m + 4*sum over edges of (d+(u) + d+(v)) (merge-only model)  */
+    int64_t edges;           /* oriented edges counted                                        */
+    int32_t nbuckets;
+    int32_t strategy[RTG_TC_MAX_BUCKETS];
+    int64_t bucket_items[RTG_TC_MAX_BUCKETS];
+    double bucket_ms[RTG_TC_MAX_BUCKETS];
+    int32_t kernel_launches;
+    int32_t reserved;
+} rtg_tc_stats;
+
+int rtg_tc_count(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *opts, uint64_t *count,
+                 rtg_tc_stats *stats /* may be NULL */);
+
+/* A reusable plan: buckets and launch parameters computed once per graph. */
+typedef struct rtg_tc_plan rtg_tc_plan;
+int rtg_tc_plan_create(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *opts,
+                       rtg_tc_plan **out, rtg_tc_stats *stats);
+int rtg_tc_plan_run(rtg_ctx *ctx, rtg_tc_plan *plan, uint64_t *count, rtg_tc_stats *stats);
+int rtg_tc_plan_destroy(rtg_tc_plan *plan);
+
+/* Per-edge counts |N+(u) & N+(v)| for every oriented edge (triangle a<b<c attributed to edge
+   a->b), written to d_out[m]. The 1A2 attribution (edge a->c) is available with form = 1.   */
+int rtg_tc_per_edge(rtg_ctx *ctx, const rtg_graph *g, int32_t form, uint32_t *d_out);
+
+/* ---- sorted-set intersection ---------------------------------------------------------- */
+#define RTG_SI_PLAIN 0     /* one wave per pair, binary search of the shorter in the longer    */
+#define RTG_SI_BATCHED 1   /* size-bucketed, per-bucket strategy                               */
+#define RTG_SI_REFERENCE 2 /* block per pair with an LDS splitter sample, the BlockBasedOBS
+                              scheme (set_intersection/baseline/bs.cu:138-209)                */
+
+typedef struct rtg_si_stats {
+    double t_kernel_ms;
+    double t_plan_ms;
+    uint64_t alg_bytes;  /* sum over pairs of 20 + list_bytes(|A|,|B|)                         */
+    int32_t nbuckets;
+    int32_t reserved;
+    int64_t bucket_items[RTG_TC_MAX_BUCKETS];
+} rtg_si_stats;
+
+/* Lists are sorted, duplicate-free uint32 runs elements[offsets[i] .. offsets[i+1]).
+   For pair k: d_out[k] = |list(pairs[2k]) & list(pairs[2k+1])|.                             */
+int rtg_si_pairs(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements,
+                 int64_t nlists, const uint32_t *d_pairs, int64_t npairs, int32_t method,
+                 uint32_t *d_out, rtg_si_stats *stats);
+/* The reference's A x B tile mode: sets [0,num_a) against [num_a, num_a+num_b);
+   d_counts[a*num_b + b] = |A_a & B_b| (BinarySearchBasedIntersection, bs.cu:91-135).        */
+int rtg_si_tile(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements,
+                uint32_t num_a, uint32_t num_b, int32_t method, uint32_t *d_counts,
+                rtg_si_stats *stats);
+
+/* ---- sorted-array lookup --------------------------------------------------------------- */
+/* out[i] = index of the first element equal to keys[i], or -1 (BSAccessCount,
+   rt_search/simple_bs.cu:35-53). data is ascending, duplicates allowed.                     */
+int rtg_lookup_u32(rtg_ctx *ctx, const uint32_t *d_data, int64_t n, const uint32_t *d_keys,
+                   int64_t q, int32_t *d_out, double *t_kernel_ms);
+int rtg_lookup_i32(rtg_ctx *ctx, const int32_t *d_data, int64_t n, const int32_t *d_keys,
+                   int64_t q, int32_t *d_out, double *t_kernel_ms);
+
+/* ---- host-side generators (seeded, counter-based; no GPU needed) ------------------------ */
+int rtg_gen_ba(int32_t n, int32_t m, uint64_t seed, int32_t *pairs /* 2*m*(n-m) */,
+               int64_t *npairs);
+int rtg_gen_rmat(uint32_t scale, int64_t nsamples, double a, double b, double c, uint64_t seed,
+                 int32_t *pairs /* 2*nsamples */);
+int rtg_gen_powerlaw_degrees(int32_t n, double gamma, double kmax, double mean, uint64_t seed,
+                             int32_t *deg, int64_t *total);
+int rtg_gen_config_model(int32_t n, const int32_t *deg, int64_t total, double p_close,
+                         uint64_t seed, int32_t *pairs /* 2*(total/2 + total) */,
+                         int64_t *npairs);
+int rtg_gen_zipf_lengths(int64_t nlists, double s, uint32_t kmax, uint64_t seed, uint32_t *len);
+int rtg_gen_sorted_lists(int64_t nlists, const uint32_t *offsets, uint32_t range, uint64_t seed,
+                         uint32_t *elements);
+int rtg_gen_sorted_array(int64_t n, uint64_t seed, uint32_t *out);
+int rtg_gen_uniform_u32(int64_t n, uint64_t seed, uint64_t stream, uint32_t *out);
+
+/* ---- host-side file ingest (the reference's formats) ----------------------------------- */
+/* Binary little-endian int32 (src,dst) pairs, npairs = file_size/8 (Graph::ReadBinaryEdges,
+   tc/graph/graph.cpp:23-39). Two calls: pairs == NULL returns the count only.               */
+int rtg_read_binary_pairs(const char *path, int32_t *pairs, int64_t *npairs);
+/* Text edge list, one "src dst" per line (Graph::Txt2Bin, tc/graph/graph.cpp:179-214). */
+int rtg_read_text_pairs(const char *path, int32_t *pairs, int64_t *npairs);
+/* Matrix Market coordinate file, 1-based -> 0-based, symmetric files expanded to both
+   directions (Graph::ReadMtx, bfs/graph/graph.cpp:17-122). */
+int rtg_read_mtx_pairs(const char *path, int32_t *pairs, int64_t *npairs);
+int rtg_write_binary_pairs(const char *path, const int32_t *pairs, int64_t npairs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTG_H */

@@ -1,0 +1,586 @@
+"""rt-graph_amd: Python binding of librtg, the MI355X (gfx950) triangle-counting, sorted-set
+intersection and sorted-lookup engine.
+
+The binding is ctypes over the C ABI declared in include/rtg.h. It mirrors the reference's
+host interface for this path:
+
+* ``Graph`` — ``ReadBinaryEdges`` / ``TCPreprocessing`` / ``ConvertToCSR`` with ``adjs_``,
+  ``offsets_`` and ``graph_info_`` (tc/graph/include/graph.h:10-30), the preprocessing run
+  on the device;
+* ``CountOnGPU(adjs, offsets, nodes, edges, device_id=0)`` (tc/bs_tc/bs_tc.h:5), which prints
+  the reference's report line and also returns the exact 64-bit count;
+* ``BinarySearchBasedIntersection`` (set_intersection/baseline/bs.cu:21-22) and
+  ``BSAccessCountFunction`` (rt_search/simple_bs.h:5).
+
+Importing fails with ImportError when librtg.so has not been built: there is no CPU
+fallback. GPU entry points raise RTGError when no device is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+import sys
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "RTGError", "Context", "DeviceArray", "DeviceGraph", "TcPlan", "Graph", "GraphInfo",
+    "CountOnGPU", "BinarySearchBasedIntersection", "BSAccessCountFunction", "Dataset",
+    "read_dataset", "write_dataset", "TC_PLAIN_THREAD", "TC_PLAIN_WAVE", "TC_BATCHED", "TC_2A1",
+    "SI_PLAIN", "SI_BATCHED", "SI_REFERENCE", "gen_ba", "gen_rmat", "gen_powerlaw_degrees",
+    "gen_config_model", "gen_zipf_lengths", "gen_sorted_lists", "gen_sorted_array",
+    "gen_uniform_u32", "read_binary_pairs", "lib_path",
+]
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "librtg.so"
+
+TC_PLAIN_THREAD, TC_PLAIN_WAVE, TC_BATCHED, TC_2A1 = 0, 1, 2, 3
+SI_PLAIN, SI_BATCHED, SI_REFERENCE = 0, 1, 2
+PAIRS_RELABEL, PAIRS_ORIENT, PAIRS_DEVICE = 0x1, 0x2, 0x4
+TC_MAX_BUCKETS = 8
+
+STATUS = {0: "RTG_OK", 1: "RTG_EINVAL", 2: "RTG_EHIP", 3: "RTG_ENOMEM", 4: "RTG_EIO",
+          5: "RTG_EUNSUPPORTED", 6: "RTG_EAGAIN", 7: "RTG_ECANCELED"}
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+if not _LIB_PATH.exists():
+    raise ImportError(f"{_LIB_PATH} is not built; run rt-graph_amd/build.py (hipcc, gfx950)")
+
+_lib = C.CDLL(str(_LIB_PATH))
+_lib.rtg_last_error.restype = C.c_char_p
+
+
+class RTGError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RTGError(rc, (_lib.rtg_last_error() or b"").decode(errors="replace"))
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return C.c_void_p(a.ptr)
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    assert a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous"
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---- structures (layout of include/rtg.h) ---------------------------------------------------
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("cu_count", C.c_int32),
+                ("xcd_count", C.c_int32), ("lds_per_block", C.c_int64), ("hbm_bytes", C.c_int64),
+                ("clock_khz", C.c_int32), ("mem_clock_khz", C.c_int32)]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [("t_total_ms", C.c_double), ("t_relabel_ms", C.c_double), ("t_orient_ms", C.c_double),
+                ("t_sort_ms", C.c_double), ("t_csr_ms", C.c_double), ("raw_pairs", C.c_int64),
+                ("valid_pairs", C.c_int64), ("nodes_in", C.c_int32), ("sort_passes", C.c_int32)]
+
+
+class TcOpts(C.Structure):
+    _fields_ = [("method", C.c_int32), ("use_hip_graph", C.c_int32), ("e_begin", C.c_int64),
+                ("e_end", C.c_int64), ("tune_cache", C.c_char_p), ("autotune", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class TcStats(C.Structure):
+    _fields_ = [("t_count_ms", C.c_double), ("t_plan_ms", C.c_double), ("alg_bytes", C.c_uint64),
+                ("merge_bytes", C.c_uint64), ("edges", C.c_int64), ("nbuckets", C.c_int32),
+                ("strategy", C.c_int32 * TC_MAX_BUCKETS), ("bucket_items", C.c_int64 * TC_MAX_BUCKETS),
+                ("bucket_ms", C.c_double * TC_MAX_BUCKETS), ("kernel_launches", C.c_int32),
+                ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {
+            "t_count_ms": self.t_count_ms, "t_plan_ms": self.t_plan_ms, "alg_bytes": self.alg_bytes,
+            "merge_bytes": self.merge_bytes, "edges": self.edges, "nbuckets": self.nbuckets,
+            "strategy": list(self.strategy[: self.nbuckets]),
+            "bucket_items": list(self.bucket_items[: self.nbuckets]),
+            "kernel_launches": self.kernel_launches,
+        }
+
+
+class SiStats(C.Structure):
+    _fields_ = [("t_kernel_ms", C.c_double), ("t_plan_ms", C.c_double), ("alg_bytes", C.c_uint64),
+                ("nbuckets", C.c_int32), ("reserved", C.c_int32),
+                ("bucket_items", C.c_int64 * TC_MAX_BUCKETS)]
+
+
+# ---- context and device memory --------------------------------------------------------------
+class Context:
+    """An rtg_ctx: one device, one stream (rtg_create / rtg_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(_lib.rtg_create(C.c_int(device), C.byref(self._h)))
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            _lib.rtg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        d = DeviceInfo()
+        _check(_lib.rtg_device_query(self._h, C.byref(d)))
+        return {"name": d.name.decode(), "arch": d.arch.decode(), "cu_count": d.cu_count,
+                "xcd_count": d.xcd_count, "lds_per_block": d.lds_per_block, "hbm_bytes": d.hbm_bytes,
+                "clock_khz": d.clock_khz, "mem_clock_khz": d.mem_clock_khz}
+
+    def synchronize(self) -> None:
+        _check(_lib.rtg_synchronize(self._h))
+
+    def empty(self, shape, dtype) -> "DeviceArray":
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        if a.nbytes:
+            _check(_lib.rtg_memcpy_h2d(self._h, C.c_void_p(d.ptr), _ptr(a), C.c_size_t(a.nbytes)))
+        return d
+
+
+class DeviceArray:
+    """Device (HBM) buffer owned by Python; freed when collected."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(np.array(shape, dtype=np.int64)).tolist()) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(_lib.rtg_malloc(ctx.handle, C.c_size_t(max(self.nbytes, 1)), C.byref(p)))
+        self.ptr = p.value
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            _check(_lib.rtg_memcpy_d2h(self.ctx.handle, _ptr(out), C.c_void_p(self.ptr), C.c_size_t(self.nbytes)))
+        return out
+
+    def copy_from(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes <= self.nbytes
+        if a.nbytes:
+            _check(_lib.rtg_memcpy_h2d(self.ctx.handle, C.c_void_p(self.ptr), _ptr(a), C.c_size_t(a.nbytes)))
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib.rtg_free(self.ctx.handle, C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx._h:
+                self.free()
+        except Exception:
+            pass
+
+
+# ---- graphs and triangle counting -------------------------------------------------------------
+class DeviceGraph:
+    """A device-resident CSR (rtg_graph)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p, build_stats: BuildStats | None = None):
+        self.ctx = ctx
+        self._h = handle
+        self.build_stats = build_stats
+        n, m, mr = C.c_int32(), C.c_int64(), C.c_int32()
+        _check(_lib.rtg_graph_info(self._h, C.byref(n), C.byref(m), C.byref(mr)))
+        self.n, self.m, self.max_row = n.value, m.value, mr.value
+
+    @classmethod
+    def from_pairs(cls, ctx: Context, pairs, relabel: bool = True, orient: bool = True) -> "DeviceGraph":
+        flags = (PAIRS_RELABEL if relabel else 0) | (PAIRS_ORIENT if orient else 0)
+        if isinstance(pairs, DeviceArray):
+            flags |= PAIRS_DEVICE
+            npairs = pairs.nbytes // 8
+        else:
+            pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+            npairs = pairs.size // 2
+        h = C.c_void_p()
+        st = BuildStats()
+        _check(_lib.rtg_graph_from_pairs(ctx.handle, _ptr(pairs), C.c_int64(npairs), C.c_uint32(flags),
+                                         C.byref(h), C.byref(st)))
+        return cls(ctx, h, st)
+
+    @classmethod
+    def from_csr(cls, ctx: Context, offsets: np.ndarray, adjs: np.ndarray) -> "DeviceGraph":
+        offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        adjs = np.ascontiguousarray(adjs, dtype=np.int32)
+        h = C.c_void_p()
+        _check(_lib.rtg_graph_from_csr(ctx.handle, _ptr(offsets), _ptr(adjs), C.c_int32(offsets.size - 1),
+                                       C.c_int64(adjs.size), C.byref(h)))
+        return cls(ctx, h)
+
+    def download(self) -> tuple[np.ndarray, np.ndarray]:
+        off = np.empty(self.n + 1, dtype=np.int32)
+        adj = np.empty(self.m, dtype=np.int32)
+        _check(_lib.rtg_graph_download(self._h, _ptr(off), _ptr(adj)))
+        return off, adj
+
+    def device_ptrs(self) -> tuple[int, int]:
+        o, a = C.c_void_p(), C.c_void_p()
+        _check(_lib.rtg_graph_device_ptrs(self._h, C.byref(o), C.byref(a)))
+        return o.value, a.value
+
+    def tc_count(self, method: int = TC_BATCHED, e_begin: int = 0, e_end: int = 0,
+                 use_hip_graph: bool = False) -> tuple[int, dict]:
+        opts = TcOpts(method=method, use_hip_graph=int(use_hip_graph), e_begin=e_begin, e_end=e_end)
+        cnt = C.c_uint64()
+        st = TcStats()
+        _check(_lib.rtg_tc_count(self.ctx.handle, self._h, C.byref(opts), C.byref(cnt), C.byref(st)))
+        return cnt.value, st.as_dict()
+
+    def tc_plan(self, method: int = TC_BATCHED, e_begin: int = 0, e_end: int = 0,
+                use_hip_graph: bool = True) -> "TcPlan":
+        return TcPlan(self, method, e_begin, e_end, use_hip_graph)
+
+    def tc_per_edge(self, form: int = 0) -> np.ndarray:
+        out = DeviceArray(self.ctx, (self.m,), np.uint32)
+        _check(_lib.rtg_tc_per_edge(self.ctx.handle, self._h, C.c_int32(form), C.c_void_p(out.ptr)))
+        return out.numpy()
+
+    def close(self) -> None:
+        if self._h:
+            _lib.rtg_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TcPlan:
+    """rtg_tc_plan: buckets, work queue and captured launch sequence, reusable across runs."""
+
+    def __init__(self, g: DeviceGraph, method: int, e_begin: int, e_end: int, use_hip_graph: bool):
+        self.g = g
+        self._h = C.c_void_p()
+        opts = TcOpts(method=method, use_hip_graph=int(use_hip_graph), e_begin=e_begin, e_end=e_end)
+        st = TcStats()
+        _check(_lib.rtg_tc_plan_create(g.ctx.handle, g._h, C.byref(opts), C.byref(self._h), C.byref(st)))
+        self.plan_stats = st.as_dict()
+
+    def run(self) -> tuple[int, dict]:
+        cnt = C.c_uint64()
+        st = TcStats()
+        _check(_lib.rtg_tc_plan_run(self.g.ctx.handle, self._h, C.byref(cnt), C.byref(st)))
+        return cnt.value, st.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            _lib.rtg_tc_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- reference-shaped host interface ----------------------------------------------------------
+class GraphInfo:
+    """graph_info_t (tc/utils/model.h:10-13)."""
+
+    def __init__(self, nodes: int = 0, edges: int = 0):
+        self.nodes, self.edges = nodes, edges
+
+    def __repr__(self):
+        return f"GraphInfo(nodes={self.nodes}, edges={self.edges})"
+
+
+def read_binary_pairs(path: str | os.PathLike) -> np.ndarray:
+    """Little-endian int32 (src, dst) pairs, npairs = file_size / 8 (graph.cpp:23-39)."""
+    size = os.path.getsize(path)
+    return np.fromfile(path, dtype="<i4", count=(size // 8) * 2).astype(np.int32, copy=False)
+
+
+class Graph:
+    """Mirror of the reference's Graph (tc/graph/include/graph.h:10-30).
+
+    ReadBinaryEdges reads the pair file; TCPreprocessing runs the relabel, raw-degree
+    orientation, degree<=1 removal, sort and dedup on the device; ConvertToCSR copies the
+    CSR into ``adjs_`` / ``offsets_`` (numpy int32) and sets ``graph_info_``.
+    """
+
+    def __init__(self, device_id: int = 0):
+        self.device_id = device_id
+        self.adjs_ = np.empty(0, dtype=np.int32)
+        self.offsets_ = np.zeros(1, dtype=np.int32)
+        self.graph_info_ = GraphInfo()
+        self._pairs = None
+        self._ctx = None
+        self._dg = None
+
+    def _context(self) -> Context:
+        if self._ctx is None:
+            self._ctx = Context(self.device_id)
+        return self._ctx
+
+    def ReadBinaryEdges(self, input_file_path) -> None:
+        self._pairs = read_binary_pairs(input_file_path)
+        print(f"The size of graph file: {os.path.getsize(input_file_path)} bytes")
+        self.graph_info_.edges = self._pairs.size // 2
+
+    def SetEdges(self, pairs: np.ndarray) -> None:
+        """Pairs from memory instead of a file (same semantics as ReadBinaryEdges)."""
+        self._pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        self.graph_info_.edges = self._pairs.size // 2
+
+    def TCPreprocessing(self) -> None:
+        if self._pairs is None:
+            raise RTGError(1, "TCPreprocessing: no edges read")
+        self._dg = DeviceGraph.from_pairs(self._context(), self._pairs, relabel=True, orient=True)
+        st = self._dg.build_stats
+        print(f"Graph Info: nodes = {st.nodes_in}, edges = {self._pairs.size // 2}")
+        print(f"TC Preprocessing Time = {st.t_total_ms:.6f} ms")
+
+    def ConvertToCSR(self) -> None:
+        if self._dg is None:
+            raise RTGError(1, "ConvertToCSR: TCPreprocessing has not run")
+        self.offsets_, self.adjs_ = self._dg.download()
+        self.graph_info_ = GraphInfo(self._dg.n, self._dg.m)
+
+    @property
+    def device_graph(self) -> DeviceGraph | None:
+        return self._dg
+
+    @staticmethod
+    def Txt2Bin(txt_file, bin_file) -> None:
+        """Text edge list ("a b" per line) to binary int32 pairs (graph.cpp:179-214)."""
+        out = []
+        with open(txt_file) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 2:
+                    try:
+                        out.append((int(parts[0]), int(parts[1])))
+                        continue
+                    except ValueError:
+                        pass
+                print(f"Error parsing line: {line.rstrip()}", file=sys.stderr)
+        np.asarray(out, dtype="<i4").reshape(-1).tofile(bin_file)
+        print("Edge list has been successfully converted to binary.")
+
+
+def CountOnGPU(adjs, offsets, nodes: int, edges: int, device_id: int = 0,
+               method: int = TC_BATCHED, quiet: bool = False) -> int:
+    """tc/bs_tc/bs_tc.cu:232-354: copies the host CSR to the device, counts, prints
+    ``BSTC: counting time = %f ms, counting result = %u`` and returns the exact count.
+    The reference's counter is a 32-bit uint; the printed %u value is the low 32 bits."""
+    offsets = np.ascontiguousarray(offsets[: nodes + 1], dtype=np.int32)
+    adjs = np.ascontiguousarray(adjs[:edges], dtype=np.int32)
+    with Context(device_id) as ctx:
+        g = DeviceGraph.from_csr(ctx, offsets, adjs)
+        cnt, st = g.tc_count(method=method)
+        g.close()
+    if not quiet:
+        print(f"BSTC: counting time = {st['t_count_ms']:f} ms, counting result = {cnt & 0xFFFFFFFF}")
+        if cnt >> 32:
+            print(f"BSTC: exact 64-bit count = {cnt}")
+    return cnt
+
+
+# ---- set intersection ---------------------------------------------------------------------------
+class Dataset:
+    """set_intersection/utils/util.h:19-28: header (num_a, num_b, max_element, intersection),
+    set_offsets[num_a+num_b+1] and elements (A sets first)."""
+
+    def __init__(self, num_a: int, num_b: int, max_element: int, intersection: int,
+                 set_offsets: np.ndarray, elements: np.ndarray):
+        self.num_of_sets_a, self.num_of_sets_b = int(num_a), int(num_b)
+        self.num_of_sets = self.num_of_sets_a + self.num_of_sets_b
+        self.max_element, self.intersection = int(max_element), int(intersection)
+        self.set_offsets = np.ascontiguousarray(set_offsets, dtype=np.uint32)
+        self.elements = np.ascontiguousarray(elements, dtype=np.uint32)
+
+
+def read_dataset(path) -> Dataset:
+    """ReadData, set_intersection/utils/util.cpp:3-23."""
+    with open(path, "rb") as f:
+        num_a, num_b, max_el, inter = struct.unpack("<4I", f.read(16))
+        offs = np.frombuffer(f.read(4 * (num_a + num_b + 1)), dtype="<u4").astype(np.uint32)
+        elems = np.frombuffer(f.read(4 * int(offs[-1])), dtype="<u4").astype(np.uint32)
+    return Dataset(num_a, num_b, max_el, inter, offs, elems)
+
+
+def write_dataset(path, ds: Dataset) -> None:
+    """Writer format of set_intersection/data_generator/data.cpp:344-373."""
+    with open(path, "wb") as f:
+        f.write(struct.pack("<4I", ds.num_of_sets_a, ds.num_of_sets_b, ds.max_element, ds.intersection))
+        f.write(ds.set_offsets.astype("<u4").tobytes())
+        f.write(ds.elements.astype("<u4").tobytes())
+
+
+def si_pairs(ctx: Context, d_offsets: DeviceArray, d_elements: DeviceArray, nlists: int,
+             d_pairs: DeviceArray, npairs: int, method: int = SI_BATCHED, d_out: DeviceArray | None = None):
+    """rtg_si_pairs: d_out[k] = |list(pairs[2k]) & list(pairs[2k+1])| (device arrays)."""
+    if d_out is None:
+        d_out = DeviceArray(ctx, (npairs,), np.uint32)
+    st = SiStats()
+    _check(_lib.rtg_si_pairs(ctx.handle, _ptr(d_offsets), _ptr(d_elements), C.c_int64(nlists), _ptr(d_pairs),
+                             C.c_int64(npairs), C.c_int32(method), _ptr(d_out), C.byref(st)))
+    return d_out, {"t_kernel_ms": st.t_kernel_ms, "t_plan_ms": st.t_plan_ms, "alg_bytes": st.alg_bytes,
+                   "bucket_items": list(st.bucket_items[: st.nbuckets])}
+
+
+def si_tile(ctx: Context, d_offsets: DeviceArray, d_elements: DeviceArray, num_a: int, num_b: int,
+            method: int = SI_BATCHED):
+    out = DeviceArray(ctx, (num_a * num_b,), np.uint32)
+    st = SiStats()
+    _check(_lib.rtg_si_tile(ctx.handle, _ptr(d_offsets), _ptr(d_elements), C.c_uint32(num_a), C.c_uint32(num_b),
+                            C.c_int32(method), _ptr(out), C.byref(st)))
+    return out, {"t_kernel_ms": st.t_kernel_ms, "alg_bytes": st.alg_bytes}
+
+
+def BinarySearchBasedIntersection(dataset: Dataset, blocks: int = 0, block_size: int = 512,
+                                  device_id: int = 0, method: int = SI_REFERENCE) -> np.ndarray:
+    """set_intersection/baseline/bs.cu:91-135: counts[a * num_b + b] = |A_a & B_b|.
+    ``blocks``/``block_size`` are accepted for signature parity; the launch shape is chosen
+    by the library."""
+    with Context(device_id) as ctx:
+        d_off = ctx.to_device(dataset.set_offsets)
+        d_el = ctx.to_device(dataset.elements if dataset.elements.size else np.zeros(1, np.uint32))
+        out, st = si_tile(ctx, d_off, d_el, dataset.num_of_sets_a, dataset.num_of_sets_b, method)
+        res = out.numpy()
+        print(f"Binary Search based Set Intertsection: kernel time = {st['t_kernel_ms']} ms")
+        for a in (out, d_el, d_off):
+            a.free()
+    return res
+
+
+# ---- sorted lookup ------------------------------------------------------------------------------
+def lookup(ctx: Context, d_data: DeviceArray, n: int, d_keys: DeviceArray, q: int,
+           d_out: DeviceArray | None = None, signed: bool = False):
+    if d_out is None:
+        d_out = DeviceArray(ctx, (q,), np.int32)
+    ms = C.c_double()
+    fn = _lib.rtg_lookup_i32 if signed else _lib.rtg_lookup_u32
+    _check(fn(ctx.handle, _ptr(d_data), C.c_int64(n), _ptr(d_keys), C.c_int64(q), _ptr(d_out), C.byref(ms)))
+    return d_out, ms.value
+
+
+def BSAccessCountFunction(data: np.ndarray, query: np.ndarray, device_id: int = 0) -> np.ndarray:
+    """rt_search/simple_bs.cu:80-142: sorts ``data`` (in place, as the reference does at :92)
+    and returns, per query, the index of the first equal element or -1."""
+    data.sort()
+    d = np.ascontiguousarray(data, dtype=np.int32)
+    qv = np.ascontiguousarray(query, dtype=np.int32)
+    with Context(device_id) as ctx:
+        dd, dq = ctx.to_device(d if d.size else np.zeros(1, np.int32)), ctx.to_device(qv if qv.size else np.zeros(1, np.int32))
+        out, ms = lookup(ctx, dd, d.size, dq, qv.size, signed=True)
+        res = out.numpy()[: qv.size]
+        print(f"Binary search time = {ms:f} ms")
+        for a in (out, dq, dd):
+            a.free()
+    return res
+
+
+# ---- generators (host, seeded, counter-based) -----------------------------------------------------
+def gen_ba(n: int, m: int, seed: int = 1) -> np.ndarray:
+    out = np.empty(2 * m * max(n - m, 0), dtype=np.int32)
+    k = C.c_int64()
+    _check(_lib.rtg_gen_ba(C.c_int32(n), C.c_int32(m), C.c_uint64(seed), _ptr(out), C.byref(k)))
+    return out[: 2 * k.value]
+
+
+def gen_rmat(scale: int, nsamples: int, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+             seed: int = 1) -> np.ndarray:
+    out = np.empty(2 * nsamples, dtype=np.int32)
+    _check(_lib.rtg_gen_rmat(C.c_uint32(scale), C.c_int64(nsamples), C.c_double(a), C.c_double(b),
+                             C.c_double(c), C.c_uint64(seed), _ptr(out)))
+    return out
+
+
+def gen_rmat_device(ctx: Context, scale: int, nsamples: int, a: float = 0.57, b: float = 0.19,
+                    c: float = 0.19, seed: int = 1) -> DeviceArray:
+    out = DeviceArray(ctx, (2 * nsamples,), np.int32)
+    _check(_lib.rtg_gen_rmat_device(ctx.handle, C.c_uint32(scale), C.c_int64(nsamples), C.c_double(a),
+                                    C.c_double(b), C.c_double(c), C.c_uint64(seed), C.c_void_p(out.ptr)))
+    return out
+
+
+def gen_powerlaw_degrees(n: int, gamma: float, kmax: float, mean: float, seed: int = 1) -> np.ndarray:
+    deg = np.empty(n, dtype=np.int32)
+    tot = C.c_int64()
+    _check(_lib.rtg_gen_powerlaw_degrees(C.c_int32(n), C.c_double(gamma), C.c_double(kmax), C.c_double(mean),
+                                         C.c_uint64(seed), _ptr(deg), C.byref(tot)))
+    return deg
+
+
+def gen_config_model(deg: np.ndarray, p_close: float, seed: int = 1) -> np.ndarray:
+    deg = np.ascontiguousarray(deg, dtype=np.int32)
+    total = int(deg.sum(dtype=np.int64))
+    out = np.empty(2 * (total // 2 + total), dtype=np.int32)
+    k = C.c_int64()
+    _check(_lib.rtg_gen_config_model(C.c_int32(deg.size), _ptr(deg), C.c_int64(total), C.c_double(p_close),
+                                     C.c_uint64(seed), _ptr(out), C.byref(k)))
+    return out[: 2 * k.value]
+
+
+def gen_zipf_lengths(nlists: int, s: float = 1.8, kmax: int = 65536, seed: int = 1) -> np.ndarray:
+    out = np.empty(nlists, dtype=np.uint32)
+    _check(_lib.rtg_gen_zipf_lengths(C.c_int64(nlists), C.c_double(s), C.c_uint32(kmax), C.c_uint64(seed),
+                                     _ptr(out)))
+    return out
+
+
+def gen_sorted_lists(lengths: np.ndarray, value_range: int = 1 << 24, seed: int = 1):
+    lengths = np.ascontiguousarray(lengths, dtype=np.uint32)
+    offsets = np.zeros(lengths.size + 1, dtype=np.uint64)
+    np.cumsum(lengths, out=offsets[1:])
+    if offsets[-1] >= 1 << 32:
+        raise RTGError(1, "gen_sorted_lists: more than 2^32 elements")
+    offsets = offsets.astype(np.uint32)
+    elems = np.empty(int(offsets[-1]), dtype=np.uint32)
+    _check(_lib.rtg_gen_sorted_lists(C.c_int64(lengths.size), _ptr(offsets), C.c_uint32(value_range),
+                                     C.c_uint64(seed), _ptr(elems)))
+    return offsets, elems
+
+
+def gen_sorted_array(n: int, seed: int = 1) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint32)
+    _check(_lib.rtg_gen_sorted_array(C.c_int64(n), C.c_uint64(seed), _ptr(out)))
+    return out
+
+
+def gen_uniform_u32(n: int, seed: int = 1, stream: int = 0x51554552) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint32)
+    _check(_lib.rtg_gen_uniform_u32(C.c_int64(n), C.c_uint64(seed), C.c_uint64(stream), _ptr(out)))
+    return out

@@ -1,0 +1,518 @@
+// Device build of the degree-oriented CSR from an edge-pair list: the reference's
+// Graph::ReadBinaryEdges relabel + Graph::TCPreprocessing + Graph::ConvertToCSR
+// (tc/graph/graph.cpp:40-62, 104-177, 71-102) as data-parallel passes over HBM:
+//
+//   relabel   max/zero detection, presence marks, exclusive scan -> dense order-preserving ids
+//   degree    raw degree by atomics (duplicates and self-loops count, graph.cpp:109-114)
+//   orient    src->dst iff deg[src] < deg[dst] or (equal and src < dst) (graph.cpp:116-124),
+//             degree<=1 vertices removed and the rest renumbered x - removes[x] (:126-152),
+//             surviving pairs packed into 64-bit keys (src << b | dst), b = bits(n')
+//   sort      stable LSD radix sort on 2b bits (replaces ThrustSortEdgesInBatch)
+//   csr       drop self-loops and repeats of the previous key (graph.cpp:159-173), compact
+//             by exclusive scan, write adjs and the row offsets (graph.cpp:71-102)
+//
+// The output arrays are byte-identical to the reference semantics (checked against the
+// CPU oracle). Without RTG_PAIRS_ORIENT the same passes build the symmetric simple CSR
+// (both directions of every non-loop pair, sorted, deduplicated, no vertex removed).
+#include "graph.hpp"
+
+#include <cstring>
+
+namespace {
+
+constexpr int kT = 256;
+
+struct MinMax {
+    int32_t mn;
+    int32_t mx;
+};
+
+__global__ void minmax_kernel(const int32_t *__restrict__ v, int64_t n, MinMax *out) {
+    int32_t mn = INT32_MAX, mx = INT32_MIN;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t x = v[i];
+        mn = min(mn, x);
+        mx = max(mx, x);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, __shfl_xor(mn, off, 64));
+        mx = max(mx, __shfl_xor(mx, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&out->mn, mn);
+        atomicMax(&out->mx, mx);
+    }
+}
+
+__global__ void mark_present(const int32_t *__restrict__ v, int64_t n, uint32_t *__restrict__ present) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        present[v[i]] = 1u;
+}
+
+__global__ void apply_map(const int32_t *__restrict__ in, int32_t *__restrict__ out, int64_t n,
+                          const uint32_t *__restrict__ newid) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (int32_t)newid[in[i]];
+}
+
+__global__ void raw_degree(const int32_t *__restrict__ v, int64_t n, uint32_t *__restrict__ deg) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&deg[v[i]], 1u);
+}
+
+__global__ void removal_flags(const uint32_t *__restrict__ deg, int32_t n, uint32_t *__restrict__ rm) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        rm[i] = deg[i] <= 1u ? 1u : 0u;
+}
+
+// Wave-aggregated append: every valid lane gets a slot in out[] via one atomic per wave.
+__device__ __forceinline__ void wave_append(bool valid, uint64_t key, uint64_t *out,
+                                            unsigned long long *counter) {
+    const uint64_t mask = __ballot(valid);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (valid) out[base + __popcll(mask & rtgd::lanemask_lt())] = key;
+}
+
+// Orientation, removal and key packing (TCPreprocessing). removes_ex is the exclusive scan of
+// the removal flags, so removes[x] (inclusive, graph.cpp:128-131) = removes_ex[x] for kept x.
+__global__ void orient_pack(const int32_t *__restrict__ pairs, int64_t np,
+                            const uint32_t *__restrict__ deg, const uint32_t *__restrict__ removes_ex,
+                            int bits, uint64_t *__restrict__ keys, unsigned long long *counter) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // Uniform trip count per wave so the ballots in wave_append see every lane.
+    const int64_t wave_start = start - (start & 63);
+    for (int64_t base = wave_start; base < np; base += stride) {
+        const int64_t i = base + (start & 63);
+        bool valid = false;
+        uint64_t key = 0;
+        if (i < np) {
+            int32_t s = pairs[2 * i], d = pairs[2 * i + 1];
+            uint32_t ds = deg[s], dd = deg[d];
+            if (ds > dd || (ds == dd && s > d)) {
+                int32_t t = s; s = d; d = t;
+                uint32_t u = ds; ds = dd; dd = u;
+            }
+            valid = ds > 1u && dd > 1u;
+            if (valid) {
+                uint64_t ns = (uint64_t)(uint32_t)(s - (int32_t)removes_ex[s]);
+                uint64_t nd = (uint64_t)(uint32_t)(d - (int32_t)removes_ex[d]);
+                key = (ns << bits) | nd;
+            }
+        }
+        wave_append(valid, key, keys, counter);
+    }
+}
+
+// Symmetric mode: both directions of every non-loop pair.
+__global__ void symmetric_pack(const int32_t *__restrict__ pairs, int64_t np, int bits,
+                               uint64_t *__restrict__ keys, unsigned long long *counter) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t wave_start = start - (start & 63);
+    for (int64_t base = wave_start; base < np; base += stride) {
+        const int64_t i = base + (start & 63);
+        bool valid = false;
+        uint64_t k1 = 0, k2 = 0;
+        if (i < np) {
+            uint64_t s = (uint32_t)pairs[2 * i], d = (uint32_t)pairs[2 * i + 1];
+            valid = s != d;
+            k1 = (s << bits) | d;
+            k2 = (d << bits) | s;
+        }
+        wave_append(valid, k1, keys, counter);
+        wave_append(valid, k2, keys, counter);
+    }
+}
+
+__global__ void dedup_flags(const uint64_t *__restrict__ keys, int64_t n, int bits,
+                            uint32_t *__restrict__ flag) {
+    const uint64_t lo = (1ull << bits) - 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        const bool loop = (k >> bits) == (k & lo);
+        const bool repeat = i > 0 && keys[i - 1] == k;
+        flag[i] = (!loop && !repeat) ? 1u : 0u;
+    }
+}
+
+__global__ void dedup_scatter(const uint64_t *__restrict__ keys, int64_t n, int bits,
+                              const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos,
+                              int32_t *__restrict__ src_c, int32_t *__restrict__ adjs) {
+    const uint64_t lo = (1ull << bits) - 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (flag[i]) {
+            const uint64_t k = keys[i];
+            const uint32_t p = pos[i];
+            src_c[p] = (int32_t)(k >> bits);
+            adjs[p] = (int32_t)(k & lo);
+        }
+    }
+}
+
+// offsets[u] = first compacted index whose source is >= u (rows are sorted by source).
+__global__ void fill_offsets(const int32_t *__restrict__ src_c, int64_t m, int32_t n,
+                             int32_t *__restrict__ offsets) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t s = src_c[j];
+        const int32_t p = j == 0 ? -1 : src_c[j - 1];
+        for (int32_t u = p + 1; u <= s; ++u) offsets[u] = (int32_t)j;
+        if (j == m - 1)
+            for (int32_t u = s + 1; u <= n; ++u) offsets[u] = (int32_t)m;
+    }
+}
+
+__global__ void fill_i32(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void max_row_kernel(const int32_t *__restrict__ offsets, int32_t n, int32_t *out) {
+    int32_t mx = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        mx = max(mx, offsets[i + 1] - offsets[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, mx);
+}
+
+// CSR validation for rtg_graph_from_csr: err bit 1 = offsets not monotone / bad bounds,
+// bit 2 = column out of range, bit 4 = row not strictly increasing (duplicate or unsorted),
+// bit 8 = self-loop.
+__global__ void validate_csr(const int32_t *__restrict__ offsets, const int32_t *__restrict__ adjs,
+                             int32_t n, int64_t m, int32_t *err) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = offsets[u], e = offsets[u + 1];
+        int32_t bad = 0;
+        if (b > e || b < 0 || (int64_t)e > m) bad |= 1;
+        else {
+            int32_t prev = -1;
+            for (int32_t j = b; j < e; ++j) {
+                const int32_t v = adjs[j];
+                if (v < 0 || v >= n) bad |= 2;
+                if (v <= prev) bad |= 4;
+                if (v == (int32_t)u) bad |= 8;
+                prev = v;
+            }
+        }
+        if (bad) atomicOr(err, bad);
+    }
+}
+
+unsigned grid_for(rtg_ctx *ctx, int64_t n) {
+    int64_t want = rtg::ceil_div(n, kT);
+    int64_t cap = (int64_t)ctx->cu_count * 16;
+    if (want > cap) want = cap;
+    if (want < 1) want = 1;
+    return (unsigned)want;
+}
+
+int bits_for(int64_t n) {
+    int b = 1;
+    while (b < 31 && ((int64_t)1 << b) < n) ++b;
+    return b;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+namespace rtg {
+
+int csr_max_row(rtg_ctx *ctx, const int32_t *d_offsets, int32_t n, int32_t *out) {
+    *out = 0;
+    if (n <= 0) return RTG_OK;
+    void *s = nullptr;
+    RTG_TRY(ctx_scratch(ctx, 256, &s));
+    int32_t *d = (int32_t *)s;
+    HIP_TRY(hipMemsetAsync(d, 0, sizeof(int32_t), ctx->stream));
+    max_row_kernel<<<grid_for(ctx, n), kT, 0, ctx->stream>>>(d_offsets, n, d);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, d, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RTG_OK;
+}
+
+}  // namespace rtg
+
+extern "C" {
+
+RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npairs, uint32_t flags,
+                                 rtg_graph **out, rtg_build_stats *stats) {
+    using namespace rtg;
+    RTG_TRY(ctx_enter(ctx));
+    if (!out) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: null out");
+    *out = nullptr;
+    if (npairs < 0 || (!pairs && npairs > 0)) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: bad pairs");
+    const bool orient = flags & RTG_PAIRS_ORIENT;
+    const int64_t max_keys = orient ? npairs : 2 * npairs;
+    if (max_keys >= ((int64_t)1 << 31)) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: too many pairs (limit 2^31 keys)");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t st = ctx->stream;
+    hipEvent_t *ev = ctx->ev;
+    const int64_t nv = 2 * npairs;  // endpoint count
+
+    // Working copy of the pairs.
+    DevBuf pbuf;
+    HIP_TRY(hipMalloc(&pbuf.p, sizeof(int32_t) * (size_t)std::max<int64_t>(nv, 1)));
+    int32_t *d_pairs = (int32_t *)pbuf.p;
+    if (npairs > 0)
+        HIP_TRY(hipMemcpyAsync(d_pairs, pairs, sizeof(int32_t) * (size_t)nv,
+                               (flags & RTG_PAIRS_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(ev[0], st));
+
+    // ---- relabel (graph.cpp:40-62) ----
+    MinMax mm{0, -1};
+    if (npairs > 0) {
+        void *s = nullptr;
+        RTG_TRY(ctx_scratch(ctx, 256, &s));
+        MinMax init{INT32_MAX, INT32_MIN};
+        HIP_TRY(hipMemcpyAsync(s, &init, sizeof(init), hipMemcpyHostToDevice, st));
+        minmax_kernel<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, nv, (MinMax *)s);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&mm, s, sizeof(mm), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (mm.mn < 0) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: negative vertex id");
+    }
+    int64_t nodes = npairs > 0 ? (int64_t)mm.mx + 1 : 0;
+    if (nodes >= ((int64_t)1 << 31) - 1) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: vertex ids too large");
+    const int64_t domain = std::max<int64_t>(nodes, 1);
+    DevBuf vbuf;  // vertex-sized scratch: present/newid, degree, removal flags/scan
+    const size_t vbytes = Arena::round(4 * (size_t)(domain + 1)) * 3 + scan_work_bytes(domain + 1) + 4096;
+    HIP_TRY(hipMalloc(&vbuf.p, vbytes));
+    Arena va{(char *)vbuf.p, 0, vbytes};
+    uint32_t *present = va.take<uint32_t>(domain + 1);
+    uint32_t *deg = va.take<uint32_t>(domain + 1);
+    uint32_t *rmx = va.take<uint32_t>(domain + 1);
+    uint32_t *d_small = va.take<uint32_t>(16);
+    void *vscan = va.take<char>(scan_work_bytes(domain + 1));
+
+    if ((flags & RTG_PAIRS_RELABEL) && npairs > 0) {
+        HIP_TRY(hipMemsetAsync(present, 0, 4 * (size_t)(domain + 1), st));
+        mark_present<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, nv, present);
+        HIP_TRY(hipGetLastError());
+        RTG_TRY(scan_exclusive_u32_w(ctx, present, present, domain, d_small, vscan));
+        uint32_t cnt = 0;
+        HIP_TRY(hipMemcpyAsync(&cnt, d_small, 4, hipMemcpyDeviceToHost, st));
+        apply_map<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, d_pairs, nv, present);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(st));
+        nodes = cnt;
+    }
+    HIP_TRY(hipEventRecord(ev[1], st));
+
+    // ---- degree, orientation, removal (graph.cpp:104-152) ----
+    int64_t n_out = nodes;
+    int bits = 1;
+    DevBuf kbuf;
+    const int64_t kcap = std::max<int64_t>(max_keys, 1);
+    const size_t kbytes = Arena::round(8 * (size_t)kcap) * 2 + radix_sort_work_bytes(kcap) +
+                          Arena::round(4 * (size_t)kcap) * 4 + scan_work_bytes(kcap) + 4096;
+    HIP_TRY(hipMalloc(&kbuf.p, kbytes));
+    Arena ka{(char *)kbuf.p, 0, kbytes};
+    uint64_t *keys0 = ka.take<uint64_t>(kcap);
+    uint64_t *keys1 = ka.take<uint64_t>(kcap);
+    void *sort_work = ka.take<char>(radix_sort_work_bytes(kcap));
+    uint32_t *flag = ka.take<uint32_t>(kcap);
+    uint32_t *pos = ka.take<uint32_t>(kcap);
+    int32_t *src_c = ka.take<int32_t>(kcap);
+    unsigned long long *counter = ka.take<unsigned long long>(4);
+    uint32_t *d_m = ka.take<uint32_t>(4);
+    void *kscan = ka.take<char>(scan_work_bytes(kcap));
+
+    HIP_TRY(hipMemsetAsync(counter, 0, sizeof(unsigned long long), st));
+    if (orient) {
+        HIP_TRY(hipMemsetAsync(deg, 0, 4 * (size_t)(domain + 1), st));
+        if (npairs > 0) {
+            raw_degree<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, nv, deg);
+            HIP_TRY(hipGetLastError());
+        }
+        removal_flags<<<grid_for(ctx, nodes), kT, 0, st>>>(deg, (int32_t)nodes, rmx);
+        HIP_TRY(hipGetLastError());
+        RTG_TRY(scan_exclusive_u32_w(ctx, rmx, rmx, nodes, d_small, vscan));
+        uint32_t removed = 0;
+        HIP_TRY(hipMemcpyAsync(&removed, d_small, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        n_out = nodes - removed;
+        bits = bits_for(std::max<int64_t>(n_out, 2));
+        if (npairs > 0) {
+            orient_pack<<<grid_for(ctx, npairs), kT, 0, st>>>(d_pairs, npairs, deg, rmx, bits, keys0, counter);
+            HIP_TRY(hipGetLastError());
+        }
+    } else {
+        bits = bits_for(std::max<int64_t>(n_out, 2));
+        if (npairs > 0) {
+            symmetric_pack<<<grid_for(ctx, npairs), kT, 0, st>>>(d_pairs, npairs, bits, keys0, counter);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    unsigned long long nkeys = 0;
+    HIP_TRY(hipMemcpyAsync(&nkeys, counter, sizeof(nkeys), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipEventRecord(ev[2], st));
+
+    // ---- sort (thrust_helper.cu:16-42) ----
+    uint64_t *kk[2] = {keys0, keys1};
+    int which = 0, passes = 0;
+    RTG_TRY(radix_sort_u64(ctx, kk, (int64_t)nkeys, 2 * bits, &which, &passes, sort_work));
+    uint64_t *sorted = kk[which];
+    HIP_TRY(hipEventRecord(ev[3], st));
+
+    // ---- dedup + CSR (graph.cpp:159-173, 71-102) ----
+    auto *g = new rtg_graph();
+    g->ctx = ctx;
+    g->n = (int32_t)n_out;
+    g->oriented = orient;
+    uint32_t m = 0;
+    if (nkeys > 0) {
+        dedup_flags<<<grid_for(ctx, nkeys), kT, 0, st>>>(sorted, nkeys, bits, flag);
+        HIP_TRY(hipGetLastError());
+        RTG_TRY(scan_exclusive_u32_w(ctx, flag, pos, nkeys, d_m, kscan));
+        HIP_TRY(hipMemcpyAsync(&m, d_m, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    g->m = m;
+    if (hipMalloc(&g->d_offsets, sizeof(int32_t) * (size_t)(n_out + 1)) != hipSuccess ||
+        hipMalloc(&g->d_adjs, sizeof(int32_t) * (size_t)std::max<uint32_t>(m, 1)) != hipSuccess) {
+        rtg_graph_destroy(g);
+        return rtg_fail(RTG_ENOMEM, "rtg_graph_from_pairs: CSR allocation failed");
+    }
+    if (m > 0) {
+        dedup_scatter<<<grid_for(ctx, nkeys), kT, 0, st>>>(sorted, nkeys, bits, flag, pos, src_c, g->d_adjs);
+        fill_offsets<<<grid_for(ctx, m), kT, 0, st>>>(src_c, m, (int32_t)n_out, g->d_offsets);
+    } else {
+        fill_i32<<<grid_for(ctx, n_out + 1), kT, 0, st>>>(g->d_offsets, n_out + 1, 0);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[4], st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int st2 = csr_max_row(ctx, g->d_offsets, g->n, &g->max_row);
+    if (st2 != RTG_OK) {
+        rtg_graph_destroy(g);
+        return st2;
+    }
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->t_relabel_ms = elapsed_ms(ev[0], ev[1]);
+        stats->t_orient_ms = elapsed_ms(ev[1], ev[2]);
+        stats->t_sort_ms = elapsed_ms(ev[2], ev[3]);
+        stats->t_csr_ms = elapsed_ms(ev[3], ev[4]);
+        stats->t_total_ms = elapsed_ms(ev[0], ev[4]);
+        stats->raw_pairs = npairs;
+        stats->valid_pairs = (int64_t)nkeys;
+        stats->nodes_in = (int32_t)nodes;
+        stats->sort_passes = passes;
+    }
+    *out = g;
+    return RTG_OK;
+}
+
+RTG_API int rtg_graph_from_csr(rtg_ctx *ctx, const int32_t *offsets, const int32_t *adjs, int32_t n,
+                               int64_t m, rtg_graph **out) {
+    using namespace rtg;
+    RTG_TRY(ctx_enter(ctx));
+    if (!out || !offsets || n < 0 || m < 0 || (m > 0 && !adjs))
+        return rtg_fail(RTG_EINVAL, "rtg_graph_from_csr: bad arguments");
+    if (offsets[0] != 0 || offsets[n] != m) return rtg_fail(RTG_EINVAL, "rtg_graph_from_csr: offsets[0] != 0 or offsets[n] != m");
+    if (m >= ((int64_t)1 << 31)) return rtg_fail(RTG_EINVAL, "rtg_graph_from_csr: m >= 2^31");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto *g = new rtg_graph();
+    g->ctx = ctx;
+    g->n = n;
+    g->m = m;
+    if (hipMalloc(&g->d_offsets, sizeof(int32_t) * (size_t)(n + 1)) != hipSuccess ||
+        hipMalloc(&g->d_adjs, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)) != hipSuccess) {
+        rtg_graph_destroy(g);
+        return rtg_fail(RTG_ENOMEM, "rtg_graph_from_csr: allocation failed");
+    }
+    hipStream_t st = ctx->stream;
+    int rc = RTG_OK;
+    do {
+        if (hipMemcpyAsync(g->d_offsets, offsets, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+            (m > 0 && hipMemcpyAsync(g->d_adjs, adjs, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, st) != hipSuccess)) {
+            rc = rtg_fail(RTG_EHIP, "rtg_graph_from_csr: copy failed");
+            break;
+        }
+        void *s = nullptr;
+        if ((rc = ctx_scratch(ctx, 256, &s)) != RTG_OK) break;
+        int32_t *err = (int32_t *)s, herr = 0;
+        if (hipMemsetAsync(err, 0, 4, st) != hipSuccess) { rc = rtg_fail(RTG_EHIP, "memset failed"); break; }
+        if (n > 0) validate_csr<<<grid_for(ctx, n), kT, 0, st>>>(g->d_offsets, g->d_adjs, n, m, err);
+        if (hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            rc = rtg_fail(RTG_EHIP, "rtg_graph_from_csr: validation failed to run");
+            break;
+        }
+        if (herr) {
+            rc = rtg_fail(RTG_EINVAL, std::string("rtg_graph_from_csr: invalid CSR (") +
+                                          ((herr & 1) ? "offsets " : "") + ((herr & 2) ? "column-range " : "") +
+                                          ((herr & 4) ? "unsorted-or-duplicate " : "") + ((herr & 8) ? "self-loop" : "") + ")");
+            break;
+        }
+        rc = csr_max_row(ctx, g->d_offsets, n, &g->max_row);
+    } while (0);
+    if (rc != RTG_OK) {
+        rtg_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return RTG_OK;
+}
+
+RTG_API int rtg_graph_info(const rtg_graph *g, int32_t *n, int64_t *m, int32_t *max_row) {
+    if (!g) return rtg_fail(RTG_EINVAL, "null graph");
+    if (n) *n = g->n;
+    if (m) *m = g->m;
+    if (max_row) *max_row = g->max_row;
+    return RTG_OK;
+}
+
+RTG_API int rtg_graph_download(const rtg_graph *g, int32_t *offsets, int32_t *adjs) {
+    if (!g) return rtg_fail(RTG_EINVAL, "null graph");
+    RTG_TRY(rtg::ctx_enter(g->ctx));
+    hipStream_t st = g->ctx->stream;
+    if (offsets) HIP_TRY(hipMemcpyAsync(offsets, g->d_offsets, sizeof(int32_t) * (size_t)(g->n + 1), hipMemcpyDeviceToHost, st));
+    if (adjs && g->m > 0) HIP_TRY(hipMemcpyAsync(adjs, g->d_adjs, sizeof(int32_t) * (size_t)g->m, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return RTG_OK;
+}
+
+RTG_API int rtg_graph_device_ptrs(const rtg_graph *g, const int32_t **d_offsets, const int32_t **d_adjs) {
+    if (!g) return rtg_fail(RTG_EINVAL, "null graph");
+    if (d_offsets) *d_offsets = g->d_offsets;
+    if (d_adjs) *d_adjs = g->d_adjs;
+    return RTG_OK;
+}
+
+RTG_API int rtg_graph_destroy(rtg_graph *g) {
+    if (!g) return RTG_OK;
+    if (g->ctx) (void)hipSetDevice(g->ctx->device);
+    if (g->d_offsets) (void)hipFree(g->d_offsets);
+    if (g->d_adjs) (void)hipFree(g->d_adjs);
+    delete g;
+    return RTG_OK;
+}
+
+}  // extern "C"

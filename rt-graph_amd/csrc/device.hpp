@@ -1,0 +1,127 @@
+// Device-side building blocks shared by the gfx950 kernels: HIP status checking that reports
+// through the C ABI, wave64 reductions and scans, and the context structure.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rtg_internal.hpp"
+
+#define RTG_WAVE 64
+
+// Evaluates a HIP call; on failure records the error and returns RTG_EHIP from the caller.
+#define HIP_TRY(call)                                                                       \
+    do {                                                                                    \
+        hipError_t _e = (call);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return rtg::set_error(RTG_EHIP, std::string(#call) + ": " + hipGetErrorString(_e) + \
+                                                " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
+    } while (0)
+
+// Same for functions whose status is an int RTG_* code.
+#define RTG_TRY(call)                \
+    do {                             \
+        int _s = (call);             \
+        if (_s != RTG_OK) return _s; \
+    } while (0)
+
+struct rtg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop{};
+    int cu_count = 0;
+    std::mutex mu;  // serialises entry points that share this context's stream and scratch
+    // Grow-only scratch owned by the context (released by rtg_destroy).
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    hipEvent_t ev[8] = {};
+};
+
+namespace rtg {
+
+// Makes the context's device current on this thread.
+int ctx_enter(rtg_ctx *ctx);
+
+// Returns at least `bytes` of device scratch owned by the context (contents undefined).
+int ctx_scratch(rtg_ctx *ctx, size_t bytes, void **out);
+
+// Carves aligned sub-buffers out of one allocation.
+struct Arena {
+    char *base = nullptr;
+    size_t used = 0;
+    size_t cap = 0;
+    template <class T>
+    T *take(size_t count) {
+        size_t off = (used + 255) & ~(size_t)255;
+        used = off + count * sizeof(T);
+        return reinterpret_cast<T *>(base + off);
+    }
+    static size_t round(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+};
+
+// Milliseconds between two recorded events.
+inline float elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0.f;
+    return ms;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- device-wide primitives (scan.hip, radix_sort.hip) ----
+// Exclusive scan of n values into out (may alias in); the total goes to *d_total (device
+// pointer) when non-null. `work` holds scan_work_bytes(n) bytes of device scratch.
+size_t scan_work_bytes(int64_t n);
+int scan_exclusive_u32_w(rtg_ctx *ctx, const uint32_t *in, uint32_t *out, int64_t n,
+                         uint32_t *d_total, void *work);
+int scan_exclusive_u64_w(rtg_ctx *ctx, const uint64_t *in, uint64_t *out, int64_t n,
+                         uint64_t *d_total, void *work);
+// Stable LSD radix sort of n uint64 keys on bits [0, bits). Input in keys[0], the result ends
+// in keys[*which] (0 or 1); keys[1] is a same-size buffer. `work` holds
+// radix_sort_work_bytes(n) bytes.
+size_t radix_sort_work_bytes(int64_t n);
+int radix_sort_u64(rtg_ctx *ctx, uint64_t *keys[2], int64_t n, int bits, int *which,
+                   int *passes, void *work);
+
+}  // namespace rtg
+
+// ---- wave64 helpers (device) ----
+#if defined(__HIPCC__)
+namespace rtgd {
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Inclusive scan across the 64 lanes.
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+}  // namespace rtgd
+#endif

@@ -1,0 +1,706 @@
+// Triangle counting on the degree-oriented CSR.
+//
+// Plain kernels (single launch, no bucketing):
+//   tc_plain_thread  thread per oriented edge, the 1A2 form of ListIntersectionKernel
+//                    (tc/bs_tc/bs_tc.cu:120-148): for w in N+(u), BinaryCheck(N+(w), v); the
+//                    per-edge row comes from a precomputed row[] instead of the reference's
+//                    binary search over offsets (:126-134), and counts are reduced per wave
+//                    before one 64-bit atomic instead of one 32-bit atomic per edge (:145).
+//   tc_plain_wave    wave64 per oriented edge: every lane binary-searches elements of the
+//                    shorter of N+(u), N+(v) in the longer.
+//
+// Batched kernel (tc_batched): every edge (u,v) with both lists non-empty is assigned to its
+// owner — the endpoint with the longer out-list — and grouped by owner at plan time. A
+// persistent grid pulls items from a queue ordered heaviest first. For an item the owner's
+// out-list is inserted into an LDS open-addressing hash table (block-wide for large owners,
+// wave-private for small ones), then the partners' out-lists — always the shorter side — are
+// streamed with coalesced loads, flattened across the 64 lanes of a wave, and probed. Work per
+// edge is min(d+(u), d+(v)) LDS probes and 4*min bytes streamed; the longer list is read once
+// per item instead of once per edge.
+#include "tc_kernels.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kPlainThreads = 256;
+
+__global__ void edge_rows_kernel(const int32_t *__restrict__ off, int32_t n, int32_t *__restrict__ row) {
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+         u += (int64_t)gridDim.x * blockDim.x)
+        for (int32_t e = off[u]; e < off[u + 1]; ++e) row[e] = (int32_t)u;
+}
+
+__global__ void reset_counters(unsigned long long *c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
+}
+
+// BinaryCheck, tc/bs_tc/bs_tc.cu:14-25, on data[0 .. len-1].
+__device__ __forceinline__ uint32_t binary_check(const int32_t *__restrict__ data, int32_t len, int32_t val) {
+    int32_t l = 0, r = len - 1;
+    while (l <= r) {
+        const int32_t mid = (l + r) >> 1;
+        const int32_t x = data[mid];
+        if (x == val) return 1u;
+        if (x < val) l = mid + 1;
+        else r = mid - 1;
+    }
+    return 0u;
+}
+
+__global__ __launch_bounds__(kPlainThreads) void tc_plain_thread(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj, const int32_t *__restrict__ row,
+    int64_t e_begin, int64_t e_end, unsigned long long *count) {
+    uint64_t local = 0;
+    for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t u = row[e], v = adj[e];
+        const int32_t b = off[u], en = off[u + 1];
+        uint32_t cnt = 0;
+        for (int32_t i = b; i < en; ++i) {
+            const int32_t w = adj[i];
+            const int32_t wb = off[w], wl = off[w + 1] - wb;
+            if (wl) cnt += binary_check(adj + wb, wl, v);
+        }
+        local += cnt;
+    }
+    local = rtgd::wave_sum_u64(local);
+    if (__lane_id() == 0 && local) atomicAdd(count, (unsigned long long)local);
+}
+
+__global__ __launch_bounds__(kPlainThreads) void tc_plain_wave(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj, const int32_t *__restrict__ row,
+    int64_t e_begin, int64_t e_end, unsigned long long *count) {
+    const int lane = __lane_id();
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    uint64_t local = 0;
+    for (int64_t e = e_begin + (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); e < e_end; e += nw) {
+        const int32_t u = row[e], v = adj[e];
+        int32_t a0 = off[u], na = off[u + 1] - a0;
+        int32_t b0 = off[v], nb = off[v + 1] - b0;
+        if (na > nb) {
+            int32_t t = a0; a0 = b0; b0 = t;
+            t = na; na = nb; nb = t;
+        }
+        if (nb == 0) continue;
+        for (int32_t i = lane; i < na; i += 64) local += binary_check(adj + b0, nb, adj[a0 + i]);
+    }
+    local = rtgd::wave_sum_u64(local);
+    if (lane == 0 && local) atomicAdd(count, (unsigned long long)local);
+}
+
+// Per-edge counts, intersection attribution: out[e] = |N+(u) & N+(v)| (wave per edge).
+__global__ __launch_bounds__(kPlainThreads) void tc_per_edge_intersect(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj, const int32_t *__restrict__ row,
+    int64_t m, uint32_t *__restrict__ out) {
+    const int lane = __lane_id();
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t e = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); e < m; e += nw) {
+        const int32_t u = row[e], v = adj[e];
+        int32_t a0 = off[u], na = off[u + 1] - a0;
+        int32_t b0 = off[v], nb = off[v + 1] - b0;
+        if (na > nb) {
+            int32_t t = a0; a0 = b0; b0 = t;
+            t = na; na = nb; nb = t;
+        }
+        uint32_t c = 0;
+        if (nb > 0)
+            for (int32_t i = lane; i < na; i += 64) c += binary_check(adj + b0, nb, adj[a0 + i]);
+        c = rtgd::wave_sum_u32(c);
+        if (lane == 0) out[e] = c;
+    }
+}
+
+// Per-edge counts, 1A2 attribution of ListIntersectionKernel (bs_tc.cu:135-145).
+__global__ __launch_bounds__(kPlainThreads) void tc_per_edge_1a2(
+    const int32_t *__restrict__ off, const int32_t *__restrict__ adj, const int32_t *__restrict__ row,
+    int64_t m, uint32_t *__restrict__ out) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t u = row[e], v = adj[e];
+        uint32_t cnt = 0;
+        for (int32_t i = off[u]; i < off[u + 1]; ++i) {
+            const int32_t w = adj[i];
+            const int32_t wb = off[w], wl = off[w + 1] - wb;
+            if (wl) cnt += binary_check(adj + wb, wl, v);
+        }
+        out[e] = cnt;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batched kernel.
+
+__device__ __forceinline__ uint32_t hslot(uint32_t x, int log_t) { return (x * 0x9E3779B1u) >> (32 - log_t); }
+
+__device__ __forceinline__ void table_insert(uint32_t *tab, int log_t, uint32_t x) {
+    const uint32_t mask = (1u << log_t) - 1u;
+    uint32_t h = hslot(x, log_t);
+    while (true) {
+        const uint32_t old = atomicCAS(&tab[h], kEmpty, x);
+        if (old == kEmpty) return;
+        h = (h + 1u) & mask;
+    }
+}
+
+__device__ __forceinline__ uint32_t table_has(const uint32_t *tab, int log_t, uint32_t x) {
+    const uint32_t mask = (1u << log_t) - 1u;
+    uint32_t h = hslot(x, log_t);
+    while (true) {
+        const uint32_t t = tab[h];
+        if (t == x) return 1u;
+        if (t == kEmpty) return 0u;
+        h = (h + 1u) & mask;
+    }
+}
+
+__device__ __forceinline__ int table_log2(int32_t d, int32_t max_slots) {
+    // smallest power of two >= 2d, at least 64 slots
+    int lg = 6;
+    while ((1 << lg) < 2 * d && (1 << lg) < max_slots) ++lg;
+    return lg;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Streams the out-lists of partners wpart[p0 .. p1) through the 64 lanes of one wave, probing
+// every element against `tab`. Partners are taken 64 at a time; their lengths are scanned
+// into the wave's LDS slots and each lane walks a cursor over them, so that lane j of a round
+// handles element j of the concatenated lists.
+__device__ __forceinline__ uint32_t stream_partners(const int32_t *__restrict__ off,
+                                                    const int32_t *__restrict__ adj,
+                                                    const int32_t *__restrict__ wpart, int64_t p0,
+                                                    int64_t p1, const uint32_t *tab, int log_t,
+                                                    int32_t *ex, int32_t *st) {
+    const int lane = __lane_id();
+    uint32_t hits = 0;
+    for (int64_t bs = p0; bs < p1; bs += 64) {
+        const int64_t pi = bs + lane;
+        int32_t ys = 0, yl = 0;
+        if (pi < p1) {
+            const int32_t y = wpart[pi];
+            ys = off[y];
+            yl = off[y + 1] - ys;
+        }
+        const int32_t incl = rtgd::wave_inclusive_scan(yl);
+        const int32_t total = __shfl(incl, 63, 64);
+        ex[lane] = incl - yl;
+        st[lane] = ys;
+        if (lane == 63) ex[64] = total;
+        wave_sync_lds();
+        int k = 0;
+        constexpr int U = 4;
+        for (int32_t jb = 0; jb < total; jb += 64 * U) {
+            uint32_t x[U];
+            bool ok[U];
+#pragma unroll
+            for (int uu = 0; uu < U; ++uu) {
+                const int32_t j = jb + uu * 64 + lane;
+                ok[uu] = j < total;
+                x[uu] = kEmpty;
+                if (ok[uu]) {
+                    while (ex[k + 1] <= j) ++k;
+                    x[uu] = (uint32_t)adj[st[k] + (j - ex[k])];
+                }
+            }
+#pragma unroll
+            for (int uu = 0; uu < U; ++uu)
+                if (ok[uu]) hits += table_has(tab, log_t, x[uu]);
+        }
+        wave_sync_lds();
+    }
+    return hits;
+}
+
+struct BatchArgs {
+    const int32_t *off;
+    const int32_t *adj;
+    const int32_t *wofs;
+    const int32_t *wpart;
+    const int32_t *small_list;
+    const TcItem *items;
+    int64_t n_items;
+    unsigned long long *count;  // [0] total, [1] queue head
+    int32_t max_table;          // block table slots
+    int32_t wave_table;         // wave-private table slots
+};
+
+template <int kThreads>
+__global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
+    constexpr int kWaves = kThreads / 64;
+    extern __shared__ uint32_t table[];
+    __shared__ int32_t ex_all[kWaves][65];
+    __shared__ int32_t st_all[kWaves][64];
+    __shared__ int64_t item_slot;
+    __shared__ int32_t next_batch;
+
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    int32_t *ex = ex_all[wave];
+    int32_t *st = st_all[wave];
+    uint64_t hits = 0;
+
+    while (true) {
+        if (threadIdx.x == 0) {
+            item_slot = (int64_t)atomicAdd(&a.count[1], 1ull);
+            next_batch = 0;
+        }
+        __syncthreads();
+        const int64_t it_idx = item_slot;
+        if (it_idx >= a.n_items) break;
+        const TcItem it = a.items[it_idx];
+        if (it.kind == 0) {
+            // Block item: owner it.a, partners [it.b, it.c).
+            const int32_t x = it.a;
+            const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
+            const int log_t = table_log2(d, a.max_table);
+            const int32_t slots = 1 << log_t;
+            for (int32_t i = threadIdx.x; i < slots; i += kThreads) table[i] = kEmpty;
+            __syncthreads();
+            for (int32_t i = threadIdx.x; i < d; i += kThreads) table_insert(table, log_t, (uint32_t)a.adj[xb + i]);
+            __syncthreads();
+            while (true) {
+                int32_t b = 0;
+                if (lane == 0) b = atomicAdd(&next_batch, 64);
+                b = __shfl(b, 0, 64);
+                const int64_t p0 = (int64_t)it.b + b;
+                if (p0 >= it.c) break;
+                const int64_t p1 = min((int64_t)it.c, p0 + 64);
+                hits += stream_partners(a.off, a.adj, a.wpart, p0, p1, table, log_t, ex, st);
+            }
+        } else {
+            // Wave items: small owners small_list[it.b, it.c), one per wave at a time.
+            uint32_t *tab = table + wave * a.wave_table;
+            for (int32_t s = it.b + wave; s < it.c; s += kWaves) {
+                const int32_t x = a.small_list[s];
+                const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
+                const int log_t = table_log2(d, a.wave_table);
+                const int32_t slots = 1 << log_t;
+                for (int32_t i = lane; i < slots; i += 64) tab[i] = kEmpty;
+                wave_sync_lds();
+                for (int32_t i = lane; i < d; i += 64) table_insert(tab, log_t, (uint32_t)a.adj[xb + i]);
+                wave_sync_lds();
+                hits += stream_partners(a.off, a.adj, a.wpart, a.wofs[x], a.wofs[x + 1], tab, log_t, ex, st);
+                wave_sync_lds();
+            }
+        }
+        __syncthreads();
+    }
+    hits = rtgd::wave_sum_u64(hits);
+    if (lane == 0 && hits) atomicAdd(&a.count[0], (unsigned long long)hits);
+}
+
+// ---- planning kernels ----
+
+// Per edge: owner/partner assignment, per-owner partner count and streamed work, and the
+// algorithmic / merge-only byte models.
+__global__ void plan_owner_counts(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                  const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
+                                  uint32_t *__restrict__ cnt, unsigned long long *__restrict__ work,
+                                  unsigned long long *bytes /* [0] alg, [1] merge, [2] edges */) {
+    uint64_t alg = 0, mrg = 0, ne = 0;
+    for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t u = row[e], v = adj[e];
+        const uint32_t du = (uint32_t)(off[u + 1] - off[u]), dv = (uint32_t)(off[v + 1] - off[v]);
+        const uint32_t s = min(du, dv), l = max(du, dv);
+        const uint32_t lg = l ? 32u - __clz(l) : 0u;  // ceil(log2(l+1))
+        alg += 24ull + 4ull * min((uint64_t)s + l, (uint64_t)s * (1u + lg));
+        mrg += 24ull + 4ull * ((uint64_t)du + dv);
+        if (s == 0) continue;
+        const int32_t owner = (du > dv || (du == dv && u < v)) ? u : v;
+        atomicAdd(&cnt[owner], 1u);
+        atomicAdd(&work[owner], (unsigned long long)s);
+        ne += 1;
+    }
+    alg = rtgd::wave_sum_u64(alg);
+    mrg = rtgd::wave_sum_u64(mrg);
+    ne = rtgd::wave_sum_u64(ne);
+    if (__lane_id() == 0) {
+        atomicAdd(&bytes[0], (unsigned long long)alg);
+        atomicAdd(&bytes[1], (unsigned long long)mrg);
+        atomicAdd(&bytes[2], (unsigned long long)ne);
+    }
+}
+
+__global__ void plan_scatter(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                             const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
+                             const uint32_t *__restrict__ wofs, uint32_t *__restrict__ cursor,
+                             int32_t *__restrict__ wpart) {
+    for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t u = row[e], v = adj[e];
+        const int32_t du = off[u + 1] - off[u], dv = off[v + 1] - off[v];
+        if (du == 0 || dv == 0) continue;
+        const bool uo = (du > dv || (du == dv && u < v));
+        const int32_t owner = uo ? u : v, partner = uo ? v : u;
+        const uint32_t slot = atomicAdd(&cursor[owner], 1u);
+        wpart[wofs[owner] + slot] = partner;
+    }
+}
+
+unsigned grid_cap(rtg_ctx *ctx, int64_t n, int threads, int per_cu) {
+    int64_t want = rtg::ceil_div(std::max<int64_t>(n, 1), threads);
+    int64_t cap = (int64_t)ctx->cu_count * per_cu;
+    return (unsigned)std::max<int64_t>(1, std::min(want, cap));
+}
+
+struct DevFree {
+    void *p = nullptr;
+    ~DevFree() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+template <class T>
+int dev_alloc(T **p, size_t count) {
+    if (hipMalloc((void **)p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return rtg::set_error(RTG_ENOMEM, "device allocation of " + std::to_string(sizeof(T) * count) + " bytes failed");
+    }
+    return RTG_OK;
+}
+
+// Tunables of the batched plan.
+struct BatchTune {
+    int32_t block_threads = 256;
+    int64_t item_work = 1 << 16;   // streamed elements per block item
+    int64_t wave_item_work = 1 << 14;
+    int32_t wave_owner_max = -1;   // -1: derived from the table size
+};
+
+int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
+    using namespace rtg;
+    const rtg_graph *g = p->g;
+    const int32_t n = g->n;
+    hipStream_t st = ctx->stream;
+    uint32_t *d_cnt = nullptr;
+    unsigned long long *d_work = nullptr, *d_bytes = nullptr;
+    DevFree f1, f2, f3, f4, f5;
+    RTG_TRY(dev_alloc(&d_cnt, (size_t)n + 1));
+    f1.p = d_cnt;
+    RTG_TRY(dev_alloc(&d_work, (size_t)n + 1));
+    f2.p = d_work;
+    RTG_TRY(dev_alloc(&d_bytes, 4));
+    f3.p = d_bytes;
+    HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (size_t)(n + 1), st));
+    HIP_TRY(hipMemsetAsync(d_work, 0, sizeof(unsigned long long) * (size_t)(n + 1), st));
+    HIP_TRY(hipMemsetAsync(d_bytes, 0, sizeof(unsigned long long) * 4, st));
+    const int64_t ne = p->e_end - p->e_begin;
+    if (ne > 0) {
+        plan_owner_counts<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin,
+                                                                     p->e_end, d_cnt, d_work, d_bytes);
+        HIP_TRY(hipGetLastError());
+    }
+    RTG_TRY(dev_alloc(&p->d_wofs, (size_t)n + 1));
+    void *scan_work = nullptr;
+    RTG_TRY(dev_alloc((char **)&scan_work, scan_work_bytes(n + 1)));
+    f4.p = scan_work;
+    RTG_TRY(scan_exclusive_u32_w(ctx, d_cnt, (uint32_t *)p->d_wofs, n + 1, nullptr, scan_work));
+    unsigned long long hb[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(hb, d_bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    p->alg_bytes = hb[0];
+    p->merge_bytes = hb[1];
+    p->n_work_edges = (int64_t)hb[2];
+    RTG_TRY(dev_alloc(&p->d_wpart, (size_t)std::max<int64_t>(p->n_work_edges, 1)));
+    HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (size_t)(n + 1), st));
+    if (ne > 0) {
+        plan_scatter<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end,
+                                                                (const uint32_t *)p->d_wofs, d_cnt, p->d_wpart);
+        HIP_TRY(hipGetLastError());
+    }
+    // Host side: item list from per-owner partner counts and work.
+    std::vector<int32_t> hoff((size_t)n + 1), hcnt((size_t)n + 1);
+    std::vector<unsigned long long> hwork((size_t)n + 1);
+    HIP_TRY(hipMemcpyAsync(hoff.data(), g->d_offsets, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hwork.data(), d_work, sizeof(unsigned long long) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hcnt.data(), p->d_wofs, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+
+    const int threads = tune.block_threads;
+    const int waves = threads / 64;
+    int32_t max_d = g->max_row;
+    // Block table: power of two >= 2 * max out-degree (at least 4 wave tables of 256 slots).
+    int32_t max_table = 1024;
+    while (max_table < 2 * max_d) max_table <<= 1;
+    if (max_table > 32768) return set_error(RTG_EUNSUPPORTED, "batched TC: out-degree above 16384 needs the global-table strategy");
+    const int32_t wave_table = max_table / waves;
+    int32_t wave_owner_max = tune.wave_owner_max >= 0 ? std::min(tune.wave_owner_max, wave_table / 2) : wave_table / 2;
+    p->max_table = max_table;
+    p->block_threads = threads;
+    p->wave_owner_max = wave_owner_max;
+
+    std::vector<TcItem> items;
+    std::vector<std::pair<uint64_t, int32_t>> small;  // (work, owner)
+    std::vector<uint64_t> item_cost;
+    for (int32_t x = 0; x < n; ++x) {
+        const int32_t c = hcnt[x + 1] - hcnt[x];
+        if (c == 0) continue;
+        const int32_t d = hoff[x + 1] - hoff[x];
+        const uint64_t w = hwork[x];
+        if (d <= wave_owner_max) {
+            small.push_back({w + (uint64_t)d, x});
+            continue;
+        }
+        int64_t chunks = std::max<int64_t>(1, (int64_t)((w + tune.item_work / 2) / tune.item_work));
+        chunks = std::min<int64_t>(chunks, c);
+        for (int64_t k = 0; k < chunks; ++k) {
+            const int32_t b = hcnt[x] + (int32_t)(c * k / chunks), e = hcnt[x] + (int32_t)(c * (k + 1) / chunks);
+            items.push_back(TcItem{0, x, b, e});
+            item_cost.push_back(w / chunks + (uint64_t)d);
+        }
+    }
+    // Heaviest first.
+    std::vector<int64_t> order(items.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t l, int64_t r) { return item_cost[l] > item_cost[r]; });
+    std::vector<TcItem> queue;
+    queue.reserve(items.size() + small.size() / 4 + 1);
+    for (int64_t i : order) queue.push_back(items[i]);
+    std::stable_sort(small.begin(), small.end(), [](const auto &l, const auto &r) { return l.first > r.first; });
+    std::vector<int32_t> small_list(small.size());
+    for (size_t i = 0; i < small.size(); ++i) small_list[i] = small[i].second;
+    // Wave items: groups of `waves` or more consecutive small owners with bounded total work.
+    size_t i = 0;
+    while (i < small.size()) {
+        size_t j = i;
+        uint64_t w = 0;
+        while (j < small.size() && (j - i < (size_t)waves || w < (uint64_t)tune.wave_item_work * waves)) {
+            w += small[j].first;
+            ++j;
+        }
+        queue.push_back(TcItem{1, 0, (int32_t)i, (int32_t)j});
+        i = j;
+    }
+    p->n_items = (int64_t)queue.size();
+    p->n_small = (int64_t)small_list.size();
+    RTG_TRY(dev_alloc(&p->d_items, queue.size()));
+    RTG_TRY(dev_alloc(&p->d_small, small_list.size()));
+    if (!queue.empty())
+        HIP_TRY(hipMemcpyAsync(p->d_items, queue.data(), sizeof(TcItem) * queue.size(), hipMemcpyHostToDevice, st));
+    if (!small_list.empty())
+        HIP_TRY(hipMemcpyAsync(p->d_small, small_list.data(), sizeof(int32_t) * small_list.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+
+    // Persistent grid: as many blocks as fit on the device at once.
+    const size_t lds = sizeof(uint32_t) * (size_t)max_table;
+    int per_cu = 0;
+    if (threads == 256) {
+        HIP_TRY(hipFuncSetAttribute((const void *)tc_batched<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tc_batched<256>, threads, lds));
+    } else if (threads == 512) {
+        HIP_TRY(hipFuncSetAttribute((const void *)tc_batched<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tc_batched<512>, threads, lds));
+    } else {
+        return set_error(RTG_EINVAL, "batched TC: block_threads must be 256 or 512");
+    }
+    if (per_cu < 1) per_cu = 1;
+    p->grid = (int32_t)std::min<int64_t>((int64_t)per_cu * ctx->cu_count, std::max<int64_t>(p->n_items, 1));
+    return RTG_OK;
+}
+
+int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
+    const rtg_graph *g = p->g;
+    hipStream_t st = ctx->stream;
+    reset_counters<<<1, 64, 0, st>>>(p->d_count);
+    HIP_TRY(hipGetLastError());
+    const int64_t ne = p->e_end - p->e_begin;
+    switch (p->opts.method) {
+        case RTG_TC_PLAIN_THREAD:
+            if (ne > 0)
+                tc_plain_thread<<<grid_cap(ctx, ne, kPlainThreads, 32), kPlainThreads, 0, st>>>(
+                    g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end, p->d_count);
+            break;
+        case RTG_TC_PLAIN_WAVE:
+            if (ne > 0)
+                tc_plain_wave<<<grid_cap(ctx, ne * 64, kPlainThreads, 32), kPlainThreads, 0, st>>>(
+                    g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end, p->d_count);
+            break;
+        case RTG_TC_BATCHED: {
+            if (p->n_items == 0) break;
+            BatchArgs a{g->d_offsets, g->d_adjs, p->d_wofs, p->d_wpart, p->d_small, p->d_items,
+                        p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64)};
+            const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
+            if (p->block_threads == 256) tc_batched<256><<<p->grid, 256, lds, st>>>(a);
+            else tc_batched<512><<<p->grid, 512, lds, st>>>(a);
+            break;
+        }
+        default:
+            return rtg::set_error(RTG_EINVAL, "unknown TC method");
+    }
+    HIP_TRY(hipGetLastError());
+    return RTG_OK;
+}
+
+}  // namespace
+
+int rtg_tc_plan_create_impl(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *opts, rtg_tc_plan **out) {
+    using namespace rtg;
+    auto *p = new rtg_tc_plan();
+    p->ctx = ctx;
+    p->g = g;
+    p->opts = *opts;
+    p->e_begin = std::max<int64_t>(0, opts->e_begin);
+    p->e_end = (opts->e_end <= 0 || opts->e_end > g->m) ? g->m : opts->e_end;
+    if (p->e_begin > p->e_end) p->e_begin = p->e_end;
+    hipStream_t st = ctx->stream;
+    int rc = RTG_OK;
+    HIP_TRY(hipEventRecord(ctx->ev[5], st));
+    do {
+        if ((rc = dev_alloc(&p->d_row, (size_t)g->m)) != RTG_OK) break;
+        if ((rc = dev_alloc(&p->d_count, 4)) != RTG_OK) break;
+        if (g->n > 0) {
+            edge_rows_kernel<<<grid_cap(ctx, g->n, 256, 16), 256, 0, st>>>(g->d_offsets, g->n, p->d_row);
+            if (hipGetLastError() != hipSuccess) { rc = set_error(RTG_EHIP, "edge_rows launch failed"); break; }
+        }
+        if (opts->method == RTG_TC_BATCHED) {
+            BatchTune tune;
+            if ((rc = plan_batched(ctx, p, tune)) != RTG_OK) break;
+        } else if (opts->method == RTG_TC_PLAIN_THREAD || opts->method == RTG_TC_PLAIN_WAVE) {
+            // byte model only
+            BatchTune tune;
+            (void)tune;
+        } else {
+            rc = set_error(RTG_EUNSUPPORTED, "rtg_tc_plan_create: method not plannable");
+            break;
+        }
+        if (opts->use_hip_graph) {
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) { rc = set_error(RTG_EHIP, "capture begin failed"); break; }
+            int lrc = launch_count(ctx, p);
+            hipError_t e = hipStreamEndCapture(st, &p->graph);
+            if (lrc != RTG_OK) { rc = lrc; break; }
+            if (e != hipSuccess) { rc = set_error(RTG_EHIP, "capture end failed"); break; }
+            if (hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess) { rc = set_error(RTG_EHIP, "graph instantiate failed"); break; }
+        }
+    } while (0);
+    if (rc == RTG_OK) {
+        if (hipEventRecord(ctx->ev[6], st) != hipSuccess || hipEventSynchronize(ctx->ev[6]) != hipSuccess)
+            rc = set_error(RTG_EHIP, "plan sync failed");
+        else
+            p->t_plan_ms = elapsed_ms(ctx->ev[5], ctx->ev[6]);
+    }
+    if (rc != RTG_OK) {
+        rtg_tc_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return RTG_OK;
+}
+
+static void fill_stats(rtg_tc_plan *p, rtg_tc_stats *s) {
+    s->t_plan_ms = p->t_plan_ms;
+    s->alg_bytes = p->alg_bytes;
+    s->merge_bytes = p->merge_bytes;
+    s->edges = p->e_end - p->e_begin;
+    if (p->opts.method == RTG_TC_BATCHED) {
+        s->nbuckets = 2;
+        s->strategy[0] = 0;  // block items: block-shared LDS hash
+        s->strategy[1] = 1;  // wave items: wave-private LDS hash
+        s->bucket_items[0] = p->n_items;
+        s->bucket_items[1] = p->n_small;
+    }
+    s->kernel_launches = 2;
+}
+
+extern "C" {
+
+RTG_API int rtg_tc_plan_create(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *opts, rtg_tc_plan **out,
+                               rtg_tc_stats *stats) {
+    RTG_TRY(rtg::ctx_enter(ctx));
+    if (!g || !opts || !out) return rtg_fail(RTG_EINVAL, "rtg_tc_plan_create: null argument");
+    if (!g->oriented) return rtg_fail(RTG_EINVAL, "rtg_tc_plan_create: graph is not degree-oriented");
+    *out = nullptr;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    RTG_TRY(rtg_tc_plan_create_impl(ctx, g, opts, out));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        fill_stats(*out, stats);
+    }
+    return RTG_OK;
+}
+
+RTG_API int rtg_tc_plan_run(rtg_ctx *ctx, rtg_tc_plan *p, uint64_t *count, rtg_tc_stats *stats) {
+    RTG_TRY(rtg::ctx_enter(ctx));
+    if (!p || !count) return rtg_fail(RTG_EINVAL, "rtg_tc_plan_run: null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipEventRecord(ctx->ev[0], st));
+    if (p->exec) HIP_TRY(hipGraphLaunch(p->exec, st));
+    else RTG_TRY(launch_count(ctx, p));
+    HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpyAsync(&c, p->d_count, sizeof(c), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *count = c;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        fill_stats(p, stats);
+        stats->t_count_ms = rtg::elapsed_ms(ctx->ev[0], ctx->ev[1]);
+    }
+    return RTG_OK;
+}
+
+RTG_API int rtg_tc_plan_destroy(rtg_tc_plan *p) {
+    if (!p) return RTG_OK;
+    if (p->ctx) (void)hipSetDevice(p->ctx->device);
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    void *bufs[] = {p->d_row, p->d_wofs, p->d_wpart, p->d_small, p->d_items, p->d_count};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete p;
+    return RTG_OK;
+}
+
+RTG_API int rtg_tc_per_edge(rtg_ctx *ctx, const rtg_graph *g, int32_t form, uint32_t *d_out) {
+    RTG_TRY(rtg::ctx_enter(ctx));
+    if (!g || (!d_out && g->m > 0)) return rtg_fail(RTG_EINVAL, "rtg_tc_per_edge: null argument");
+    if (!g->oriented) return rtg_fail(RTG_EINVAL, "rtg_tc_per_edge: graph is not degree-oriented");
+    if (form != 0 && form != 1) return rtg_fail(RTG_EINVAL, "rtg_tc_per_edge: form must be 0 or 1");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (g->m == 0) return RTG_OK;
+    hipStream_t st = ctx->stream;
+    int32_t *row = nullptr;
+    RTG_TRY(dev_alloc(&row, (size_t)g->m));
+    DevFree fr;
+    fr.p = row;
+    edge_rows_kernel<<<grid_cap(ctx, g->n, 256, 16), 256, 0, st>>>(g->d_offsets, g->n, row);
+    if (form == 0)
+        tc_per_edge_intersect<<<grid_cap(ctx, g->m * 64, kPlainThreads, 32), kPlainThreads, 0, st>>>(
+            g->d_offsets, g->d_adjs, row, g->m, d_out);
+    else
+        tc_per_edge_1a2<<<grid_cap(ctx, g->m, kPlainThreads, 32), kPlainThreads, 0, st>>>(
+            g->d_offsets, g->d_adjs, row, g->m, d_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    return RTG_OK;
+}
+
+RTG_API int rtg_tc_count(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *opts, uint64_t *count,
+                         rtg_tc_stats *stats) {
+    rtg_tc_opts def{};
+    def.method = RTG_TC_BATCHED;
+    if (!opts) opts = &def;
+    if (opts->method == RTG_TC_2A1) {
+        RTG_TRY(rtg::ctx_enter(ctx));
+        if (!g || !count) return rtg_fail(RTG_EINVAL, "rtg_tc_count: null argument");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        return rtg_tc_count_2a1_impl(ctx, g, count, stats);
+    }
+    rtg_tc_plan *p = nullptr;
+    rtg_tc_stats pst{};
+    RTG_TRY(rtg_tc_plan_create(ctx, g, opts, &p, &pst));
+    int rc = rtg_tc_plan_run(ctx, p, count, stats);
+    if (rc == RTG_OK && stats) stats->t_plan_ms = pst.t_plan_ms;
+    rtg_tc_plan_destroy(p);
+    return rc;
+}
+
+}  // extern "C"

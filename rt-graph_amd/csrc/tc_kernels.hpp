@@ -1,0 +1,45 @@
+// Triangle-counting plan state shared by the TC sources.
+#pragma once
+
+#include "graph.hpp"
+
+// A work item of the batched kernel. kind 0: block item — owner vertex `a` whose out-list is
+// staged in the block's LDS hash table, partners wpart[b .. c). kind 1: wave item — small
+// owners small_list[b .. c), one owner at a time per wave, each with a wave-private table.
+struct TcItem {
+    int32_t kind;
+    int32_t a;
+    int32_t b;
+    int32_t c;
+};
+
+struct rtg_tc_plan {
+    rtg_ctx *ctx = nullptr;
+    const rtg_graph *g = nullptr;
+    rtg_tc_opts opts{};
+    int64_t e_begin = 0, e_end = 0;
+    // device arrays
+    int32_t *d_row = nullptr;        // edge -> source row (all methods)
+    int32_t *d_wofs = nullptr;       // batched: partner-list offsets per owner (n+1)
+    int32_t *d_wpart = nullptr;      // batched: partners grouped by owner
+    int32_t *d_small = nullptr;      // batched: small owners, heaviest first
+    TcItem *d_items = nullptr;       // batched: work queue, heaviest first
+    unsigned long long *d_count = nullptr;  // [0] = triangle count, [1] = queue head
+    int64_t n_items = 0;
+    int64_t n_small = 0;
+    int64_t n_work_edges = 0;        // edges with a non-empty shorter list
+    // launch parameters
+    int32_t max_table = 0;           // LDS hash-table slots per block
+    int32_t block_threads = 256;
+    int32_t wave_owner_max = 0;      // owners with d+ <= this run as wave items
+    int32_t grid = 0;
+    // byte model
+    uint64_t alg_bytes = 0, merge_bytes = 0;
+    // captured launch sequence
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double t_plan_ms = 0.0;
+};
+
+// Two-hop (2A1) count, tc_2a1.hip. Caller holds ctx->mu.
+int rtg_tc_count_2a1_impl(rtg_ctx *ctx, const rtg_graph *g, uint64_t *count, rtg_tc_stats *stats);

@@ -1,0 +1,93 @@
+"""GPU parity of sorted-set intersection (pairs and the reference's A x B tile mode) and of the
+sorted-array lookup, against the oracle (exact integers)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_oracle, load_rtg
+from test_oracle import reference_style_dataset
+
+pytestmark = pytest.mark.gpu
+
+rtg = load_rtg()
+orc = load_oracle()
+SI_METHODS = [rtg.SI_PLAIN, rtg.SI_BATCHED, rtg.SI_REFERENCE]
+
+
+def zipf_workload(nlists, seed):
+    lens = rtg.gen_zipf_lengths(nlists, 1.8, 65536, seed=seed)
+    offs, el = rtg.gen_sorted_lists(lens, 1 << 24, seed=seed)
+    pairs = np.arange(nlists, dtype=np.uint32)  # pair k = lists (2k, 2k+1)
+    return offs, el, pairs
+
+
+@pytest.mark.parametrize("method", SI_METHODS)
+def test_si_pairs_zipf(ctx, method):
+    offs, el, pairs = zipf_workload(1 << 15, 7)
+    exp = orc.si_pairs(offs, el, pairs)
+    d_off, d_el, d_p = ctx.to_device(offs), ctx.to_device(el), ctx.to_device(pairs)
+    out, st = rtg.si_pairs(ctx, d_off, d_el, offs.size - 1, d_p, pairs.size // 2, method)
+    assert np.array_equal(out.numpy(), exp)
+
+
+@pytest.mark.parametrize("method", SI_METHODS)
+def test_si_pairs_edge_cases(ctx, method):
+    rng = np.random.default_rng(3)
+    big = np.sort(rng.choice(1 << 24, size=65536, replace=False)).astype(np.uint32)
+    lists = [np.zeros(0, np.uint32), big, big[::3].copy(), big[:1].copy(), np.array([big[-1]], np.uint32),
+             np.arange(100, dtype=np.uint32), np.arange(50, 150, dtype=np.uint32), big[1000:9000].copy()]
+    offs = np.zeros(len(lists) + 1, np.uint32)
+    np.cumsum([x.size for x in lists], out=offs[1:])
+    el = np.concatenate(lists).astype(np.uint32)
+    pairs = np.array([(a, b) for a in range(len(lists)) for b in range(len(lists))], np.uint32).reshape(-1)
+    exp = orc.si_pairs(offs, el, pairs)
+    out, _ = rtg.si_pairs(ctx, ctx.to_device(offs), ctx.to_device(el), len(lists), ctx.to_device(pairs),
+                          pairs.size // 2, method)
+    assert np.array_equal(out.numpy(), exp)
+
+
+@pytest.mark.parametrize("method", SI_METHODS)
+@pytest.mark.parametrize("len_c", [0, 5, 300])
+def test_si_tile_reference_invariant(ctx, method, len_c):
+    rng = np.random.default_rng(len_c + 1)
+    offs, el = reference_style_dataset(4, 6, 2000, 400, len_c, rng)
+    out, _ = rtg.si_tile(ctx, ctx.to_device(offs), ctx.to_device(el), 4, 6, method)
+    assert np.all(out.numpy() == len_c)
+
+
+def test_binary_search_based_intersection_dataset(tmp_path):
+    rng = np.random.default_rng(9)
+    offs, el = reference_style_dataset(2, 3, 500, 700, 120, rng)
+    ds = rtg.Dataset(2, 3, int(el.max()), 120, offs, el)
+    path = tmp_path / "x.s_data"
+    rtg.write_dataset(path, ds)
+    ds2 = rtg.read_dataset(path)
+    res = rtg.BinarySearchBasedIntersection(ds2)
+    assert res.tolist() == [120] * 6
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 1 << 16, (1 << 20) + 3])
+def test_lookup_u32(ctx, n):
+    rng = np.random.default_rng(n)
+    data = np.sort(rng.integers(0, 1 << 32, size=n, dtype=np.uint64)).astype(np.uint32)
+    if n > 10:
+        data[n // 2: n // 2 + 5] = data[n // 2]  # duplicates
+    keys = np.concatenate([rng.integers(0, 1 << 32, size=5000, dtype=np.uint64).astype(np.uint32),
+                           data[rng.integers(0, max(n, 1), size=5000)] if n else np.zeros(0, np.uint32),
+                           np.array([0, 0xFFFFFFFF], np.uint32)])
+    exp = orc.lookup(data, keys)
+    d_data = ctx.to_device(data if n else np.zeros(1, np.uint32))
+    out, _ = rtg.lookup(ctx, d_data, n, ctx.to_device(keys), keys.size)
+    assert np.array_equal(out.numpy(), exp)
+
+
+def test_lookup_signed_reference_interface():
+    rng = np.random.default_rng(2)
+    data = rng.integers(-1000, 1000, size=20000).astype(np.int32)
+    q = rng.integers(-1100, 1100, size=30000).astype(np.int32)
+    ref_sorted = np.sort(data)
+    exp = orc.lookup(ref_sorted, q, signed=True)
+    got = rtg.BSAccessCountFunction(data, q)
+    assert np.array_equal(data, ref_sorted)  # sorted in place, as the reference does
+    assert np.array_equal(got, exp)

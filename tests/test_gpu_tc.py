@@ -1,0 +1,201 @@
+"""GPU parity of the triangle-counting path: the device CSR build must be byte-identical to
+the oracle's restatement of the reference preprocessing, and every counting method must
+return the oracle's exact count (integer work: bit-exact, no tolerance)."""
+from __future__ import annotations
+
+import hashlib
+import itertools
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import load_oracle, load_rtg
+
+pytestmark = pytest.mark.gpu
+
+rtg = load_rtg()
+orc = load_oracle()
+GOLD = json.loads((Path(__file__).parent / "golden" / "golden.json").read_text())
+METHODS = [rtg.TC_PLAIN_THREAD, rtg.TC_PLAIN_WAVE, rtg.TC_BATCHED]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _graphs():
+    rng = np.random.default_rng(5)
+    g = {}
+    g["hand"] = np.array([1, 2, 2, 3, 3, 1, 3, 4, 4, 5, 7, 3, 2, 1, 3, 3, 1, 2], dtype=np.int32)
+    g["k40"] = np.array(list(itertools.combinations(range(40), 2)), dtype=np.int32).reshape(-1)
+    g["wheel"] = np.array([(0, i) for i in range(1, 301)] + [(i, i % 300 + 1) for i in range(1, 301)],
+                          dtype=np.int32).reshape(-1)
+    e = rng.integers(0, 3000, size=(60000, 2)).astype(np.int32)  # duplicates and self-loops
+    g["gnm_dups"] = e.reshape(-1)
+    g["ba_c1"] = rtg.gen_ba(100_000, 5, 1)
+    g["rmat_s12"] = rtg.gen_rmat(12, 16 << 12, seed=1)
+    g["rmat_s16"] = rtg.gen_rmat(16, 44 << 16, seed=1)
+    deg = rtg.gen_powerlaw_degrees(20_000, 2.1, 2_000.0, 20.0, seed=1)
+    g["plcm"] = rtg.gen_config_model(deg, 0.3, seed=1)
+    # K_1200 (out-degrees up to 1199) plus a vertex x whose 2500 neighbours A_i each have 2501
+    # leaves, so x keeps out-degree 2500 after the leaves are removed; A_i - A_i+1 closes
+    # 2499 triangles through x.
+    k = 1200
+    clique = np.array(list(itertools.combinations(range(k), 2)), dtype=np.int32)
+    x, a0, na, leaves = k, k + 1, 2500, 2501
+    A = np.arange(a0, a0 + na, dtype=np.int32)
+    star = np.stack([np.full(na, x, np.int32), A], 1)
+    path = np.stack([A[:-1], A[1:]], 1)
+    lf = np.stack([np.repeat(A, leaves), a0 + na + np.arange(na * leaves, dtype=np.int32)], 1)
+    g["hub"] = np.concatenate([clique, star, path, lf]).astype(np.int32).reshape(-1)
+    return g
+
+
+GRAPHS = _graphs()
+_ORACLE = {}
+
+
+def oracle_csr(name):
+    if name not in _ORACLE:
+        off, adj = orc.csr_from_pairs(GRAPHS[name])
+        _ORACLE[name] = (off, adj, orc.tc_count_intersect(off, adj))
+    return _ORACLE[name]
+
+
+@pytest.mark.parametrize("name", list(GRAPHS))
+def test_csr_build_byte_identical(ctx, name):
+    print(f"[csr] {name}", flush=True)
+    off, adj, _ = oracle_csr(name)
+    g = rtg.DeviceGraph.from_pairs(ctx, GRAPHS[name])
+    doff, dadj = g.download()
+    assert doff.dtype == np.int32 and dadj.dtype == np.int32
+    assert np.array_equal(doff, off)
+    assert np.array_equal(dadj, adj)
+    assert g.max_row == (int(np.diff(off).max()) if off.size > 1 else 0)
+
+
+@pytest.mark.parametrize("name", list(GRAPHS))
+@pytest.mark.parametrize("method", METHODS)
+def test_count_matches_oracle(ctx, name, method):
+    print(f"[count] {name} method={method}", flush=True)
+    off, adj, t = oracle_csr(name)
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    cnt, st = g.tc_count(method=method)
+    assert cnt == t
+    assert st["edges"] == adj.size
+
+
+def test_golden_counts_c1(ctx):
+    e = GOLD["ba_n100000_m5_seed1"]
+    g = rtg.DeviceGraph.from_pairs(ctx, rtg.gen_ba(100_000, 5, 1))
+    off, adj = g.download()
+    assert sha(off.astype("<i4")) == e["offsets_sha256"] and sha(adj.astype("<i4")) == e["adjs_sha256"]
+    for m in METHODS + [rtg.TC_2A1]:
+        assert g.tc_count(method=m)[0] == e["triangles"]
+
+
+def test_two_hop_2a1(ctx):
+    for name in ("hand", "k40", "rmat_s12", "plcm", "gnm_dups"):
+        off, adj, t = oracle_csr(name)
+        g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+        assert g.tc_count(method=rtg.TC_2A1)[0] == t, name
+
+
+@pytest.mark.parametrize("name", ["rmat_s16", "plcm", "hub"])
+def test_per_edge_counts(ctx, name):
+    off, adj, _ = oracle_csr(name)
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    for form in (0, 1):
+        assert np.array_equal(g.tc_per_edge(form), orc.tc_per_edge(off, adj, form))
+
+
+def test_edge_ranges_partition(ctx):
+    off, adj, t = oracle_csr("rmat_s16")
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    m = adj.size
+    cuts = [0, m // 5, m // 2, m - 7, m]
+    for method in METHODS:
+        parts = [g.tc_count(method=method, e_begin=a, e_end=b)[0] for a, b in zip(cuts, cuts[1:])]
+        assert sum(parts) == t
+        assert parts[1] == orc.tc_count_1a2(off, adj, cuts[1], cuts[2])
+
+
+def test_hip_graph_replays(ctx):
+    off, adj, t = oracle_csr("plcm")
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=True)
+    for i in range(4):
+        cnt, st = plan.run()
+        assert cnt == t, i
+        # eager work between replays
+        assert g.tc_count(method=rtg.TC_PLAIN_WAVE)[0] == t
+    plan.close()
+
+
+def test_symmetric_csr(ctx):
+    p = GRAPHS["gnm_dups"]
+    rp, nodes = orc.relabel(p)
+    off, adj = orc.symmetric_csr(rp, nodes)
+    g = rtg.DeviceGraph.from_pairs(ctx, p, relabel=True, orient=False)
+    doff, dadj = g.download()
+    assert np.array_equal(doff, off) and np.array_equal(dadj, adj)
+
+
+def test_device_pairs_input_and_generator(ctx):
+    host = rtg.gen_rmat(14, 20 << 14, seed=3)
+    dev = rtg.gen_rmat_device(ctx, 14, 20 << 14, seed=3)
+    assert np.array_equal(dev.numpy(), host)
+    g1 = rtg.DeviceGraph.from_pairs(ctx, dev)
+    g2 = rtg.DeviceGraph.from_pairs(ctx, host)
+    a, b = g1.download(), g2.download()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_invalid_csr_rejected(ctx):
+    with pytest.raises(rtg.RTGError):
+        rtg.DeviceGraph.from_csr(ctx, np.array([0, 2, 2], np.int32), np.array([1, 1], np.int32))  # duplicate
+    with pytest.raises(rtg.RTGError):
+        rtg.DeviceGraph.from_csr(ctx, np.array([0, 1, 1], np.int32), np.array([5], np.int32))  # out of range
+    with pytest.raises(rtg.RTGError):
+        rtg.DeviceGraph.from_pairs(ctx, np.array([0, -3], np.int32))
+
+
+def test_empty_graph(ctx):
+    g = rtg.DeviceGraph.from_pairs(ctx, np.zeros(0, np.int32))
+    assert g.n == 0 and g.m == 0
+    for m in METHODS + [rtg.TC_2A1]:
+        assert g.tc_count(method=m)[0] == 0
+
+
+def test_reference_interface(capsys):
+    off, adj, t = oracle_csr("rmat_s12")
+    cnt = rtg.CountOnGPU(adj, off, off.size - 1, adj.size, 0)
+    out = capsys.readouterr().out
+    assert cnt == t
+    assert f"counting result = {t & 0xFFFFFFFF}" in out and "BSTC: counting time = " in out
+    G = rtg.Graph(0)
+    G.SetEdges(GRAPHS["rmat_s12"])
+    G.TCPreprocessing()
+    G.ConvertToCSR()
+    assert np.array_equal(G.offsets_, off) and np.array_equal(G.adjs_, adj)
+    assert (G.graph_info_.nodes, G.graph_info_.edges) == (off.size - 1, adj.size)
+
+
+def test_full_rmat_s21_ef44(ctx):
+    """The full C4 workload against its golden count, CSR hashes and byte model."""
+    e = GOLD["rmat_s21_ef44_seed1"]
+    print("[full] generating R-MAT s21 ef44 on the device", flush=True)
+    dev = rtg.gen_rmat_device(ctx, 21, 44 << 21, seed=1)
+    g = rtg.DeviceGraph.from_pairs(ctx, dev)
+    dev.free()
+    assert (g.n, g.m, g.max_row) == (e["n"], e["m"], e["max_out_degree"])
+    off, adj = g.download()
+    assert sha(off.astype("<i4")) == e["offsets_sha256"] and sha(adj.astype("<i4")) == e["adjs_sha256"]
+    print("[full] batched count", flush=True)
+    cnt, st = g.tc_count(method=rtg.TC_BATCHED)
+    assert cnt == e["triangles"]
+    assert st["alg_bytes"] == e["alg_bytes"]
+    print("[full] plain wave count", flush=True)
+    assert g.tc_count(method=rtg.TC_PLAIN_WAVE)[0] == e["triangles"]
