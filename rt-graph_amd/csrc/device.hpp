@@ -118,6 +118,18 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
     return v;
 }
 
+// Wave-uniform copies (scalar registers) of a value every lane holds identically. Work-queue
+// loops take their exit decision on these, so the compiler sees a uniform loop: a loop whose
+// exit it believes divergent can be restructured so that a lane-0-only queue grab at the top
+// lands outside the loop the other lanes keep iterating, which then never sees a new item.
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = __lane_id();
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));

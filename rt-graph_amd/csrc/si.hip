@@ -205,29 +205,33 @@ __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, Buckets
     __shared__ uint32_t samp[kSamples];
     __shared__ uint32_t red[kThreads / 64];
     __shared__ unsigned long long slot;
-    // Block bucket: block per pair.
-    const unsigned long long n2 = B.count[2];
-    while (true) {
-        if (threadIdx.x == 0) slot = atomicAdd(&B.count[5], 1ull);
-        __syncthreads();
-        const unsigned long long i = slot;
-        __syncthreads();
-        if (i >= n2) break;
-        const uint32_t k = B.list[2][i];
+    // Block bucket: block per pair. Every grab is published through LDS and read back as a
+    // wave-uniform (scalar) value, so the loop below is uniform.
+    const uint64_t n2 = rtgd::uniform_u64(B.count[2]);
+    if (threadIdx.x == 0) slot = atomicAdd(&B.count[5], 1ull);
+    __syncthreads();
+    uint64_t i2 = rtgd::uniform_u64(slot);
+    while (i2 < n2) {
+        const uint32_t k = B.list[2][i2];
         uint32_t a, b;
         P.get(k, a, b);
-        const uint32_t c = block_pair(L, a, b, samp, red);
-        if (threadIdx.x == 0) out[k] = c;
+        const uint32_t c = block_pair(L, a, b, samp, red);  // ends with a barrier
+        if (threadIdx.x == 0) {
+            out[k] = c;
+            slot = atomicAdd(&B.count[5], 1ull);
+        }
+        __syncthreads();
+        i2 = rtgd::uniform_u64(slot);
     }
     // Wave bucket: wave per pair, 4 pairs per grab.
-    const unsigned long long n1 = B.count[1];
+    const uint64_t n1 = rtgd::uniform_u64(B.count[1]);
     while (true) {
-        unsigned long long i = 0;
-        if (__lane_id() == 0) i = atomicAdd(&B.count[4], 4ull);
-        i = __shfl(i, 0, 64);
+        uint64_t g = 0;
+        if (__lane_id() == 0) g = atomicAdd(&B.count[4], 4ull);
+        const uint64_t i = rtgd::uniform_u64(g);
         if (i >= n1) break;
-        const unsigned long long e = (i + 4 < n1) ? i + 4 : n1;
-        for (unsigned long long j = i; j < e; ++j) {
+        const uint64_t e = (i + 4 < n1) ? i + 4 : n1;
+        for (uint64_t j = i; j < e; ++j) {
             const uint32_t k = B.list[1][j];
             uint32_t a, b;
             P.get(k, a, b);
@@ -236,12 +240,13 @@ __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, Buckets
         }
     }
     // Thread bucket: thread per pair, a wave's worth per grab.
-    const unsigned long long n0 = B.count[0];
+    const uint64_t n0 = rtgd::uniform_u64(B.count[0]);
     while (true) {
-        unsigned long long i = 0;
-        if (__lane_id() == 0) i = atomicAdd(&B.count[3], 64ull);
-        i = __shfl(i, 0, 64) + __lane_id();
-        if (__shfl(i, 0, 64) >= n0) break;
+        uint64_t g = 0;
+        if (__lane_id() == 0) g = atomicAdd(&B.count[3], 64ull);
+        const uint64_t base = rtgd::uniform_u64(g);
+        if (base >= n0) break;
+        const uint64_t i = base + __lane_id();
         if (i < n0) {
             const uint32_t k = B.list[0][i];
             uint32_t a, b;

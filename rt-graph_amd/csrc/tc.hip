@@ -237,22 +237,24 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
     extern __shared__ uint32_t table[];
     __shared__ int32_t ex_all[kWaves][65];
     __shared__ int32_t st_all[kWaves][64];
-    __shared__ int64_t item_slot;
+    __shared__ uint64_t item_slot;
     __shared__ int32_t next_batch;
 
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    const int lane = __lane_id();
+    const int wave = (int)rtgd::uniform_u32(threadIdx.x >> 6);
     int32_t *ex = ex_all[wave];
     int32_t *st = st_all[wave];
     uint64_t hits = 0;
 
-    while (true) {
-        if (threadIdx.x == 0) {
-            item_slot = (int64_t)atomicAdd(&a.count[1], 1ull);
-            next_batch = 0;
-        }
-        __syncthreads();
-        const int64_t it_idx = item_slot;
-        if (it_idx >= a.n_items) break;
+    // Queue grabs are published through LDS and read back as wave-uniform (scalar) values so
+    // that every loop below is uniform.
+    if (threadIdx.x == 0) {
+        item_slot = atomicAdd(&a.count[1], 1ull);
+        next_batch = 0;
+    }
+    __syncthreads();
+    uint64_t it_idx = rtgd::uniform_u64(item_slot);
+    while (it_idx < (uint64_t)a.n_items) {
         const TcItem it = a.items[it_idx];
         if (it.kind == 0) {
             // Block item: owner it.a, partners [it.b, it.c).
@@ -265,9 +267,9 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
             for (int32_t i = threadIdx.x; i < d; i += kThreads) table_insert(table, log_t, (uint32_t)a.adj[xb + i]);
             __syncthreads();
             while (true) {
-                int32_t b = 0;
-                if (lane == 0) b = atomicAdd(&next_batch, 64);
-                b = __shfl(b, 0, 64);
+                int32_t g = 0;
+                if (lane == 0) g = atomicAdd(&next_batch, 64);
+                const int32_t b = (int32_t)rtgd::uniform_u32((uint32_t)g);
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + 64);
@@ -289,7 +291,13 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 wave_sync_lds();
             }
         }
+        __syncthreads();  // every wave is done with the table, item_slot and next_batch
+        if (threadIdx.x == 0) {
+            item_slot = atomicAdd(&a.count[1], 1ull);
+            next_batch = 0;
+        }
         __syncthreads();
+        it_idx = rtgd::uniform_u64(item_slot);
     }
     hits = rtgd::wave_sum_u64(hits);
     if (lane == 0 && hits) atomicAdd(&a.count[0], (unsigned long long)hits);
