@@ -165,13 +165,20 @@ ORC_API int64_t orc_symmetric_csr(const int32_t *pairs, int64_t npairs, int32_t 
 }
 
 namespace {
-// Row of every edge index (the reference finds it by binary search over offsets per edge,
-// bs_tc.cu:126-134; the result is the same row).
-std::vector<int32_t> edge_rows(const int32_t *offsets, int32_t n) {
-    std::vector<int32_t> row((size_t)offsets[n]);
+// Row of every edge index in [e_begin, e_end), stored at row[e - e_begin] (the reference finds
+// it by binary search over offsets per edge, bs_tc.cu:126-134; the result is the same row).
+std::vector<int32_t> edge_rows(const int32_t *offsets, int32_t n, int64_t e_begin = 0, int64_t e_end = -1) {
+    if (e_end < 0) e_end = offsets[n];
+    std::vector<int32_t> row((size_t)std::max<int64_t>(e_end - e_begin, 0));
+    if (e_end <= e_begin) return row;
+    // first row whose range contains e_begin / last containing e_end - 1
+    const int32_t u0 = (int32_t)(std::upper_bound(offsets, offsets + n + 1, (int32_t)e_begin) - offsets) - 1;
+    const int32_t u1 = (int32_t)(std::upper_bound(offsets, offsets + n + 1, (int32_t)(e_end - 1)) - offsets) - 1;
 #pragma omp parallel for schedule(dynamic, 1024)
-    for (int32_t u = 0; u < n; ++u)
-        for (int64_t e = offsets[u]; e < offsets[u + 1]; ++e) row[e] = u;
+    for (int32_t u = u0; u <= u1; ++u) {
+        const int64_t b = std::max<int64_t>(offsets[u], e_begin), e = std::min<int64_t>(offsets[u + 1], e_end);
+        for (int64_t k = b; k < e; ++k) row[k - e_begin] = u;
+    }
     return row;
 }
 }  // namespace
@@ -185,11 +192,13 @@ ORC_API uint64_t orc_tc_count_1a2(const int32_t *offsets, const int32_t *adjs, i
     set_threads(threads);
     const int64_t m = offsets[n];
     if (e_end <= 0 || e_end > m) e_end = m;
-    std::vector<int32_t> row = edge_rows(offsets, n);
+    if (e_begin < 0) e_begin = 0;
+    if (e_begin >= e_end) return 0;
+    std::vector<int32_t> row = edge_rows(offsets, n, e_begin, e_end);
     uint64_t total = 0;
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : total)
     for (int64_t e = e_begin; e < e_end; ++e) {
-        const int32_t u = row[e], v = adjs[e];
+        const int32_t u = row[e - e_begin], v = adjs[e];
         uint64_t cnt = 0;
         for (int64_t i = offsets[u]; i < offsets[u + 1]; ++i) {
             const int32_t w = adjs[i];
@@ -209,11 +218,13 @@ ORC_API uint64_t orc_tc_count_intersect(const int32_t *offsets, const int32_t *a
     set_threads(threads);
     const int64_t m = offsets[n];
     if (e_end <= 0 || e_end > m) e_end = m;
-    std::vector<int32_t> row = edge_rows(offsets, n);
+    if (e_begin < 0) e_begin = 0;
+    if (e_begin >= e_end) return 0;
+    std::vector<int32_t> row = edge_rows(offsets, n, e_begin, e_end);
     uint64_t total = 0;
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : total)
     for (int64_t e = e_begin; e < e_end; ++e) {
-        const int32_t u = row[e], v = adjs[e];
+        const int32_t u = row[e - e_begin], v = adjs[e];
         total += merge_count(adjs + offsets[u], offsets[u + 1] - offsets[u], adjs + offsets[v],
                              offsets[v + 1] - offsets[v]);
     }

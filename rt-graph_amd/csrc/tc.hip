@@ -132,35 +132,77 @@ __global__ __launch_bounds__(kPlainThreads) void tc_per_edge_1a2(
 
 // ---------------------------------------------------------------------------------------------
 // Batched kernel.
+//
+// The owner's out-list goes into a two-choice cuckoo table in LDS: 2*T slots, x stored at
+// slot h1(x) of the first half or h2(x) of the second, so that a lookup is exactly two LDS
+// reads with no probe loop (a linear-probing loop runs until the longest chain among the 64
+// lanes ends, which dominated the instruction count). T is about 1.25 * d, so the load is at
+// most 0.4; the parallel insertion (atomic exchanges, bounded eviction chains) is retried with
+// new hash multipliers on failure and, after kRetries failures, the item falls back to linear
+// probing over the same slots.
 
-__device__ __forceinline__ uint32_t hslot(uint32_t x, int log_t) { return (x * 0x9E3779B1u) >> (32 - log_t); }
+constexpr int kRetries = 4;
+constexpr int kEvict = 64;
 
-__device__ __forceinline__ void table_insert(uint32_t *tab, int log_t, uint32_t x) {
-    const uint32_t mask = (1u << log_t) - 1u;
-    uint32_t h = hslot(x, log_t);
+// Slots per half for an out-list of length d.
+__host__ __device__ __forceinline__ uint32_t cuckoo_half(int32_t d) {
+    uint32_t t = (uint32_t)d + ((uint32_t)d >> 2) + 1u;
+    return (t + 31u) & ~31u;
+}
+
+__device__ __forceinline__ uint32_t hmul(uint32_t x, uint32_t k, uint32_t t) { return __umulhi(x * k, t); }
+
+struct Table {
+    uint32_t *tab;
+    uint32_t t;      // slots per half (cuckoo) / half the slots (linear)
+    uint32_t k1, k2;
+    uint32_t linear; // 0: cuckoo, 1: linear probing over 2t slots
+};
+
+__device__ __forceinline__ bool cuckoo_insert(const Table &c, uint32_t x) {
+    uint32_t cur = x;
+    for (int it = 0; it < kEvict; ++it) {
+        uint32_t old = atomicExch(&c.tab[hmul(cur, c.k1, c.t)], cur);
+        if (old == kEmpty) return true;
+        cur = old;
+        old = atomicExch(&c.tab[c.t + hmul(cur, c.k2, c.t)], cur);
+        if (old == kEmpty) return true;
+        cur = old;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void linear_insert(const Table &c, uint32_t x) {
+    const uint32_t size = 2u * c.t;
+    uint32_t h = hmul(x, c.k1, size);
     while (true) {
-        const uint32_t old = atomicCAS(&tab[h], kEmpty, x);
+        const uint32_t old = atomicCAS(&c.tab[h], kEmpty, x);
         if (old == kEmpty) return;
-        h = (h + 1u) & mask;
+        h = (h + 1u == size) ? 0u : h + 1u;
     }
 }
 
-__device__ __forceinline__ uint32_t table_has(const uint32_t *tab, int log_t, uint32_t x) {
-    const uint32_t mask = (1u << log_t) - 1u;
-    uint32_t h = hslot(x, log_t);
+__device__ __forceinline__ uint32_t cuckoo_has(const Table &c, uint32_t x) {
+    const uint32_t a = c.tab[hmul(x, c.k1, c.t)];
+    const uint32_t b = c.tab[c.t + hmul(x, c.k2, c.t)];
+    return (uint32_t)(a == x) | (uint32_t)(b == x);
+}
+
+__device__ __forceinline__ uint32_t linear_has(const Table &c, uint32_t x) {
+    const uint32_t size = 2u * c.t;
+    uint32_t h = hmul(x, c.k1, size);
     while (true) {
-        const uint32_t t = tab[h];
+        const uint32_t t = c.tab[h];
         if (t == x) return 1u;
         if (t == kEmpty) return 0u;
-        h = (h + 1u) & mask;
+        h = (h + 1u == size) ? 0u : h + 1u;
     }
 }
 
-__device__ __forceinline__ int table_log2(int32_t d, int32_t max_slots) {
-    // smallest power of two >= 2d, at least 64 slots
-    int lg = 6;
-    while ((1 << lg) < 2 * d && (1 << lg) < max_slots) ++lg;
-    return lg;
+__device__ __forceinline__ void table_seed(Table &c, int r) {
+    c.k1 = 0x9E3779B1u + 0x7F4A7C16u * (uint32_t)r;
+    c.k2 = (0x85EBCA77u ^ (0xC2B2AE3Du * (uint32_t)(r + 1))) | 1u;
+    c.k1 |= 1u;
 }
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -169,51 +211,127 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Streams the out-lists of partners wpart[p0 .. p1) through the 64 lanes of one wave, probing
-// every element against `tab`. Partners are taken 64 at a time; their lengths are scanned
-// into the wave's LDS slots and each lane walks a cursor over them, so that lane j of a round
-// handles element j of the concatenated lists.
-__device__ __forceinline__ uint32_t stream_partners(const int32_t *__restrict__ off,
-                                                    const int32_t *__restrict__ adj,
-                                                    const int32_t *__restrict__ wpart, int64_t p0,
-                                                    int64_t p1, const uint32_t *tab, int log_t,
-                                                    int32_t *ex, int32_t *st) {
+// Block-wide build of the owner table from adj[xb .. xb+d). Returns the table (uniform).
+template <int kThreads>
+__device__ Table build_block_table(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d,
+                                   uint32_t *fail_flag) {
+    Table c{tab, cuckoo_half(d), 0, 0, 0};
+    for (int r = 0; r < kRetries; ++r) {
+        table_seed(c, r);
+        for (uint32_t i = threadIdx.x; i < 2u * c.t; i += kThreads) tab[i] = kEmpty;
+        if (threadIdx.x == 0) *fail_flag = 0u;
+        __syncthreads();
+        bool ok = true;
+        for (int32_t i = threadIdx.x; i < d; i += kThreads) ok &= cuckoo_insert(c, (uint32_t)adj[xb + i]);
+        if (!ok) *fail_flag = 1u;
+        __syncthreads();
+        if (rtgd::uniform_u32(*fail_flag) == 0u) return c;
+        __syncthreads();  // everyone has read the flag before it is reset
+    }
+    c.linear = 1u;
+    for (uint32_t i = threadIdx.x; i < 2u * c.t; i += kThreads) tab[i] = kEmpty;
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < d; i += kThreads) linear_insert(c, (uint32_t)adj[xb + i]);
+    __syncthreads();
+    return c;
+}
+
+// Wave-private build (small owners).
+__device__ Table build_wave_table(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d) {
     const int lane = __lane_id();
+    Table c{tab, cuckoo_half(d), 0, 0, 0};
+    for (int r = 0; r < kRetries; ++r) {
+        table_seed(c, r);
+        for (uint32_t i = lane; i < 2u * c.t; i += 64) tab[i] = kEmpty;
+        wave_sync_lds();
+        bool ok = true;
+        for (int32_t i = lane; i < d; i += 64) ok &= cuckoo_insert(c, (uint32_t)adj[xb + i]);
+        wave_sync_lds();
+        if (__ballot(!ok) == 0ull) return c;
+    }
+    c.linear = 1u;
+    for (uint32_t i = lane; i < 2u * c.t; i += 64) tab[i] = kEmpty;
+    wave_sync_lds();
+    for (int32_t i = lane; i < d; i += 64) linear_insert(c, (uint32_t)adj[xb + i]);
+    wave_sync_lds();
+    return c;
+}
+
+template <bool kLinear>
+__device__ __forceinline__ uint32_t probe(const Table &c, uint32_t x) {
+    if constexpr (kLinear) return linear_has(c, x);
+    else return cuckoo_has(c, x);
+}
+
+// Streams the out-lists of partner spans wspan[p0 .. p1) ({start, length} in adj) through the
+// 64 lanes of one wave and probes every element against the owner table. Partners come 64 at
+// a time. Lists of at least `long_min` elements are streamed one after another with lane j
+// reading element j (coalesced, no index arithmetic beyond base + j); shorter ones are
+// concatenated: their lengths are scanned into the wave's LDS slots and each lane walks a
+// cursor, so that lane j of a round handles element j of the concatenation.
+template <bool kLinear>
+__device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
+                                                 int64_t p0, int64_t p1, const Table &c, int32_t long_min,
+                                                 int32_t *ex, int32_t *st) {
+    const int lane = __lane_id();
+    const uint64_t lt = rtgd::lanemask_lt();
+    constexpr int U = 4;
     uint32_t hits = 0;
     for (int64_t bs = p0; bs < p1; bs += 64) {
         const int64_t pi = bs + lane;
-        int32_t ys = 0, yl = 0;
-        if (pi < p1) {
-            const int32_t y = wpart[pi];
-            ys = off[y];
-            yl = off[y + 1] - ys;
-        }
-        const int32_t incl = rtgd::wave_inclusive_scan(yl);
-        const int32_t total = __shfl(incl, 63, 64);
-        ex[lane] = incl - yl;
-        st[lane] = ys;
-        if (lane == 63) ex[64] = total;
-        wave_sync_lds();
-        int k = 0;
-        constexpr int U = 4;
-        for (int32_t jb = 0; jb < total; jb += 64 * U) {
-            uint32_t x[U];
-            bool ok[U];
+        int2 sp = make_int2(0, 0);
+        if (pi < p1) sp = wspan[pi];
+        const bool is_long = sp.y >= long_min;
+        uint64_t lm = __ballot(is_long);
+        while (lm) {
+            const int p = __builtin_ctzll(lm);
+            lm &= lm - 1;
+            const int32_t s0 = __builtin_amdgcn_readlane(sp.x, p);
+            const int32_t l0 = __builtin_amdgcn_readlane(sp.y, p);
+            for (int32_t jb = 0; jb < l0; jb += 64 * U) {
+                uint32_t x[U];
 #pragma unroll
-            for (int uu = 0; uu < U; ++uu) {
-                const int32_t j = jb + uu * 64 + lane;
-                ok[uu] = j < total;
-                x[uu] = kEmpty;
-                if (ok[uu]) {
-                    while (ex[k + 1] <= j) ++k;
-                    x[uu] = (uint32_t)adj[st[k] + (j - ex[k])];
+                for (int u = 0; u < U; ++u) {
+                    const int32_t j = jb + u * 64 + lane;
+                    x[u] = j < l0 ? (uint32_t)adj[s0 + j] : kEmpty;
                 }
-            }
 #pragma unroll
-            for (int uu = 0; uu < U; ++uu)
-                if (ok[uu]) hits += table_has(tab, log_t, x[uu]);
+                for (int u = 0; u < U; ++u)
+                    if (x[u] != kEmpty) hits += probe<kLinear>(c, x[u]);
+            }
         }
-        wave_sync_lds();
+        const bool is_short = !is_long && sp.y > 0;
+        const uint64_t sm = __ballot(is_short);
+        if (sm) {
+            const int32_t sl = is_short ? sp.y : 0;
+            const int32_t incl = rtgd::wave_inclusive_scan(sl);
+            const int32_t total = __builtin_amdgcn_readlane(incl, 63);
+            const int ns = __popcll(sm);
+            if (is_short) {
+                const int ci = __popcll(sm & lt);
+                ex[ci] = incl - sl;
+                st[ci] = sp.x;
+            }
+            if (lane == 0) ex[ns] = total;
+            wave_sync_lds();
+            int k = 0;
+            for (int32_t jb = 0; jb < total; jb += 64 * U) {
+                uint32_t x[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int32_t j = jb + u * 64 + lane;
+                    x[u] = kEmpty;
+                    if (j < total) {
+                        while (ex[k + 1] <= j) ++k;
+                        x[u] = (uint32_t)adj[st[k] + (j - ex[k])];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (x[u] != kEmpty) hits += probe<kLinear>(c, x[u]);
+            }
+            wave_sync_lds();
+        }
     }
     return hits;
 }
@@ -222,13 +340,14 @@ struct BatchArgs {
     const int32_t *off;
     const int32_t *adj;
     const int32_t *wofs;
-    const int32_t *wpart;
+    const int2 *wspan;
     const int32_t *small_list;
     const TcItem *items;
     int64_t n_items;
     unsigned long long *count;  // [0] total, [1] queue head
     int32_t max_table;          // block table slots
     int32_t wave_table;         // wave-private table slots
+    int32_t long_min;           // direct-stream threshold
 };
 
 template <int kThreads>
@@ -239,6 +358,7 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
     __shared__ int32_t st_all[kWaves][64];
     __shared__ uint64_t item_slot;
     __shared__ int32_t next_batch;
+    __shared__ uint32_t fail_flag;
 
     const int lane = __lane_id();
     const int wave = (int)rtgd::uniform_u32(threadIdx.x >> 6);
@@ -260,12 +380,7 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
             // Block item: owner it.a, partners [it.b, it.c).
             const int32_t x = it.a;
             const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
-            const int log_t = table_log2(d, a.max_table);
-            const int32_t slots = 1 << log_t;
-            for (int32_t i = threadIdx.x; i < slots; i += kThreads) table[i] = kEmpty;
-            __syncthreads();
-            for (int32_t i = threadIdx.x; i < d; i += kThreads) table_insert(table, log_t, (uint32_t)a.adj[xb + i]);
-            __syncthreads();
+            const Table c = build_block_table<kThreads>(table, a.adj, xb, d, &fail_flag);
             while (true) {
                 int32_t g = 0;
                 if (lane == 0) g = atomicAdd(&next_batch, 64);
@@ -273,7 +388,8 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + 64);
-                hits += stream_partners(a.off, a.adj, a.wpart, p0, p1, table, log_t, ex, st);
+                if (c.linear) hits += stream_spans<true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
+                else hits += stream_spans<false>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
             }
         } else {
             // Wave items: small owners small_list[it.b, it.c), one per wave at a time.
@@ -281,13 +397,9 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
             for (int32_t s = it.b + wave; s < it.c; s += kWaves) {
                 const int32_t x = a.small_list[s];
                 const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
-                const int log_t = table_log2(d, a.wave_table);
-                const int32_t slots = 1 << log_t;
-                for (int32_t i = lane; i < slots; i += 64) tab[i] = kEmpty;
-                wave_sync_lds();
-                for (int32_t i = lane; i < d; i += 64) table_insert(tab, log_t, (uint32_t)a.adj[xb + i]);
-                wave_sync_lds();
-                hits += stream_partners(a.off, a.adj, a.wpart, a.wofs[x], a.wofs[x + 1], tab, log_t, ex, st);
+                const Table c = build_wave_table(tab, a.adj, xb, d);
+                if (c.linear) hits += stream_spans<true>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
+                else hits += stream_spans<false>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
                 wave_sync_lds();
             }
         }
@@ -339,7 +451,7 @@ __global__ void plan_owner_counts(const int32_t *__restrict__ off, const int32_t
 __global__ void plan_scatter(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
                              const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
                              const uint32_t *__restrict__ wofs, uint32_t *__restrict__ cursor,
-                             int32_t *__restrict__ wpart) {
+                             int2 *__restrict__ wspan) {
     for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int32_t u = row[e], v = adj[e];
@@ -348,7 +460,7 @@ __global__ void plan_scatter(const int32_t *__restrict__ off, const int32_t *__r
         const bool uo = (du > dv || (du == dv && u < v));
         const int32_t owner = uo ? u : v, partner = uo ? v : u;
         const uint32_t slot = atomicAdd(&cursor[owner], 1u);
-        wpart[wofs[owner] + slot] = partner;
+        wspan[wofs[owner] + slot] = make_int2(off[partner], uo ? dv : du);
     }
 }
 
@@ -417,11 +529,11 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
     p->alg_bytes = hb[0];
     p->merge_bytes = hb[1];
     p->n_work_edges = (int64_t)hb[2];
-    RTG_TRY(dev_alloc(&p->d_wpart, (size_t)std::max<int64_t>(p->n_work_edges, 1)));
+    RTG_TRY(dev_alloc(&p->d_wspan, (size_t)std::max<int64_t>(p->n_work_edges, 1)));
     HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (size_t)(n + 1), st));
     if (ne > 0) {
         plan_scatter<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end,
-                                                                (const uint32_t *)p->d_wofs, d_cnt, p->d_wpart);
+                                                                (const uint32_t *)p->d_wofs, d_cnt, p->d_wspan);
         HIP_TRY(hipGetLastError());
     }
     // Host side: item list from per-owner partner counts and work.
@@ -435,12 +547,16 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
     const int threads = tune.block_threads;
     const int waves = threads / 64;
     int32_t max_d = g->max_row;
-    // Block table: power of two >= 2 * max out-degree (at least 4 wave tables of 256 slots).
-    int32_t max_table = 1024;
-    while (max_table < 2 * max_d) max_table <<= 1;
-    if (max_table > 32768) return set_error(RTG_EUNSUPPORTED, "batched TC: out-degree above 16384 needs the global-table strategy");
+    // Block table: both cuckoo halves of the largest owner, at least 2048 slots, a multiple of
+    // 64 * waves so that the wave-private tables split it evenly.
+    int32_t max_table = std::max<int32_t>(2048, 2 * (int32_t)cuckoo_half(max_d));
+    max_table = (max_table + 64 * waves - 1) / (64 * waves) * (64 * waves);
+    if ((size_t)max_table * 4 > 150 * 1024)
+        return set_error(RTG_EUNSUPPORTED, "batched TC: out-degree " + std::to_string(max_d) + " needs more LDS than a workgroup has");
     const int32_t wave_table = max_table / waves;
-    int32_t wave_owner_max = tune.wave_owner_max >= 0 ? std::min(tune.wave_owner_max, wave_table / 2) : wave_table / 2;
+    int32_t fit = 0;  // largest out-degree whose table fits a wave's share
+    while (2 * (int32_t)cuckoo_half(fit + 1) <= wave_table) ++fit;
+    int32_t wave_owner_max = tune.wave_owner_max >= 0 ? std::min(tune.wave_owner_max, fit) : fit;
     p->max_table = max_table;
     p->block_threads = threads;
     p->wave_owner_max = wave_owner_max;
@@ -533,8 +649,9 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
             break;
         case RTG_TC_BATCHED: {
             if (p->n_items == 0) break;
-            BatchArgs a{g->d_offsets, g->d_adjs, p->d_wofs, p->d_wpart, p->d_small, p->d_items,
-                        p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64)};
+            BatchArgs a{g->d_offsets, g->d_adjs, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
+                        p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
+                        p->long_min};
             const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
             if (p->block_threads == 256) tc_batched<256><<<p->grid, 256, lds, st>>>(a);
             else tc_batched<512><<<p->grid, 512, lds, st>>>(a);
@@ -660,7 +777,7 @@ RTG_API int rtg_tc_plan_destroy(rtg_tc_plan *p) {
     if (p->ctx) (void)hipSetDevice(p->ctx->device);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    void *bufs[] = {p->d_row, p->d_wofs, p->d_wpart, p->d_small, p->d_items, p->d_count};
+    void *bufs[] = {p->d_row, p->d_wofs, p->d_wspan, p->d_small, p->d_items, p->d_count};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;

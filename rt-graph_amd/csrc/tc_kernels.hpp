@@ -21,7 +21,7 @@ struct rtg_tc_plan {
     // device arrays
     int32_t *d_row = nullptr;        // edge -> source row (all methods)
     int32_t *d_wofs = nullptr;       // batched: partner-list offsets per owner (n+1)
-    int32_t *d_wpart = nullptr;      // batched: partners grouped by owner
+    int2 *d_wspan = nullptr;         // batched: partner out-list spans {start, length} by owner
     int32_t *d_small = nullptr;      // batched: small owners, heaviest first
     TcItem *d_items = nullptr;       // batched: work queue, heaviest first
     unsigned long long *d_count = nullptr;  // [0] = triangle count, [1] = queue head
@@ -32,6 +32,7 @@ struct rtg_tc_plan {
     int32_t max_table = 0;           // LDS hash-table slots per block
     int32_t block_threads = 256;
     int32_t wave_owner_max = 0;      // owners with d+ <= this run as wave items
+    int32_t long_min = 48;           // partner lists at least this long are streamed directly
     int32_t grid = 0;
     // byte model
     uint64_t alg_bytes = 0, merge_bytes = 0;
