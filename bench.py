@@ -65,7 +65,7 @@ def cpu_baseline(off: np.ndarray, adj: np.ndarray, target_s: float) -> dict:
     m = adj.size
     threads = orc.max_threads()
     nr = 64
-    # calibration on a 0.05% sample
+    # calibration on a 0.2% sample
     def run(frac: float):
         per = max(1, int(m * frac / nr))
         t0 = time.perf_counter()
@@ -74,7 +74,7 @@ def cpu_baseline(off: np.ndarray, adj: np.ndarray, target_s: float) -> dict:
             b = (m * r) // nr
             tot += orc.tc_count_1a2(off, adj, b, min(m, b + per), threads)
         return per * nr, time.perf_counter() - t0
-    e0, t0 = run(0.0005)
+    e0, t0 = run(0.002)
     rate = e0 / max(t0, 1e-6)
     frac = min(1.0, max(0.0005, target_s * rate / m))
     edges, secs = run(frac)
@@ -103,7 +103,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="rmat_s21_ef44", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
