@@ -106,10 +106,12 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path shards by "
-              "oriented-edge ranges, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the count shards by "
+              "oriented-edge ranges (rtg_tc_opts.e_begin/e_end) with one 8-byte sum across GPUs",
+              file=sys.stderr)
         return 2
 
     rtg = load_pkg()
@@ -127,8 +129,9 @@ def main() -> int:
     log(f"CSR build: n={g.n} m={g.m} max_out={g.max_row} device {bst.t_total_ms:.2f} ms "
         f"(relabel {bst.t_relabel_ms:.2f}, orient {bst.t_orient_ms:.2f}, sort {bst.t_sort_ms:.2f}, csr {bst.t_csr_ms:.2f})")
 
-    plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=True)
-    log(f"batched plan: {plan.plan_stats['t_plan_ms']:.1f} ms, items={plan.plan_stats['bucket_items']}")
+    plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=True, autotune=not args.no_autotune)
+    desc = plan.describe()
+    log(f"batched plan: {plan.plan_stats['t_plan_ms']:.1f} ms (incl. autotune), items={plan.plan_stats['bucket_items']}, {desc}")
     for _ in range(args.warmup):
         cnt, st = plan.run()
     ctx.synchronize()
@@ -175,7 +178,8 @@ def main() -> int:
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": args.workload, "generator": f"R-MAT scale {scale}, edge factor {ef}, "
                    "(0.57,0.19,0.19,0.05), seed 1", "n": g.n, "m_oriented": g.m, "triangles": cnt,
-                   "method": "batched (owner-grouped LDS hash, persistent queue, hipGraph)"},
+                   "method": "batched: owner-grouped two-choice cuckoo LDS tables, persistent work queue, hipGraph",
+                   "plan": desc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,

@@ -124,8 +124,10 @@ typedef struct rtg_tc_opts {
     int32_t use_hip_graph;   /* batched: replay the captured launch sequence                  */
     int64_t e_begin;         /* oriented-edge range [e_begin, e_end) (e_end <= 0: all)         */
     int64_t e_end;
-    const char *tune_cache;  /* batched: JSON tuning cache path (NULL: built-in defaults)      */
-    int32_t autotune;        /* batched: time candidate strategies when no cache entry exists */
+    const char *tune_cache;  /* batched: JSON tuning cache (key: device + graph fingerprint);
+                                read if it has the key, written after autotuning             */
+    int32_t autotune;        /* batched: time the candidate configurations when the cache has
+                                no entry (RTG_TC_TUNE in the environment overrides both)     */
     int32_t reserved;
 } rtg_tc_opts;
 
@@ -153,6 +155,9 @@ int rtg_tc_plan_create(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *opts
                        rtg_tc_plan **out, rtg_tc_stats *stats);
 int rtg_tc_plan_run(rtg_ctx *ctx, rtg_tc_plan *plan, uint64_t *count, rtg_tc_stats *stats);
 int rtg_tc_plan_destroy(rtg_tc_plan *plan);
+/* The plan's configuration as "key=value,..." text (method, tune, grid, table size, and
+   whether the tune came from the autotuner, the cache, RTG_TC_TUNE or the defaults). */
+int rtg_tc_plan_describe(const rtg_tc_plan *plan, char *buf, size_t len);
 
 /* Per-edge counts |N+(u) & N+(v)| for every oriented edge (triangle a<b<c attributed to edge
    a->b), written to d_out[m]. The 1A2 attribution (edge a->c) is available with form = 1.   */

@@ -265,8 +265,8 @@ class DeviceGraph:
         return cnt.value, st.as_dict()
 
     def tc_plan(self, method: int = TC_BATCHED, e_begin: int = 0, e_end: int = 0,
-                use_hip_graph: bool = True) -> "TcPlan":
-        return TcPlan(self, method, e_begin, e_end, use_hip_graph)
+                use_hip_graph: bool = True, autotune: bool = False, tune_cache: str | None = None) -> "TcPlan":
+        return TcPlan(self, method, e_begin, e_end, use_hip_graph, autotune, tune_cache)
 
     def tc_per_edge(self, form: int = 0) -> np.ndarray:
         out = DeviceArray(self.ctx, (self.m,), np.uint32)
@@ -288,13 +288,25 @@ class DeviceGraph:
 class TcPlan:
     """rtg_tc_plan: buckets, work queue and captured launch sequence, reusable across runs."""
 
-    def __init__(self, g: DeviceGraph, method: int, e_begin: int, e_end: int, use_hip_graph: bool):
+    def __init__(self, g: DeviceGraph, method: int, e_begin: int, e_end: int, use_hip_graph: bool,
+                 autotune: bool = False, tune_cache: str | None = None):
         self.g = g
         self._h = C.c_void_p()
-        opts = TcOpts(method=method, use_hip_graph=int(use_hip_graph), e_begin=e_begin, e_end=e_end)
+        self._cache = None if tune_cache is None else os.fsencode(tune_cache)
+        opts = TcOpts(method=method, use_hip_graph=int(use_hip_graph), e_begin=e_begin, e_end=e_end,
+                      tune_cache=self._cache, autotune=int(autotune))
         st = TcStats()
         _check(_lib.rtg_tc_plan_create(g.ctx.handle, g._h, C.byref(opts), C.byref(self._h), C.byref(st)))
         self.plan_stats = st.as_dict()
+
+    def describe(self) -> dict:
+        buf = C.create_string_buffer(1024)
+        _check(_lib.rtg_tc_plan_describe(self._h, buf, C.c_size_t(1024)))
+        out = {}
+        for kv in buf.value.decode().split(","):
+            k, _, v = kv.partition("=")
+            out[k] = v
+        return out
 
     def run(self) -> tuple[int, dict]:
         cnt = C.c_uint64()

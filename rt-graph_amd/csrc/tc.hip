@@ -20,7 +20,12 @@
 #include "tc_kernels.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iterator>
 #include <cstring>
+#include <string>
 #include <numeric>
 
 namespace {
@@ -269,13 +274,12 @@ __device__ __forceinline__ uint32_t probe(const Table &c, uint32_t x) {
 // reading element j (coalesced, no index arithmetic beyond base + j); shorter ones are
 // concatenated: their lengths are scanned into the wave's LDS slots and each lane walks a
 // cursor, so that lane j of a round handles element j of the concatenation.
-template <bool kLinear>
+template <bool kLinear, int U>
 __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                  int64_t p0, int64_t p1, const Table &c, int32_t long_min,
                                                  int32_t *ex, int32_t *st) {
     const int lane = __lane_id();
     const uint64_t lt = rtgd::lanemask_lt();
-    constexpr int U = 4;
     uint32_t hits = 0;
     for (int64_t bs = p0; bs < p1; bs += 64) {
         const int64_t pi = bs + lane;
@@ -348,9 +352,10 @@ struct BatchArgs {
     int32_t max_table;          // block table slots
     int32_t wave_table;         // wave-private table slots
     int32_t long_min;           // direct-stream threshold
+    int32_t grab;               // partners a wave takes per grab in a block item (<= 64)
 };
 
-template <int kThreads>
+template <int kThreads, int U>
 __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
     constexpr int kWaves = kThreads / 64;
     extern __shared__ uint32_t table[];
@@ -383,13 +388,13 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
             const Table c = build_block_table<kThreads>(table, a.adj, xb, d, &fail_flag);
             while (true) {
                 int32_t g = 0;
-                if (lane == 0) g = atomicAdd(&next_batch, 64);
+                if (lane == 0) g = atomicAdd(&next_batch, a.grab);
                 const int32_t b = (int32_t)rtgd::uniform_u32((uint32_t)g);
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
-                const int64_t p1 = min((int64_t)it.c, p0 + 64);
-                if (c.linear) hits += stream_spans<true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
-                else hits += stream_spans<false>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
+                const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
+                if (c.linear) hits += stream_spans<true, U>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
+                else hits += stream_spans<false, U>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
             }
         } else {
             // Wave items: small owners small_list[it.b, it.c), one per wave at a time.
@@ -398,8 +403,8 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 const int32_t x = a.small_list[s];
                 const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
                 const Table c = build_wave_table(tab, a.adj, xb, d);
-                if (c.linear) hits += stream_spans<true>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
-                else hits += stream_spans<false>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
+                if (c.linear) hits += stream_spans<true, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
+                else hits += stream_spans<false, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
                 wave_sync_lds();
             }
         }
@@ -413,6 +418,23 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
     }
     hits = rtgd::wave_sum_u64(hits);
     if (lane == 0 && hits) atomicAdd(&a.count[0], (unsigned long long)hits);
+}
+
+using BatchKernel = void (*)(BatchArgs);
+
+// Instantiated (block size, unroll) pairs of the batched kernel.
+BatchKernel batched_kernel(int threads, int unroll) {
+    if (threads == 256 && unroll == 2) return tc_batched<256, 2>;
+    if (threads == 256 && unroll == 4) return tc_batched<256, 4>;
+    if (threads == 256 && unroll == 8) return tc_batched<256, 8>;
+    if (threads == 512 && unroll == 2) return tc_batched<512, 2>;
+    if (threads == 512 && unroll == 4) return tc_batched<512, 4>;
+    if (threads == 512 && unroll == 8) return tc_batched<512, 8>;
+    return nullptr;
+}
+
+void launch_batched(BatchKernel k, int grid, int threads, size_t lds, hipStream_t st, const BatchArgs &a) {
+    hipLaunchKernelGGL(k, dim3(grid), dim3(threads), lds, st, a);
 }
 
 // ---- planning kernels ----
@@ -487,22 +509,58 @@ int dev_alloc(T **p, size_t count) {
     return RTG_OK;
 }
 
-// Tunables of the batched plan.
+// Tunables of the batched plan, written as a comma list of key=value (keys threads, unroll,
+// item, wave_item, wave_max, long, grab); chosen by the autotuner, read from the tuning cache or
+// set by the RTG_TC_TUNE environment variable.
 struct BatchTune {
     int32_t block_threads = 256;
-    int64_t item_work = 1 << 16;   // streamed elements per block item
-    int64_t wave_item_work = 1 << 14;
-    int32_t wave_owner_max = -1;   // -1: derived from the table size
+    int32_t unroll = 8;
+    int64_t item_work = 1 << 21;   // streamed elements per block item
+    int64_t wave_item_work = 1 << 12;
+    int32_t wave_owner_max = -1;   // -1: as large as the wave's table share allows
+    int32_t long_min = 128;
+    int32_t grab = 64;
 };
 
-int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
+std::string tune_string(const BatchTune &t) {
+    return "threads=" + std::to_string(t.block_threads) + ",unroll=" + std::to_string(t.unroll) +
+           ",item=" + std::to_string(t.item_work) + ",wave_item=" + std::to_string(t.wave_item_work) +
+           ",wave_max=" + std::to_string(t.wave_owner_max) + ",long=" + std::to_string(t.long_min) +
+           ",grab=" + std::to_string(t.grab);
+}
+
+void tune_from_string(BatchTune &t, const std::string &s) {
+    size_t pos = 0;
+    while (pos < s.size()) {
+        size_t e = s.find(',', pos);
+        if (e == std::string::npos) e = s.size();
+        const std::string kv = s.substr(pos, e - pos);
+        const size_t eq = kv.find('=');
+        if (eq != std::string::npos) {
+            const std::string k = kv.substr(0, eq);
+            const long long v = std::atoll(kv.c_str() + eq + 1);
+            if (k == "threads") t.block_threads = (int32_t)v;
+            else if (k == "unroll") t.unroll = (int32_t)v;
+            else if (k == "item") t.item_work = std::max<long long>(1, v);
+            else if (k == "wave_item") t.wave_item_work = std::max<long long>(1, v);
+            else if (k == "wave_max") t.wave_owner_max = (int32_t)v;
+            else if (k == "long") t.long_min = (int32_t)std::max<long long>(1, v);
+            else if (k == "grab") t.grab = (int32_t)std::max<long long>(1, std::min<long long>(64, v));
+        }
+        pos = e + 1;
+    }
+}
+
+// Graph-level part of the batched plan: owner assignment, partner spans grouped by owner, the
+// byte models, and host copies of the per-owner counts that item building needs.
+int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     using namespace rtg;
     const rtg_graph *g = p->g;
     const int32_t n = g->n;
     hipStream_t st = ctx->stream;
     uint32_t *d_cnt = nullptr;
     unsigned long long *d_work = nullptr, *d_bytes = nullptr;
-    DevFree f1, f2, f3, f4, f5;
+    DevFree f1, f2, f3, f4;
     RTG_TRY(dev_alloc(&d_cnt, (size_t)n + 1));
     f1.p = d_cnt;
     RTG_TRY(dev_alloc(&d_work, (size_t)n + 1));
@@ -536,17 +594,28 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
                                                                 (const uint32_t *)p->d_wofs, d_cnt, p->d_wspan);
         HIP_TRY(hipGetLastError());
     }
-    // Host side: item list from per-owner partner counts and work.
-    std::vector<int32_t> hoff((size_t)n + 1), hcnt((size_t)n + 1);
-    std::vector<unsigned long long> hwork((size_t)n + 1);
-    HIP_TRY(hipMemcpyAsync(hoff.data(), g->d_offsets, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(hwork.data(), d_work, sizeof(unsigned long long) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(hcnt.data(), p->d_wofs, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
+    p->hoff.resize((size_t)n + 1);
+    p->hcnt.resize((size_t)n + 1);
+    p->hwork.resize((size_t)n + 1);
+    HIP_TRY(hipMemcpyAsync(p->hoff.data(), g->d_offsets, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(p->hwork.data(), d_work, sizeof(unsigned long long) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(p->hcnt.data(), p->d_wofs, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return RTG_OK;
+}
 
+// Tune-dependent part: table sizes, the work queue (heaviest first) and the kernel instance.
+int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
+    using namespace rtg;
+    const rtg_graph *g = p->g;
+    const int32_t n = g->n;
+    hipStream_t st = ctx->stream;
+    const std::vector<int32_t> &hoff = p->hoff, &hcnt = p->hcnt;
+    const std::vector<unsigned long long> &hwork = p->hwork;
     const int threads = tune.block_threads;
+    if (threads != 256 && threads != 512) return set_error(RTG_EINVAL, "batched TC: threads must be 256 or 512");
     const int waves = threads / 64;
-    int32_t max_d = g->max_row;
+    const int32_t max_d = g->max_row;
     // Block table: both cuckoo halves of the largest owner, at least 2048 slots, a multiple of
     // 64 * waves so that the wave-private tables split it evenly.
     int32_t max_table = std::max<int32_t>(2048, 2 * (int32_t)cuckoo_half(max_d));
@@ -556,10 +625,7 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
     const int32_t wave_table = max_table / waves;
     int32_t fit = 0;  // largest out-degree whose table fits a wave's share
     while (2 * (int32_t)cuckoo_half(fit + 1) <= wave_table) ++fit;
-    int32_t wave_owner_max = tune.wave_owner_max >= 0 ? std::min(tune.wave_owner_max, fit) : fit;
-    p->max_table = max_table;
-    p->block_threads = threads;
-    p->wave_owner_max = wave_owner_max;
+    const int32_t wave_owner_max = tune.wave_owner_max >= 0 ? std::min(tune.wave_owner_max, fit) : fit;
 
     std::vector<TcItem> items;
     std::vector<std::pair<uint64_t, int32_t>> small;  // (work, owner)
@@ -603,8 +669,13 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
         queue.push_back(TcItem{1, 0, (int32_t)i, (int32_t)j});
         i = j;
     }
+    if (p->d_items) (void)hipFree(p->d_items);
+    if (p->d_small) (void)hipFree(p->d_small);
+    p->d_items = nullptr;
+    p->d_small = nullptr;
     p->n_items = (int64_t)queue.size();
     p->n_small = (int64_t)small_list.size();
+    p->n_block_items = (int64_t)items.size();
     RTG_TRY(dev_alloc(&p->d_items, queue.size()));
     RTG_TRY(dev_alloc(&p->d_small, small_list.size()));
     if (!queue.empty())
@@ -615,19 +686,155 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
 
     // Persistent grid: as many blocks as fit on the device at once.
     const size_t lds = sizeof(uint32_t) * (size_t)max_table;
+    BatchKernel k = batched_kernel(threads, tune.unroll);
+    if (!k) return set_error(RTG_EINVAL, "batched TC: unsupported (threads, unroll) = (" + std::to_string(threads) + ", " +
+                                             std::to_string(tune.unroll) + ")");
     int per_cu = 0;
-    if (threads == 256) {
-        HIP_TRY(hipFuncSetAttribute((const void *)tc_batched<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tc_batched<256>, threads, lds));
-    } else if (threads == 512) {
-        HIP_TRY(hipFuncSetAttribute((const void *)tc_batched<512>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tc_batched<512>, threads, lds));
-    } else {
-        return set_error(RTG_EINVAL, "batched TC: block_threads must be 256 or 512");
-    }
+    HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, lds));
     if (per_cu < 1) per_cu = 1;
+    p->max_table = max_table;
+    p->block_threads = threads;
+    p->wave_owner_max = wave_owner_max;
+    p->kernel = (void *)k;
+    p->unroll = tune.unroll;
+    p->long_min = tune.long_min;
+    p->grab = tune.grab;
+    p->tune = tune_string(tune);
     p->grid = (int32_t)std::min<int64_t>((int64_t)per_cu * ctx->cu_count, std::max<int64_t>(p->n_items, 1));
     return RTG_OK;
+}
+
+// ---- autotuning ----------------------------------------------------------------------------
+// Candidate configurations, timed on the plan's own graph. The first is the default.
+const char *kCandidates[] = {
+    "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
+    "threads=256,unroll=8,item=524288,wave_item=4096,wave_max=-1,long=128,grab=64",
+    "threads=256,unroll=4,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
+    "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=64,grab=64",
+    "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=32",
+    "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=64,long=128,grab=64",
+    "threads=256,unroll=8,item=2097152,wave_item=16384,wave_max=0,long=128,grab=64",
+    "threads=256,unroll=2,item=131072,wave_item=4096,wave_max=-1,long=32,grab=16",
+    "threads=512,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
+};
+
+uint64_t fnv1a(const void *data, size_t bytes, uint64_t h = 1469598103934665603ull) {
+    const unsigned char *c = (const unsigned char *)data;
+    for (size_t i = 0; i < bytes; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    return h;
+}
+
+// Cache key: device architecture and CU count, graph shape, a hash of the row offsets and the
+// edge range.
+std::string tune_key(rtg_ctx *ctx, const rtg_tc_plan *p) {
+    uint64_t h = fnv1a(p->hoff.data(), sizeof(int32_t) * p->hoff.size());
+    int64_t r[2] = {p->e_begin, p->e_end};
+    h = fnv1a(r, sizeof(r), h);
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s/cu%d/n%d/m%lld/%016llx", ctx->prop.gcnArchName, ctx->cu_count, p->g->n,
+                  (long long)p->g->m, (unsigned long long)h);
+    return buf;
+}
+
+// The cache file is a flat JSON object of "key": "tune string" entries.
+std::vector<std::pair<std::string, std::string>> read_tune_cache(const char *path) {
+    std::vector<std::pair<std::string, std::string>> out;
+    if (!path) return out;
+    std::ifstream f(path);
+    if (!f) return out;
+    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::vector<std::string> strs;
+    size_t i = 0;
+    while (i < text.size()) {
+        if (text[i] == '"') {
+            std::string cur;
+            ++i;
+            while (i < text.size() && text[i] != '"') {
+                if (text[i] == '\\' && i + 1 < text.size()) ++i;
+                cur.push_back(text[i++]);
+            }
+            strs.push_back(cur);
+        }
+        ++i;
+    }
+    for (size_t k = 0; k + 1 < strs.size(); k += 2) out.push_back({strs[k], strs[k + 1]});
+    return out;
+}
+
+int write_tune_cache(const char *path, const std::vector<std::pair<std::string, std::string>> &entries) {
+    std::ofstream f(path, std::ios::out | std::ios::trunc);
+    if (!f) return rtg::set_error(RTG_EIO, std::string("cannot write tuning cache ") + path);
+    f << "{\n";
+    for (size_t i = 0; i < entries.size(); ++i)
+        f << "  \"" << entries[i].first << "\": \"" << entries[i].second << "\"" << (i + 1 < entries.size() ? "," : "") << "\n";
+    f << "}\n";
+    return RTG_OK;
+}
+
+int launch_count(rtg_ctx *ctx, rtg_tc_plan *p);
+
+// Times every candidate on the plan (median of three runs after one warm-up) and returns the
+// fastest in *best.
+int autotune(rtg_ctx *ctx, rtg_tc_plan *p, BatchTune *best) {
+    float best_ms = 1e30f;
+    std::string best_s;
+    for (const char *cand : kCandidates) {
+        BatchTune t;
+        tune_from_string(t, cand);
+        if (plan_items(ctx, p, t) != RTG_OK) continue;
+        float ms[3];
+        bool ok = true;
+        for (int r = -1; r < 3 && ok; ++r) {
+            if (hipEventRecord(ctx->ev[2], ctx->stream) != hipSuccess || launch_count(ctx, p) != RTG_OK ||
+                hipEventRecord(ctx->ev[3], ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev[3]) != hipSuccess)
+                ok = false;
+            else if (r >= 0)
+                ms[r] = rtg::elapsed_ms(ctx->ev[2], ctx->ev[3]);
+        }
+        if (!ok) continue;
+        std::sort(ms, ms + 3);
+        if (ms[1] < best_ms) {
+            best_ms = ms[1];
+            best_s = cand;
+        }
+    }
+    if (best_s.empty()) return rtg::set_error(RTG_EHIP, "autotune: no candidate ran");
+    *best = BatchTune{};
+    tune_from_string(*best, best_s);
+    p->autotuned = true;
+    return RTG_OK;
+}
+
+// Chooses the tune for a batched plan: RTG_TC_TUNE, else the cache entry, else the
+// autotuner (when asked for), else the defaults; then builds the work queue for it.
+int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p) {
+    RTG_TRY(plan_graph(ctx, p));
+    BatchTune tune;
+    const char *env = std::getenv("RTG_TC_TUNE");
+    const std::string key = tune_key(ctx, p);
+    auto cache = read_tune_cache(p->opts.tune_cache);
+    bool have = false;
+    if (env) {
+        tune_from_string(tune, env);
+        have = true;
+    } else {
+        for (const auto &kv : cache)
+            if (kv.first == key) {
+                tune_from_string(tune, kv.second);
+                have = true;
+                p->tune_from_cache = true;
+            }
+    }
+    if (!have && p->opts.autotune) {
+        RTG_TRY(autotune(ctx, p, &tune));
+        if (p->opts.tune_cache) {
+            cache.erase(std::remove_if(cache.begin(), cache.end(), [&](const auto &kv) { return kv.first == key; }), cache.end());
+            cache.push_back({key, tune_string(tune)});
+            RTG_TRY(write_tune_cache(p->opts.tune_cache, cache));
+        }
+    }
+    return plan_items(ctx, p, tune);
 }
 
 int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
@@ -651,10 +858,9 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
             if (p->n_items == 0) break;
             BatchArgs a{g->d_offsets, g->d_adjs, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
                         p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
-                        p->long_min};
+                        p->long_min, p->grab};
             const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
-            if (p->block_threads == 256) tc_batched<256><<<p->grid, 256, lds, st>>>(a);
-            else tc_batched<512><<<p->grid, 512, lds, st>>>(a);
+            launch_batched((BatchKernel)p->kernel, p->grid, p->block_threads, lds, st, a);
             break;
         }
         default:
@@ -686,13 +892,8 @@ int rtg_tc_plan_create_impl(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts 
             if (hipGetLastError() != hipSuccess) { rc = set_error(RTG_EHIP, "edge_rows launch failed"); break; }
         }
         if (opts->method == RTG_TC_BATCHED) {
-            BatchTune tune;
-            if ((rc = plan_batched(ctx, p, tune)) != RTG_OK) break;
-        } else if (opts->method == RTG_TC_PLAIN_THREAD || opts->method == RTG_TC_PLAIN_WAVE) {
-            // byte model only
-            BatchTune tune;
-            (void)tune;
-        } else {
+            if ((rc = plan_batched(ctx, p)) != RTG_OK) break;
+        } else if (opts->method != RTG_TC_PLAIN_THREAD && opts->method != RTG_TC_PLAIN_WAVE) {
             rc = set_error(RTG_EUNSUPPORTED, "rtg_tc_plan_create: method not plannable");
             break;
         }
@@ -728,7 +929,7 @@ static void fill_stats(rtg_tc_plan *p, rtg_tc_stats *s) {
         s->nbuckets = 2;
         s->strategy[0] = 0;  // block items: block-shared LDS hash
         s->strategy[1] = 1;  // wave items: wave-private LDS hash
-        s->bucket_items[0] = p->n_items;
+        s->bucket_items[0] = p->n_block_items;
         s->bucket_items[1] = p->n_small;
     }
     s->kernel_launches = 2;
@@ -769,6 +970,17 @@ RTG_API int rtg_tc_plan_run(rtg_ctx *ctx, rtg_tc_plan *p, uint64_t *count, rtg_t
         fill_stats(p, stats);
         stats->t_count_ms = rtg::elapsed_ms(ctx->ev[0], ctx->ev[1]);
     }
+    return RTG_OK;
+}
+
+RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
+    if (!p || !buf || len == 0) return rtg_fail(RTG_EINVAL, "rtg_tc_plan_describe: bad arguments");
+    std::string d = "method=" + std::to_string(p->opts.method);
+    if (p->opts.method == RTG_TC_BATCHED)
+        d += "," + p->tune + ",grid=" + std::to_string(p->grid) + ",table_slots=" + std::to_string(p->max_table) +
+             ",wave_owner_max=" + std::to_string(p->wave_owner_max) + ",source=" +
+             (p->autotuned ? "autotune" : (p->tune_from_cache ? "cache" : (std::getenv("RTG_TC_TUNE") ? "env" : "default")));
+    std::snprintf(buf, len, "%s", d.c_str());
     return RTG_OK;
 }
 

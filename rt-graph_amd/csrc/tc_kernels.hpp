@@ -3,6 +3,9 @@
 
 #include "graph.hpp"
 
+#include <string>
+#include <vector>
+
 // A work item of the batched kernel. kind 0: block item — owner vertex `a` whose out-list is
 // staged in the block's LDS hash table, partners wpart[b .. c). kind 1: wave item — small
 // owners small_list[b .. c), one owner at a time per wave, each with a wave-private table.
@@ -33,13 +36,22 @@ struct rtg_tc_plan {
     int32_t block_threads = 256;
     int32_t wave_owner_max = 0;      // owners with d+ <= this run as wave items
     int32_t long_min = 48;           // partner lists at least this long are streamed directly
+    int32_t grab = 64;               // partners per wave grab inside a block item
     int32_t grid = 0;
+    int32_t unroll = 4;
+    void *kernel = nullptr;          // batched: selected kernel instantiation
     // byte model
     uint64_t alg_bytes = 0, merge_bytes = 0;
     // captured launch sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double t_plan_ms = 0.0;
+    // batched planning state
+    std::vector<int32_t> hoff, hcnt;        // host copies: row offsets, partner-list offsets
+    std::vector<unsigned long long> hwork;  // streamed elements per owner
+    int64_t n_block_items = 0;
+    std::string tune;                       // chosen configuration
+    bool autotuned = false, tune_from_cache = false;
 };
 
 // Two-hop (2A1) count, tc_2a1.hip. Caller holds ctx->mu.

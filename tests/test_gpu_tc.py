@@ -134,6 +134,33 @@ def test_hip_graph_replays(ctx):
     plan.close()
 
 
+def test_autotune_and_cache(ctx, tmp_path):
+    off, adj, t = oracle_csr("plcm")
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    cache = tmp_path / "tune.json"
+    p1 = g.tc_plan(method=rtg.TC_BATCHED, autotune=True, tune_cache=str(cache))
+    assert p1.describe()["source"] == "autotune"
+    assert p1.run()[0] == t
+    assert cache.exists() and "gfx" in cache.read_text()
+    p2 = g.tc_plan(method=rtg.TC_BATCHED, autotune=True, tune_cache=str(cache))
+    d2 = p2.describe()
+    assert d2["source"] == "cache" and d2["unroll"] == p1.describe()["unroll"]
+    assert p2.run()[0] == t
+
+
+def test_every_candidate_configuration(ctx, monkeypatch):
+    off, adj, t = oracle_csr("hub")
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    for tune in ("threads=256,unroll=2,item=1000,wave_item=64,wave_max=0,long=1,grab=1",
+                 "threads=512,unroll=4,item=50000,wave_item=100000,wave_max=2000,long=1000000,grab=64",
+                 "threads=256,unroll=8,item=131072,wave_item=4096,wave_max=64,long=32,grab=16"):
+        monkeypatch.setenv("RTG_TC_TUNE", tune)
+        p = g.tc_plan(method=rtg.TC_BATCHED)
+        assert p.describe()["source"] == "env"
+        assert p.run()[0] == t, tune
+        p.close()
+
+
 def test_symmetric_csr(ctx):
     p = GRAPHS["gnm_dups"]
     rp, nodes = orc.relabel(p)
