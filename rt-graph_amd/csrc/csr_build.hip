@@ -71,68 +71,96 @@ __global__ void removal_flags(const uint32_t *__restrict__ deg, int32_t n, uint3
         rm[i] = deg[i] <= 1u ? 1u : 0u;
 }
 
-// Wave-aggregated append: every valid lane gets a slot in out[] via one atomic per wave.
-__device__ __forceinline__ void wave_append(bool valid, uint64_t key, uint64_t *out,
-                                            unsigned long long *counter) {
-    const uint64_t mask = __ballot(valid);
-    if (mask == 0) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (valid) out[base + __popcll(mask & rtgd::lanemask_lt())] = key;
+// Block-aggregated append: each thread holds up to kPer keys; one exclusive scan over the block
+// and one atomic per block place them contiguously in out[] (order across blocks is arbitrary;
+// the keys are sorted afterwards).
+constexpr int kPer = 8;
+constexpr int kPackThreads = 256;
+
+template <int kMax>
+__device__ __forceinline__ void block_append(const uint64_t (&keys)[kMax], uint32_t mask, uint64_t *out,
+                                             unsigned long long *counter) {
+    const int cnt = __popc(mask);
+    __shared__ uint32_t wsum[kPackThreads / 64];
+    __shared__ unsigned long long base_s;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t incl = rtgd::wave_inclusive_scan((uint32_t)cnt);
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kPackThreads / 64; ++w) {
+        const uint32_t v = wsum[w];
+        woff += (w < wave) ? v : 0u;
+        tot += v;
+    }
+    if (threadIdx.x == 0) base_s = tot ? atomicAdd(counter, (unsigned long long)tot) : 0ull;
+    __syncthreads();
+    const unsigned long long base = base_s + woff + (incl - (uint32_t)cnt);
+#pragma unroll
+    for (int i = 0; i < kMax; ++i)
+        if ((mask >> i) & 1u) out[base + __popc(mask & ((1u << i) - 1u))] = keys[i];
+    __syncthreads();  // wsum and base_s are reused by the next tile
 }
 
 // Orientation, removal and key packing (TCPreprocessing). removes_ex is the exclusive scan of
 // the removal flags, so removes[x] (inclusive, graph.cpp:128-131) = removes_ex[x] for kept x.
-__global__ void orient_pack(const int32_t *__restrict__ pairs, int64_t np,
-                            const uint32_t *__restrict__ deg, const uint32_t *__restrict__ removes_ex,
-                            int bits, uint64_t *__restrict__ keys, unsigned long long *counter) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // Uniform trip count per wave so the ballots in wave_append see every lane.
-    const int64_t wave_start = start - (start & 63);
-    for (int64_t base = wave_start; base < np; base += stride) {
-        const int64_t i = base + (start & 63);
-        bool valid = false;
-        uint64_t key = 0;
-        if (i < np) {
-            int32_t s = pairs[2 * i], d = pairs[2 * i + 1];
-            uint32_t ds = deg[s], dd = deg[d];
-            if (ds > dd || (ds == dd && s > d)) {
-                int32_t t = s; s = d; d = t;
-                uint32_t u = ds; ds = dd; dd = u;
-            }
-            valid = ds > 1u && dd > 1u;
-            if (valid) {
-                uint64_t ns = (uint64_t)(uint32_t)(s - (int32_t)removes_ex[s]);
-                uint64_t nd = (uint64_t)(uint32_t)(d - (int32_t)removes_ex[d]);
-                key = (ns << bits) | nd;
+__global__ __launch_bounds__(kPackThreads) void orient_pack(const int32_t *__restrict__ pairs, int64_t np,
+                                                            const uint32_t *__restrict__ deg,
+                                                            const uint32_t *__restrict__ removes_ex, int bits,
+                                                            uint64_t *__restrict__ keys, unsigned long long *counter) {
+    const int64_t tile = (int64_t)kPackThreads * kPer;
+    for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < np; t0 += (int64_t)gridDim.x * tile) {
+        uint64_t k[kPer];
+        uint32_t mask = 0;
+        const int64_t b = t0 + (int64_t)threadIdx.x * kPer;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int64_t idx = b + i;
+            k[i] = 0;
+            if (idx < np) {
+                const int2 pr = reinterpret_cast<const int2 *>(pairs)[idx];
+                int32_t s = pr.x, d = pr.y;
+                uint32_t ds = deg[s], dd = deg[d];
+                if (ds > dd || (ds == dd && s > d)) {
+                    int32_t t = s; s = d; d = t;
+                    uint32_t u = ds; ds = dd; dd = u;
+                }
+                if (ds > 1u && dd > 1u) {
+                    const uint64_t ns = (uint64_t)(uint32_t)(s - (int32_t)removes_ex[s]);
+                    const uint64_t nd = (uint64_t)(uint32_t)(d - (int32_t)removes_ex[d]);
+                    k[i] = (ns << bits) | nd;
+                    mask |= 1u << i;
+                }
             }
         }
-        wave_append(valid, key, keys, counter);
+        block_append<kPer>(k, mask, keys, counter);
     }
 }
 
 // Symmetric mode: both directions of every non-loop pair.
-__global__ void symmetric_pack(const int32_t *__restrict__ pairs, int64_t np, int bits,
-                               uint64_t *__restrict__ keys, unsigned long long *counter) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t wave_start = start - (start & 63);
-    for (int64_t base = wave_start; base < np; base += stride) {
-        const int64_t i = base + (start & 63);
-        bool valid = false;
-        uint64_t k1 = 0, k2 = 0;
-        if (i < np) {
-            uint64_t s = (uint32_t)pairs[2 * i], d = (uint32_t)pairs[2 * i + 1];
-            valid = s != d;
-            k1 = (s << bits) | d;
-            k2 = (d << bits) | s;
+__global__ __launch_bounds__(kPackThreads) void symmetric_pack(const int32_t *__restrict__ pairs, int64_t np, int bits,
+                                                               uint64_t *__restrict__ keys, unsigned long long *counter) {
+    const int64_t tile = (int64_t)kPackThreads * kPer;
+    for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < np; t0 += (int64_t)gridDim.x * tile) {
+        uint64_t k[2 * kPer];
+        uint32_t mask = 0;
+        const int64_t b = t0 + (int64_t)threadIdx.x * kPer;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int64_t idx = b + i;
+            k[2 * i] = k[2 * i + 1] = 0;
+            if (idx < np) {
+                const int2 pr = reinterpret_cast<const int2 *>(pairs)[idx];
+                const uint64_t s = (uint32_t)pr.x, d = (uint32_t)pr.y;
+                if (s != d) {
+                    k[2 * i] = (s << bits) | d;
+                    k[2 * i + 1] = (d << bits) | s;
+                    mask |= 3u << (2 * i);
+                }
+            }
         }
-        wave_append(valid, k1, keys, counter);
-        wave_append(valid, k2, keys, counter);
+        block_append<2 * kPer>(k, mask, keys, counter);
     }
 }
 
@@ -358,13 +386,15 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
         n_out = nodes - removed;
         bits = bits_for(std::max<int64_t>(n_out, 2));
         if (npairs > 0) {
-            orient_pack<<<grid_for(ctx, npairs), kT, 0, st>>>(d_pairs, npairs, deg, rmx, bits, keys0, counter);
+            orient_pack<<<(unsigned)std::min<int64_t>(ceil_div(npairs, (int64_t)kPackThreads * kPer), (int64_t)ctx->cu_count * 8),
+                          kPackThreads, 0, st>>>(d_pairs, npairs, deg, rmx, bits, keys0, counter);
             HIP_TRY(hipGetLastError());
         }
     } else {
         bits = bits_for(std::max<int64_t>(n_out, 2));
         if (npairs > 0) {
-            symmetric_pack<<<grid_for(ctx, npairs), kT, 0, st>>>(d_pairs, npairs, bits, keys0, counter);
+            symmetric_pack<<<(unsigned)std::min<int64_t>(ceil_div(npairs, (int64_t)kPackThreads * kPer), (int64_t)ctx->cu_count * 8),
+                             kPackThreads, 0, st>>>(d_pairs, npairs, bits, keys0, counter);
             HIP_TRY(hipGetLastError());
         }
     }
