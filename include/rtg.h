@@ -109,6 +109,10 @@ int rtg_gen_rmat_device(rtg_ctx *ctx, uint32_t scale, int64_t nsamples, double a
                         double c, uint64_t seed, int32_t *d_pairs /* 2*nsamples */);
 int rtg_gen_uniform_u32_device(rtg_ctx *ctx, int64_t n, uint64_t seed, uint64_t stream,
                                uint32_t *d_out);
+int rtg_gen_sorted_array_device(rtg_ctx *ctx, int64_t n, uint64_t seed, uint32_t *d_out);
+/* deg is a host array (rtg_gen_powerlaw_degrees); d_pairs holds 2*(total/2 + total) ints. */
+int rtg_gen_config_model_device(rtg_ctx *ctx, int32_t n, const int32_t *deg, int64_t total,
+                                double p_close, uint64_t seed, int32_t *d_pairs, int64_t *npairs);
 
 /* ---- triangle counting ----------------------------------------------------------------- */
 #define RTG_TC_PLAIN_THREAD 0  /* thread per oriented edge, 1A2 form of ListIntersectionKernel
@@ -196,6 +200,79 @@ int rtg_lookup_u32(rtg_ctx *ctx, const uint32_t *d_data, int64_t n, const uint32
                    int64_t q, int32_t *d_out, double *t_kernel_ms);
 int rtg_lookup_i32(rtg_ctx *ctx, const int32_t *d_data, int64_t n, const int32_t *d_keys,
                    int64_t q, int32_t *d_out, double *t_kernel_ms);
+
+/* ---- query server: continuous batching over a device-resident graph and sorted array ---- */
+#define RTG_REQ_PAIR 0     /* |N(u) & N(v)| on the symmetric CSR                                  */
+#define RTG_REQ_LOOKUP 1   /* first-occurrence lookup of a batch of keys in the sorted array      */
+/* request status */
+#define RTG_REQ_PENDING 0
+#define RTG_REQ_RUNNING 1
+#define RTG_REQ_DONE 2
+#define RTG_REQ_CANCELLED 3
+#define RTG_REQ_TIMEOUT 4
+#define RTG_REQ_FAILED 5
+
+typedef struct rtg_server rtg_server;
+
+typedef struct rtg_server_opts {
+    int32_t slots;              /* concurrent batches (streams); default 4                      */
+    int32_t max_batch_requests; /* requests per batch; default 8192                             */
+    int64_t max_batch_pairs;    /* pair queries per batch; default 8192                         */
+    int64_t max_batch_keys;     /* lookup keys per batch; default 2^21                          */
+    int64_t mem_budget_bytes;   /* admission: bytes of admitted, unfinished requests; 1 GiB     */
+    int64_t timeout_us;         /* queued longer than this: dropped with RTG_REQ_TIMEOUT (0=off)*/
+    int32_t poll_us;            /* dispatcher sleep when idle; default 2                        */
+    int32_t null_executor;      /* no device: batches complete after null_latency_us, no results */
+    int32_t null_latency_us;
+    int32_t reserved;
+} rtg_server_opts;
+
+typedef struct rtg_server_stats {
+    int64_t admitted, rejected, completed, cancelled, failed, batches;
+    int64_t max_batch_requests, max_busy_slots;
+    double mean_batch_requests;
+    double mean_us, p50_us, p99_us, p999_us, max_us; /* arrival -> result visible to the host  */
+    double throughput_qps, offered_qps;
+    char last_error[128];
+} rtg_server_stats;
+
+/* g must be symmetric (rtg_graph_from_pairs without RTG_PAIRS_ORIENT); d_array ascending.
+   ctx, g and d_array may be NULL with opts->null_executor. */
+int rtg_server_create(rtg_ctx *ctx, const rtg_graph *g, const uint32_t *d_array, int64_t n_array,
+                      const rtg_server_opts *opts, rtg_server **out);
+/* Submit returns RTG_EAGAIN (request rejected, nothing queued) when the memory budget is full.
+   Lookup keys are read and results written (host memory) until the request is terminal. */
+int rtg_server_submit_pair(rtg_server *s, int32_t u, int32_t v, uint64_t *req_id);
+int rtg_server_submit_lookup(rtg_server *s, const uint32_t *keys, int32_t nkeys, int32_t *out,
+                             uint64_t *req_id);
+/* Status of a request; a terminal status (DONE, CANCELLED, TIMEOUT, FAILED) is reported once,
+   after which the id is forgotten. */
+int rtg_server_poll(rtg_server *s, uint64_t req_id, int32_t *status, uint32_t *pair_result,
+                    int64_t *latency_ns);
+int rtg_server_cancel(rtg_server *s, uint64_t req_id); /* only while pending (else EAGAIN)   */
+int rtg_server_get_stats(rtg_server *s, rtg_server_stats *out);
+int rtg_server_reset_stats(rtg_server *s);
+int rtg_server_destroy(rtg_server *s);                 /* waits for in-flight batches        */
+
+/* Synthetic Poisson stream (the serving workload): rate, count, warm-up (not measured),
+   fraction of pair queries (endpoints drawn in proportion to degree), lookup batch size
+   U{1..max_keys} of uniform keys. */
+typedef struct rtg_stream_spec {
+    double rate_qps;
+    int64_t nqueries;
+    int64_t warmup;
+    double pair_frac;
+    int32_t max_keys;
+    int32_t reserved;
+    uint64_t seed;
+} rtg_stream_spec;
+int rtg_server_replay(rtg_server *s, const rtg_stream_spec *spec, uint32_t *pair_out /* nullable */,
+                      uint64_t *lookup_sum /* nullable: per lookup, sum of (result + 2) */,
+                      rtg_server_stats *stats);
+/* The stream's queries, for checking: pool == NULL returns the key-pool size in *pool_n. */
+int rtg_stream_queries(const rtg_graph *g, const rtg_stream_spec *spec, int32_t *kinds, int32_t *a,
+                       int32_t *b, int64_t *key_off, int32_t *nkeys, uint32_t *pool,
+                       int64_t *pool_n);
 
 /* ---- host-side generators (seeded, counter-based; no GPU needed) ------------------------ */
 int rtg_gen_ba(int32_t n, int32_t m, uint64_t seed, int32_t *pairs /* 2*m*(n-m) */,

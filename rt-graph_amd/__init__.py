@@ -548,6 +548,26 @@ def gen_rmat_device(ctx: Context, scale: int, nsamples: int, a: float = 0.57, b:
     return out
 
 
+def gen_sorted_array_device(ctx: Context, n: int, seed: int = 1) -> DeviceArray:
+    out = DeviceArray(ctx, (n,), np.uint32)
+    _check(_lib.rtg_gen_sorted_array_device(ctx.handle, C.c_int64(n), C.c_uint64(seed), C.c_void_p(out.ptr)))
+    return out
+
+
+def gen_config_model_device(ctx: Context, deg: np.ndarray, p_close: float, seed: int = 1) -> DeviceArray:
+    """Device version of gen_config_model (same pairs, same order); returns a DeviceArray of
+    2 * npairs int32 (over-allocated buffer trimmed by shape)."""
+    deg = np.ascontiguousarray(deg, dtype=np.int32)
+    total = int(deg.sum(dtype=np.int64))
+    out = DeviceArray(ctx, (2 * (total // 2 + total),), np.int32)
+    k = C.c_int64()
+    _check(_lib.rtg_gen_config_model_device(ctx.handle, C.c_int32(deg.size), _ptr(deg), C.c_int64(total),
+                                            C.c_double(p_close), C.c_uint64(seed), C.c_void_p(out.ptr), C.byref(k)))
+    out.shape = (2 * k.value,)
+    out.nbytes = 8 * k.value
+    return out
+
+
 def gen_powerlaw_degrees(n: int, gamma: float, kmax: float, mean: float, seed: int = 1) -> np.ndarray:
     deg = np.empty(n, dtype=np.int32)
     tot = C.c_int64()
@@ -596,3 +616,134 @@ def gen_uniform_u32(n: int, seed: int = 1, stream: int = 0x51554552) -> np.ndarr
     out = np.empty(n, dtype=np.uint32)
     _check(_lib.rtg_gen_uniform_u32(C.c_int64(n), C.c_uint64(seed), C.c_uint64(stream), _ptr(out)))
     return out
+
+
+# ---- query server ---------------------------------------------------------------------------------
+REQ_PAIR, REQ_LOOKUP = 0, 1
+REQ_PENDING, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_TIMEOUT, REQ_FAILED = range(6)
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_batch_requests", C.c_int32), ("max_batch_pairs", C.c_int64),
+                ("max_batch_keys", C.c_int64), ("mem_budget_bytes", C.c_int64), ("timeout_us", C.c_int64),
+                ("poll_us", C.c_int32), ("null_executor", C.c_int32), ("null_latency_us", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("admitted", C.c_int64), ("rejected", C.c_int64), ("completed", C.c_int64),
+                ("cancelled", C.c_int64), ("failed", C.c_int64), ("batches", C.c_int64),
+                ("max_batch_requests", C.c_int64), ("max_busy_slots", C.c_int64),
+                ("mean_batch_requests", C.c_double), ("mean_us", C.c_double), ("p50_us", C.c_double),
+                ("p99_us", C.c_double), ("p999_us", C.c_double), ("max_us", C.c_double),
+                ("throughput_qps", C.c_double), ("offered_qps", C.c_double), ("last_error", C.c_char * 128)]
+
+    def as_dict(self) -> dict:
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "last_error"}
+        d["last_error"] = self.last_error.decode(errors="replace")
+        return d
+
+
+class StreamSpec(C.Structure):
+    _fields_ = [("rate_qps", C.c_double), ("nqueries", C.c_int64), ("warmup", C.c_int64),
+                ("pair_frac", C.c_double), ("max_keys", C.c_int32), ("reserved", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
+class Server:
+    """rtg_server: continuous-batching query server over a symmetric device graph and a sorted
+    device array. With null_executor=True it runs without a device (no results computed)."""
+
+    def __init__(self, ctx: Context | None, graph: DeviceGraph | None, d_array: DeviceArray | None,
+                 n_array: int = 0, **opts):
+        self.ctx, self.graph, self.d_array = ctx, graph, d_array
+        o = ServerOpts()
+        for k, v in opts.items():
+            setattr(o, k, int(v))
+        self._h = C.c_void_p()
+        _check(_lib.rtg_server_create(ctx.handle if ctx else None, graph._h if graph else None,
+                                      C.c_void_p(d_array.ptr) if d_array else None, C.c_int64(n_array),
+                                      C.byref(o), C.byref(self._h)))
+        self._keep = {}
+
+    def submit_pair(self, u: int, v: int) -> int:
+        rid = C.c_uint64()
+        _check(_lib.rtg_server_submit_pair(self._h, C.c_int32(u), C.c_int32(v), C.byref(rid)))
+        return rid.value
+
+    def submit_lookup(self, keys: np.ndarray) -> tuple[int, np.ndarray]:
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        out = np.full(keys.size, -3, dtype=np.int32)
+        rid = C.c_uint64()
+        _check(_lib.rtg_server_submit_lookup(self._h, _ptr(keys), C.c_int32(keys.size), _ptr(out), C.byref(rid)))
+        self._keep[rid.value] = (keys, out)  # buffers must outlive the request
+        return rid.value, out
+
+    def poll(self, rid: int):
+        st, pr, lat = C.c_int32(), C.c_uint32(), C.c_int64()
+        _check(_lib.rtg_server_poll(self._h, C.c_uint64(rid), C.byref(st), C.byref(pr), C.byref(lat)))
+        if st.value not in (REQ_PENDING, REQ_RUNNING):
+            self._keep.pop(rid, None)
+        return st.value, pr.value, lat.value
+
+    def wait(self, rid: int, timeout_s: float = 30.0):
+        import time
+        t0 = time.perf_counter()
+        while True:
+            st, pr, lat = self.poll(rid)
+            if st not in (REQ_PENDING, REQ_RUNNING):
+                return st, pr, lat
+            if time.perf_counter() - t0 > timeout_s:
+                raise TimeoutError(f"request {rid} still {st}")
+            time.sleep(20e-6)
+
+    def cancel(self, rid: int) -> None:
+        _check(_lib.rtg_server_cancel(self._h, C.c_uint64(rid)))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(_lib.rtg_server_get_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self) -> None:
+        _check(_lib.rtg_server_reset_stats(self._h))
+
+    def replay(self, rate_qps: float, nqueries: int, warmup: int = 0, pair_frac: float = 0.7,
+               max_keys: int = 4096, seed: int = 1, want_results: bool = False):
+        spec = StreamSpec(rate_qps=rate_qps, nqueries=nqueries, warmup=warmup, pair_frac=pair_frac,
+                          max_keys=max_keys, seed=seed)
+        pr = np.zeros(nqueries, dtype=np.uint32) if want_results else None
+        ls = np.zeros(nqueries, dtype=np.uint64) if want_results else None
+        st = ServerStats()
+        _check(_lib.rtg_server_replay(self._h, C.byref(spec), _ptr(pr), _ptr(ls), C.byref(st)))
+        return st.as_dict(), pr, ls
+
+    def close(self) -> None:
+        if self._h:
+            _lib.rtg_server_destroy(self._h)
+            self._h = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stream_queries(graph: DeviceGraph | None, rate_qps: float, nqueries: int, pair_frac: float = 0.7,
+                   max_keys: int = 4096, seed: int = 1) -> dict:
+    """The replay's query list (rtg_stream_queries), for recomputing every answer."""
+    spec = StreamSpec(rate_qps=rate_qps, nqueries=nqueries, warmup=0, pair_frac=pair_frac, max_keys=max_keys, seed=seed)
+    pn = C.c_int64()
+    _check(_lib.rtg_stream_queries(graph._h if graph else None, C.byref(spec), None, None, None, None, None, None,
+                                   C.byref(pn)))
+    kinds = np.zeros(nqueries, np.int32)
+    a = np.zeros(nqueries, np.int32)
+    b = np.zeros(nqueries, np.int32)
+    off = np.zeros(nqueries, np.int64)
+    nk = np.zeros(nqueries, np.int32)
+    pool = np.zeros(pn.value, np.uint32)
+    _check(_lib.rtg_stream_queries(graph._h if graph else None, C.byref(spec), _ptr(kinds), _ptr(a), _ptr(b),
+                                   _ptr(off), _ptr(nk), _ptr(pool), C.byref(pn)))
+    return {"kind": kinds, "u": a, "v": b, "key_off": off, "nkeys": nk, "pool": pool}

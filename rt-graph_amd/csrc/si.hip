@@ -16,6 +16,7 @@
 //                 pair) and the thread bucket (thread per pair), each through an atomic
 //                 cursor, so that no launch parameter depends on the bucket sizes.
 #include "device.hpp"
+#include "si.hpp"
 
 #include <cstring>
 
@@ -342,6 +343,31 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
 }
 
 }  // namespace
+
+namespace rtg {
+
+size_t si_batched_work_bytes(int64_t npairs) {
+    return Arena::round(sizeof(uint32_t) * (size_t)std::max<int64_t>(npairs, 1)) * 3 + 1024;
+}
+
+int si_launch_batched(rtg_ctx *ctx, hipStream_t st, const uint32_t *d_off, const uint32_t *d_el,
+                      const uint32_t *d_pairs, int64_t npairs, void *work, uint32_t *d_out) {
+    if (npairs <= 0) return RTG_OK;
+    Lists L{d_off, d_el};
+    PairSrc P{d_pairs, 0, 1};
+    Arena ar{(char *)work, 0, si_batched_work_bytes(npairs)};
+    Buckets B{};
+    for (auto &l : B.list) l = ar.take<uint32_t>((size_t)npairs);
+    B.count = ar.take<unsigned long long>(8);
+    si_reset<<<1, 64, 0, st>>>(B.count);
+    si_classify<<<grid_for(ctx, npairs, 16), kThreads, 0, st>>>(L, P, npairs, B, 16u, 2048u, d_out);
+    const unsigned drain = (unsigned)std::min<int64_t>((int64_t)ctx->cu_count * 8, std::max<int64_t>(npairs, 1));
+    si_drain<<<drain, kThreads, 0, st>>>(L, P, B, d_out);
+    HIP_TRY(hipGetLastError());
+    return RTG_OK;
+}
+
+}  // namespace rtg
 
 extern "C" {
 

@@ -1,0 +1,102 @@
+"""Query-server scheduling without a GPU (null executor: batches complete after a fixed delay and
+compute nothing): memory admission, continuous batching, cancellation, timeouts, error paths."""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import pytest
+
+from conftest import load_rtg
+
+rtg = load_rtg()
+
+
+def server(**kw):
+    kw.setdefault("null_executor", 1)
+    return rtg.Server(None, None, None, 0, **kw)
+
+
+def test_admission_by_memory_budget():
+    s = server(slots=1, null_latency_us=200_000, mem_budget_bytes=1000)
+    keys = np.arange(100, dtype=np.uint32)  # footprint 16 + 8*100 = 816 bytes
+    r1, _ = s.submit_lookup(keys)
+    with pytest.raises(rtg.RTGError) as e:
+        s.submit_lookup(keys)
+    assert e.value.code == 6  # RTG_EAGAIN: rejected, nothing queued
+    assert s.wait(r1)[0] == rtg.REQ_DONE
+    r3, _ = s.submit_lookup(keys)  # budget freed by completion
+    assert s.wait(r3)[0] == rtg.REQ_DONE
+    st = s.stats()
+    assert (st["admitted"], st["rejected"], st["completed"]) == (2, 1, 2)
+    s.close()
+
+
+def test_continuous_batching_groups_waiting_requests():
+    s = server(slots=2, null_latency_us=3000)
+    ids = [s.submit_pair(i, i + 1) for i in range(300)]
+    for r in ids:
+        assert s.wait(r)[0] == rtg.REQ_DONE
+    st = s.stats()
+    assert st["completed"] == 300
+    assert st["batches"] < 300 and st["max_batch_requests"] > 1
+    assert st["max_busy_slots"] == 2
+    assert st["p99_us"] >= st["p50_us"] > 0
+    s.close()
+
+
+def test_batch_capacity_limits():
+    s = server(slots=1, null_latency_us=1000, max_batch_requests=7, max_batch_keys=64)
+    ids = [s.submit_pair(0, 1) for _ in range(20)]
+    ids += [s.submit_lookup(np.arange(40, dtype=np.uint32))[0] for _ in range(5)]
+    for r in ids:
+        assert s.wait(r)[0] == rtg.REQ_DONE
+    st = s.stats()
+    assert st["max_batch_requests"] <= 7
+    with pytest.raises(rtg.RTGError):
+        s.submit_lookup(np.arange(65, dtype=np.uint32))  # larger than a batch
+    s.close()
+
+
+def test_cancel_pending_only():
+    s = server(slots=1, null_latency_us=100_000)
+    a = s.submit_pair(1, 2)
+    time.sleep(0.02)  # a is running
+    b = s.submit_pair(3, 4)
+    c = s.submit_pair(5, 6)
+    s.cancel(c)
+    with pytest.raises(rtg.RTGError):
+        s.cancel(a)  # already running
+    assert s.wait(c)[0] == rtg.REQ_CANCELLED
+    assert s.wait(a)[0] == rtg.REQ_DONE and s.wait(b)[0] == rtg.REQ_DONE
+    assert s.stats()["cancelled"] == 1
+    s.close()
+
+
+def test_queue_timeout():
+    s = server(slots=1, null_latency_us=50_000, timeout_us=5_000)
+    first = s.submit_pair(0, 0)
+    time.sleep(0.005)
+    late = [s.submit_pair(0, 0) for _ in range(3)]
+    assert s.wait(first)[0] == rtg.REQ_DONE
+    assert all(s.wait(r)[0] == rtg.REQ_TIMEOUT for r in late)
+    s.close()
+
+
+def test_error_paths_and_destroy_with_queue():
+    s = server(slots=1, null_latency_us=100_000)
+    with pytest.raises(rtg.RTGError):
+        s.poll(123456)
+    with pytest.raises(rtg.RTGError):
+        s.submit_lookup(np.zeros(0, np.uint32))
+    for _ in range(5):
+        s.submit_pair(0, 1)
+    s.close()  # waits for the queue to drain, then stops
+
+
+def test_replay_null_executor():
+    s = server(slots=4, null_latency_us=100)
+    st, _, _ = s.replay(rate_qps=50_000, nqueries=2000, warmup=100, pair_frac=0.7, max_keys=64, seed=3)
+    assert st["completed"] + st["rejected"] + st["cancelled"] == 1900
+    assert st["completed"] > 0 and st["p99_us"] > 0
+    s.close()
