@@ -86,6 +86,37 @@ def cpu_baseline(off: np.ndarray, adj: np.ndarray, target_s: float) -> dict:
                       f"CSR, {edges} of {m} oriented edges, {secs:.1f} s"}
 
 
+def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
+    """Query serving: power-law configuration-model graph (n = 3,000,000, exponent 2.1, degrees
+    up to 30,000, triangle closing 0.3, seed 1) as a device-resident symmetric CSR, plus 2^28
+    sorted uniform uint32 values; a Poisson stream of 70% degree-biased pair intersections and
+    30% lookups of U{1..4096} uniform keys, replayed against the continuous-batching server."""
+    t0 = time.perf_counter()
+    deg = rtg.gen_powerlaw_degrees(3_000_000, 2.1, 30_000.0, 50.0, seed=1)
+    d_pairs = rtg.gen_config_model_device(ctx, deg, 0.3, seed=1)
+    g = rtg.DeviceGraph.from_pairs(ctx, d_pairs, relabel=False, orient=False)
+    npairs = d_pairs.shape[0] // 2
+    d_pairs.free()
+    d_arr = rtg.gen_sorted_array_device(ctx, 1 << 28, seed=1)
+    log(f"serving data: n={g.n} symmetric entries={g.m} (undirected {g.m // 2}), max degree {g.max_row}, "
+        f"2^28 sorted keys, built in {time.perf_counter() - t0:.1f}s")
+    srv = rtg.Server(ctx, g, d_arr, 1 << 28, slots=4)
+    warm = min(1 << 16, nqueries // 8)
+    st, _, _ = srv.replay(rate_qps=rate, nqueries=nqueries + warm, warmup=warm, pair_frac=0.7, max_keys=4096, seed=1)
+    srv.close()
+    log(f"serving: offered {rate:.3g} q/s, sustained {st['throughput_qps']:.3g} q/s, p50 {st['p50_us']:.1f} us, "
+        f"p99 {st['p99_us']:.1f} us, admitted {st['admitted']} rejected {st['rejected']}, "
+        f"mean batch {st['mean_batch_requests']:.1f} requests, busy slots up to {st['max_busy_slots']}")
+    out = {"graph": {"n": g.n, "undirected_edges": g.m // 2, "generated_pairs": npairs, "max_degree": g.max_row},
+           "array_keys": 1 << 28, "queries": nqueries, "warmup": warm,
+           **{k: st[k] for k in ("offered_qps", "throughput_qps", "p50_us", "p99_us", "p999_us", "mean_us", "max_us",
+                                  "admitted", "rejected", "completed", "cancelled", "failed", "batches",
+                                  "mean_batch_requests", "max_busy_slots")}}
+    g.close()
+    d_arr.free()
+    return out
+
+
 def read_traffic(kernel: str, workload: str):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     p = ROOT / "profiles" / f"{kernel}_{workload}_pmc.txt"
@@ -110,6 +141,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-queries", type=int, default=1 << 21)
+    ap.add_argument("--serve-rate", type=float, default=2.0e6)
     ap.add_argument("--tune-cache", default=str(ROOT / "rt-graph_amd" / "tune_cache.json"),
                     help="tuning cache read by the batched plan (written when autotuning a new key)")
     args = ap.parse_args()
@@ -177,6 +211,12 @@ def main() -> int:
         cpu = cpu_baseline(off, adj, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.4g} edges/s on {cpu['cores']} threads")
 
+    serving = None
+    if not args.no_serving:
+        plan.close()
+        g.close()
+        serving = serving_leg(rtg, ctx, args.serve_rate, args.serve_queries)
+
     traffic = read_traffic("tc_batched", args.workload)
     out = {
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
@@ -192,6 +232,7 @@ def main() -> int:
         "plain_ms": plain, "batched_over_plain": (plain_ms / t_ms) if plain_ms else None,
         "t_build_ms": bst.t_total_ms, "t_plan_ms": plan.plan_stats["t_plan_ms"],
         "wall_s_timed": wall, "alg_bytes": alg, "merge_bytes": st["merge_bytes"],
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     plan.close()

@@ -233,6 +233,10 @@ typedef struct rtg_server_stats {
     double mean_batch_requests;
     double mean_us, p50_us, p99_us, p999_us, max_us; /* arrival -> result visible to the host  */
     double throughput_qps, offered_qps;
+    double mean_batch_device_us;  /* device time of a batch (copies and kernels)               */
+    double mean_batch_pack_us;    /* host time packing a batch into staging buffers            */
+    double mean_batch_unpack_us;  /* host time delivering a batch's results                    */
+    double mean_batch_slot_us;    /* launch -> results delivered                               */
     char last_error[128];
 } rtg_server_stats;
 

@@ -636,7 +636,10 @@ class ServerStats(C.Structure):
                 ("max_batch_requests", C.c_int64), ("max_busy_slots", C.c_int64),
                 ("mean_batch_requests", C.c_double), ("mean_us", C.c_double), ("p50_us", C.c_double),
                 ("p99_us", C.c_double), ("p999_us", C.c_double), ("max_us", C.c_double),
-                ("throughput_qps", C.c_double), ("offered_qps", C.c_double), ("last_error", C.c_char * 128)]
+                ("throughput_qps", C.c_double), ("offered_qps", C.c_double),
+                ("mean_batch_device_us", C.c_double), ("mean_batch_pack_us", C.c_double),
+                ("mean_batch_unpack_us", C.c_double), ("mean_batch_slot_us", C.c_double),
+                ("last_error", C.c_char * 128)]
 
     def as_dict(self) -> dict:
         d = {f: getattr(self, f) for f, _ in self._fields_ if f != "last_error"}

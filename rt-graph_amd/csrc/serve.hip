@@ -57,7 +57,8 @@ struct Req {
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t start = nullptr, done = nullptr;
+    int64_t t_launch = 0, pack_ns = 0;
     uint32_t *h_pairs = nullptr, *h_pair_out = nullptr, *h_keys = nullptr;
     int32_t *h_lookup_out = nullptr;
     uint32_t *d_pairs = nullptr, *d_pair_out = nullptr, *d_keys = nullptr;
@@ -88,6 +89,7 @@ struct rtg_server {
     int64_t admitted = 0, rejected = 0, completed = 0, cancelled = 0, failed = 0, batches = 0;
     int64_t max_batch = 0, max_busy = 0, batched_requests = 0;
     int64_t t_first_arrival = 0, t_last_done = 0;
+    double device_ms = 0, pack_ms = 0, unpack_ms = 0, slot_ms = 0;  // summed over batches
     std::vector<int64_t> latencies;  // ns, completed requests
     std::string error;
 };
@@ -136,6 +138,7 @@ void take_batch(rtg_server *s, Slot &slot, int64_t now) {
 
 int launch_slot(rtg_server *s, Slot &slot) {
     int64_t np = 0, nk = 0;
+    const int64_t t_pack = now_ns();
     for (Req *r : slot.reqs) {
         r->status.store(RTG_REQ_RUNNING, std::memory_order_release);
         if (r->kind == RTG_REQ_PAIR) {
@@ -147,11 +150,14 @@ int launch_slot(rtg_server *s, Slot &slot) {
             nk += r->nkeys;
         }
     }
+    slot.t_launch = now_ns();
+    slot.pack_ns = slot.t_launch - t_pack;
     if (s->opts.null_executor) {
         slot.fake_deadline = now_ns() + (int64_t)s->opts.null_latency_us * 1000;
         return RTG_OK;
     }
     hipStream_t st = slot.stream;
+    HIP_TRY(hipEventRecord(slot.start, st));
     if (np) {
         HIP_TRY(hipMemcpyAsync(slot.d_pairs, slot.h_pairs, 8 * (size_t)np, hipMemcpyHostToDevice, st));
         RTG_TRY(rtg::si_launch_batched(s->ctx, st, (const uint32_t *)s->g->d_offsets, (const uint32_t *)s->g->d_adjs,
@@ -169,13 +175,21 @@ int launch_slot(rtg_server *s, Slot &slot) {
 
 void complete_slot(rtg_server *s, Slot &slot, bool ok) {
     const bool null_exec = s->opts.null_executor != 0;
+    const int64_t t_unpack = now_ns();
+    float dev_ms = 0.f;
+    if (ok && !null_exec && hipEventElapsedTime(&dev_ms, slot.start, slot.done) != hipSuccess) dev_ms = 0.f;
     for (Req *r : slot.reqs) {
         if (ok && !null_exec) {
             if (r->kind == RTG_REQ_PAIR) r->pair_result = slot.h_pair_out[r->pos];
             else std::memcpy(r->out, slot.h_lookup_out + r->pos, sizeof(int32_t) * (size_t)r->nkeys);
         }
     }
+    const int64_t t_end = now_ns();
     std::lock_guard<std::mutex> lock(s->mu);
+    s->device_ms += dev_ms;
+    s->pack_ms += (double)slot.pack_ns / 1e6;
+    s->unpack_ms += (double)(t_end - t_unpack) / 1e6;
+    s->slot_ms += (double)(t_end - slot.t_launch) / 1e6;
     for (Req *r : slot.reqs) {
         release(s, r, ok ? RTG_REQ_DONE : RTG_REQ_FAILED);
         if (ok) {
@@ -264,6 +278,7 @@ void free_slot(Slot &slot, bool device) {
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (slot.done) (void)hipEventDestroy(slot.done);
+    if (slot.start) (void)hipEventDestroy(slot.start);
     if (slot.stream) (void)hipStreamDestroy(slot.stream);
 }
 
@@ -279,7 +294,8 @@ int alloc_slot(rtg_server *s, Slot &slot) {
         return RTG_OK;
     }
     HIP_TRY(hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming));
+    HIP_TRY(hipEventCreate(&slot.start));
+    HIP_TRY(hipEventCreate(&slot.done));
     HIP_TRY(hipHostMalloc((void **)&slot.h_pairs, 8 * (size_t)np, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&slot.h_pair_out, 4 * (size_t)np, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&slot.h_keys, 4 * (size_t)nk, hipHostMallocDefault));
@@ -444,6 +460,12 @@ RTG_API int rtg_server_get_stats(rtg_server *s, rtg_server_stats *out) {
     out->max_batch_requests = s->max_batch;
     out->max_busy_slots = s->max_busy;
     out->mean_batch_requests = s->batches ? (double)s->batched_requests / (double)s->batches : 0.0;
+    if (s->batches) {
+        out->mean_batch_device_us = 1e3 * s->device_ms / (double)s->batches;
+        out->mean_batch_pack_us = 1e3 * s->pack_ms / (double)s->batches;
+        out->mean_batch_unpack_us = 1e3 * s->unpack_ms / (double)s->batches;
+        out->mean_batch_slot_us = 1e3 * s->slot_ms / (double)s->batches;
+    }
     lat = s->latencies;
     if (!lat.empty()) {
         double sum = 0;
@@ -470,6 +492,7 @@ RTG_API int rtg_server_reset_stats(rtg_server *s) {
     s->admitted = s->rejected = s->completed = s->cancelled = s->failed = s->batches = 0;
     s->max_batch = s->max_busy = s->batched_requests = 0;
     s->t_first_arrival = s->t_last_done = 0;
+    s->device_ms = s->pack_ms = s->unpack_ms = s->slot_ms = 0;
     s->latencies.clear();
     return RTG_OK;
 }
@@ -555,12 +578,14 @@ RTG_API int rtg_server_replay(rtg_server *s, const rtg_stream_spec *spec, uint32
         }
         qs[i] = q;
     }
-    // Output arena for lookups (reused results are summed at completion).
-    std::vector<int32_t> outs((size_t)std::max<int64_t>(total_keys, 1));
+    // Lookup outputs: one region per query when the caller wants the sums, else one shared
+    // discard region (only the dispatcher thread writes it).
+    const bool keep = lookup_sum != nullptr;
+    std::vector<int32_t> outs((size_t)std::max<int64_t>(keep ? total_keys : max_keys, 1));
     std::vector<int64_t> out_off((size_t)nq, 0);
     int64_t acc = 0;
     for (int64_t i = 0; i < nq; ++i)
-        if (qs[i].kind == RTG_REQ_LOOKUP) {
+        if (keep && qs[i].kind == RTG_REQ_LOOKUP) {
             out_off[i] = acc;
             acc += qs[i].nkeys;
         }
