@@ -100,6 +100,13 @@ def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
     d_arr = rtg.gen_sorted_array_device(ctx, 1 << 28, seed=1)
     log(f"serving data: n={g.n} symmetric entries={g.m} (undirected {g.m // 2}), max degree {g.max_row}, "
         f"2^28 sorted keys, built in {time.perf_counter() - t0:.1f}s")
+    # the lookup half on its own: 2^22 uniform keys through the array's search index
+    nk = 1 << 22
+    d_keys = ctx.to_device(rtg.gen_uniform_u32(nk, seed=3))
+    with rtg.LookupIndex(ctx, d_arr, 1 << 28) as ix:
+        ix.run(d_keys, nk)
+        lk_ms = sorted(ix.run(d_keys, nk)[1] for _ in range(5))[2]
+    d_keys.free()
     srv = rtg.Server(ctx, g, d_arr, 1 << 28, slots=4)
     warm = min(1 << 16, nqueries // 8)
     st, _, _ = srv.replay(rate_qps=rate, nqueries=nqueries + warm, warmup=warm, pair_frac=0.7, max_keys=4096, seed=1)
@@ -109,9 +116,11 @@ def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
         f"mean batch {st['mean_batch_requests']:.1f} requests, busy slots up to {st['max_busy_slots']}")
     out = {"graph": {"n": g.n, "undirected_edges": g.m // 2, "generated_pairs": npairs, "max_degree": g.max_row},
            "array_keys": 1 << 28, "queries": nqueries, "warmup": warm,
+           "lookup_index_keys_per_s": nk / (lk_ms * 1e-3),
            **{k: st[k] for k in ("offered_qps", "throughput_qps", "p50_us", "p99_us", "p999_us", "mean_us", "max_us",
                                   "admitted", "rejected", "completed", "cancelled", "failed", "batches",
-                                  "mean_batch_requests", "max_busy_slots")}}
+                                  "mean_batch_requests", "max_busy_slots", "mean_batch_device_us",
+                                  "mean_batch_pack_us", "mean_batch_unpack_us", "mean_batch_slot_us")}}
     g.close()
     d_arr.free()
     return out
@@ -142,7 +151,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-queries", type=int, default=1 << 21)
+    ap.add_argument("--serve-queries", type=int, default=1 << 24)
     ap.add_argument("--serve-rate", type=float, default=2.0e6)
     ap.add_argument("--tune-cache", default=str(ROOT / "rt-graph_amd" / "tune_cache.json"),
                     help="tuning cache read by the batched plan (written when autotuning a new key)")

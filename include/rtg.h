@@ -200,6 +200,22 @@ int rtg_lookup_u32(rtg_ctx *ctx, const uint32_t *d_data, int64_t n, const uint32
                    int64_t q, int32_t *d_out, double *t_kernel_ms);
 int rtg_lookup_i32(rtg_ctx *ctx, const int32_t *d_data, int64_t n, const int32_t *d_keys,
                    int64_t q, int32_t *d_out, double *t_kernel_ms);
+/* The same lookup through a search index built once per array, for callers that look up in
+   one array many times (the reference re-searches its array per call, simple_bs.cu:80-142;
+   this is the analogue of its build-once search structure). The index is a 32-ary tree of
+   128-byte nodes over d_data (about n/8 bytes of device memory); d_data is referenced, not
+   copied, and must stay allocated and unchanged while the index lives. Keys passed to
+   rtg_lookup_index_run have the array's type. levels = tree levels above the data (0: the
+   array is searched in LDS alone).                                                          */
+typedef struct rtg_lookup_index rtg_lookup_index;
+int rtg_lookup_index_create_u32(rtg_ctx *ctx, const uint32_t *d_data, int64_t n,
+                                rtg_lookup_index **out);
+int rtg_lookup_index_create_i32(rtg_ctx *ctx, const int32_t *d_data, int64_t n,
+                                rtg_lookup_index **out);
+int rtg_lookup_index_info(const rtg_lookup_index *ix, int32_t *levels, int64_t *bytes);
+int rtg_lookup_index_run(rtg_ctx *ctx, const rtg_lookup_index *ix, const void *d_keys,
+                         int64_t q, int32_t *d_out, double *t_kernel_ms);
+int rtg_lookup_index_destroy(rtg_lookup_index *ix);
 
 /* ---- query server: continuous batching over a device-resident graph and sorted array ---- */
 #define RTG_REQ_PAIR 0     /* |N(u) & N(v)| on the symmetric CSR                                  */
@@ -221,7 +237,7 @@ typedef struct rtg_server_opts {
     int64_t max_batch_keys;     /* lookup keys per batch; default 2^21                          */
     int64_t mem_budget_bytes;   /* admission: bytes of admitted, unfinished requests; 1 GiB     */
     int64_t timeout_us;         /* queued longer than this: dropped with RTG_REQ_TIMEOUT (0=off)*/
-    int32_t poll_us;            /* dispatcher sleep when idle; default 2                        */
+    int32_t poll_us;            /* dispatcher wait when idle (submits wake it); default 1000    */
     int32_t null_executor;      /* no device: batches complete after null_latency_us, no results */
     int32_t null_latency_us;
     int32_t reserved;
@@ -240,7 +256,8 @@ typedef struct rtg_server_stats {
     char last_error[128];
 } rtg_server_stats;
 
-/* g must be symmetric (rtg_graph_from_pairs without RTG_PAIRS_ORIENT); d_array ascending.
+/* g must be symmetric (rtg_graph_from_pairs without RTG_PAIRS_ORIENT); d_array ascending and
+   16-byte aligned (the server builds an rtg_lookup_index over it).
    ctx, g and d_array may be NULL with opts->null_executor. */
 int rtg_server_create(rtg_ctx *ctx, const rtg_graph *g, const uint32_t *d_array, int64_t n_array,
                       const rtg_server_opts *opts, rtg_server **out);

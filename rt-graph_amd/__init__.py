@@ -26,7 +26,7 @@ from pathlib import Path
 import numpy as np
 
 __all__ = [
-    "RTGError", "Context", "DeviceArray", "DeviceGraph", "TcPlan", "Graph", "GraphInfo",
+    "RTGError", "Context", "DeviceArray", "DeviceGraph", "TcPlan", "LookupIndex", "Graph", "GraphInfo",
     "CountOnGPU", "BinarySearchBasedIntersection", "BSAccessCountFunction", "Dataset",
     "read_dataset", "write_dataset", "TC_PLAIN_THREAD", "TC_PLAIN_WAVE", "TC_BATCHED", "TC_2A1",
     "SI_PLAIN", "SI_BATCHED", "SI_REFERENCE", "gen_ba", "gen_rmat", "gen_powerlaw_degrees",
@@ -506,6 +506,55 @@ def lookup(ctx: Context, d_data: DeviceArray, n: int, d_keys: DeviceArray, q: in
     fn = _lib.rtg_lookup_i32 if signed else _lib.rtg_lookup_u32
     _check(fn(ctx.handle, _ptr(d_data), C.c_int64(n), _ptr(d_keys), C.c_int64(q), _ptr(d_out), C.byref(ms)))
     return d_out, ms.value
+
+
+class LookupIndex:
+    """Search index built once over a device-resident ascending array (rtg_lookup_index);
+    run() gives the same results as lookup(). The array must stay allocated and unchanged."""
+
+    def __init__(self, ctx: Context, d_data: DeviceArray | int, n: int, signed: bool = False):
+        self.ctx, self.n, self.signed = ctx, n, signed
+        self._data = d_data  # keep the array alive while the index refers to it
+        h = C.c_void_p()
+        fn = _lib.rtg_lookup_index_create_i32 if signed else _lib.rtg_lookup_index_create_u32
+        _check(fn(ctx.handle, _ptr(d_data) if n else None, C.c_int64(n), C.byref(h)))
+        self._h = h
+
+    @property
+    def levels(self) -> int:
+        lv = C.c_int32()
+        _check(_lib.rtg_lookup_index_info(self._h, C.byref(lv), None))
+        return lv.value
+
+    @property
+    def nbytes(self) -> int:
+        b = C.c_int64()
+        _check(_lib.rtg_lookup_index_info(self._h, None, C.byref(b)))
+        return b.value
+
+    def run(self, d_keys: DeviceArray, q: int, d_out: DeviceArray | None = None):
+        if d_out is None:
+            d_out = DeviceArray(self.ctx, (q,), np.int32)
+        ms = C.c_double()
+        _check(_lib.rtg_lookup_index_run(self.ctx.handle, self._h, _ptr(d_keys), C.c_int64(q), _ptr(d_out), C.byref(ms)))
+        return d_out, ms.value
+
+    def close(self) -> None:
+        if self._h:
+            _lib.rtg_lookup_index_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def BSAccessCountFunction(data: np.ndarray, query: np.ndarray, device_id: int = 0) -> np.ndarray:

@@ -6,13 +6,14 @@
 // slots) and queued. A dispatcher thread owns `slots` execution slots, each with its own HIP
 // stream, event, pinned staging buffers and device buffers. Whenever a slot is free and
 // requests are waiting, the dispatcher packs as many as the slot holds into one batch and
-// launches it at once — copies in, the bucketed intersection kernel, the sampled lookup
+// launches it at once — copies in, the probe-item intersection kernel, the indexed lookup
 // kernel, copies out — so that requests arriving while earlier batches run join the next
 // free slot and their work overlaps on the device. Completed batches are detected by event
 // queries; each request records arrival and completion times on the host steady clock.
 //
 // Results do not depend on how requests are batched: each pair and each key is computed
-// independently by the same kernels as rtg_si_pairs and rtg_lookup_u32.
+// independently: a pair's count is a sum of per-element membership tests of its shorter
+// list in its longer one, and a key's result depends on the key alone.
 //
 // With null_executor set, the server runs without a device: a batch "completes" after
 // null_latency_us and writes no results (pair results 0, lookup outputs untouched), so that
@@ -28,10 +29,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <chrono>
+#include <condition_variable>
+#include <memory>
 #include <cstring>
 #include <deque>
 #include <thread>
-#include <unordered_map>
 
 namespace {
 
@@ -47,6 +49,7 @@ struct Req {
     int32_t u = 0, v = 0;
     const uint32_t *keys = nullptr;
     int32_t nkeys = 0;
+    int32_t items = 0;  // pair: 64-element probe items of the shorter list
     int32_t *out = nullptr;
     uint32_t pair_result = 0;
     int64_t t_arrive = 0, t_done = 0;
@@ -59,14 +62,59 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, done = nullptr;
     int64_t t_launch = 0, pack_ns = 0;
-    uint32_t *h_pairs = nullptr, *h_pair_out = nullptr, *h_keys = nullptr;
+    uint32_t *h_pairs = nullptr, *h_pair_out = nullptr, *h_keys = nullptr, *h_items = nullptr;
     int32_t *h_lookup_out = nullptr;
-    uint32_t *d_pairs = nullptr, *d_pair_out = nullptr, *d_keys = nullptr;
+    uint32_t *d_pairs = nullptr, *d_pair_out = nullptr, *d_keys = nullptr, *d_items = nullptr;
     int32_t *d_lookup_out = nullptr;
-    void *d_si_work = nullptr;
     std::vector<Req *> reqs;
     bool busy = false;
     int64_t fake_deadline = 0;
+};
+
+// Requests by id. Ids are consecutive from 1, so the table is a list of fixed chunks of
+// pointers: inserting never moves existing entries (no rehash or reallocation while the
+// server lock is held), and a chunk is released once every id in it has been collected.
+class LiveTable {
+   public:
+    static constexpr int kBits = 16;
+    static constexpr uint64_t kChunk = 1ull << kBits;
+
+    void put(uint64_t id, Req *r) {
+        const uint64_t i = id - 1, c = i >> kBits;
+        if (c >= chunks_.size()) {
+            chunks_.resize(c + 1);
+            used_.resize(c + 1, 0);
+        }
+        if (!chunks_[c]) chunks_[c].reset(new Req *[kChunk]());
+        chunks_[c][i & (kChunk - 1)] = r;
+        ++used_[c];
+        ++size_;
+    }
+    Req *get(uint64_t id) const {
+        if (id == 0) return nullptr;
+        const uint64_t i = id - 1, c = i >> kBits;
+        return c < chunks_.size() && chunks_[c] ? chunks_[c][i & (kChunk - 1)] : nullptr;
+    }
+    // Removes a present id; frees its chunk when no id of it can be added or is still present.
+    void erase(uint64_t id, uint64_t next_id) {
+        const uint64_t i = id - 1, c = i >> kBits;
+        chunks_[c][i & (kChunk - 1)] = nullptr;
+        --size_;
+        if (--used_[c] == 0 && next_id > (c + 1) * kChunk) chunks_[c].reset();
+    }
+    template <class F>
+    void for_each(F f) const {
+        for (const auto &ch : chunks_)
+            if (ch)
+                for (uint64_t k = 0; k < kChunk; ++k)
+                    if (ch[k]) f(ch[k]);
+    }
+    int64_t size() const { return size_; }
+
+   private:
+    std::vector<std::unique_ptr<Req *[]>> chunks_;
+    std::vector<int64_t> used_;
+    int64_t size_ = 0;
 };
 
 }  // namespace
@@ -76,11 +124,15 @@ struct rtg_server {
     const rtg_graph *g = nullptr;
     const uint32_t *d_array = nullptr;
     int64_t n_array = 0;
+    rtg_lookup_index *index = nullptr;  // built over d_array at creation
+    std::vector<int32_t> hoff;          // host copy of the CSR offsets (probe items per pair)
+    int64_t max_items = 0;              // probe items per batch
     rtg_server_opts opts{};
     std::vector<Slot> slots;
     std::mutex mu;  // pending, live, stats
     std::deque<Req *> pending;
-    std::unordered_map<uint64_t, Req *> live;
+    LiveTable live;
+    std::condition_variable wake;  // dispatcher idle wait; notified by submit
     std::atomic<uint64_t> next_id{1};
     std::atomic<int64_t> inflight_bytes{0};
     std::atomic<bool> stop{false};
@@ -90,7 +142,7 @@ struct rtg_server {
     int64_t max_batch = 0, max_busy = 0, batched_requests = 0;
     int64_t t_first_arrival = 0, t_last_done = 0;
     double device_ms = 0, pack_ms = 0, unpack_ms = 0, slot_ms = 0;  // summed over batches
-    std::vector<int64_t> latencies;  // ns, completed requests
+    std::deque<int64_t> latencies;  // ns, completed requests (chunked: no reallocation under the lock)
     std::string error;
 };
 
@@ -109,7 +161,7 @@ void release(rtg_server *s, Req *r, int32_t status) {
 // Takes requests from the queue into `slot` while it has room; expired requests are
 // cancelled instead. Caller holds s->mu.
 void take_batch(rtg_server *s, Slot &slot, int64_t now) {
-    int64_t np = 0, nk = 0;
+    int64_t np = 0, nk = 0, ni = 0;
     slot.reqs.clear();
     while (!s->pending.empty() && (int64_t)slot.reqs.size() < s->opts.max_batch_requests) {
         Req *r = s->pending.front();
@@ -124,8 +176,9 @@ void take_batch(rtg_server *s, Slot &slot, int64_t now) {
             continue;
         }
         if (r->kind == RTG_REQ_PAIR) {
-            if (np + 1 > s->opts.max_batch_pairs) break;
+            if (np + 1 > s->opts.max_batch_pairs || ni + r->items > s->max_items) break;
             r->pos = np++;
+            ni += r->items;
         } else {
             if (nk + r->nkeys > s->opts.max_batch_keys) break;
             r->pos = nk;
@@ -137,13 +190,18 @@ void take_batch(rtg_server *s, Slot &slot, int64_t now) {
 }
 
 int launch_slot(rtg_server *s, Slot &slot) {
-    int64_t np = 0, nk = 0;
+    int64_t np = 0, nk = 0, ni = 0;
     const int64_t t_pack = now_ns();
     for (Req *r : slot.reqs) {
         r->status.store(RTG_REQ_RUNNING, std::memory_order_release);
         if (r->kind == RTG_REQ_PAIR) {
             slot.h_pairs[2 * r->pos] = (uint32_t)r->u;
             slot.h_pairs[2 * r->pos + 1] = (uint32_t)r->v;
+            for (int32_t c = 0; c < r->items; ++c) {
+                slot.h_items[2 * ni] = (uint32_t)r->pos;
+                slot.h_items[2 * ni + 1] = (uint32_t)(64 * c);
+                ++ni;
+            }
             ++np;
         } else {
             std::memcpy(slot.h_keys + r->pos, r->keys, sizeof(uint32_t) * (size_t)r->nkeys);
@@ -160,13 +218,17 @@ int launch_slot(rtg_server *s, Slot &slot) {
     HIP_TRY(hipEventRecord(slot.start, st));
     if (np) {
         HIP_TRY(hipMemcpyAsync(slot.d_pairs, slot.h_pairs, 8 * (size_t)np, hipMemcpyHostToDevice, st));
-        RTG_TRY(rtg::si_launch_batched(s->ctx, st, (const uint32_t *)s->g->d_offsets, (const uint32_t *)s->g->d_adjs,
-                                       slot.d_pairs, np, slot.d_si_work, slot.d_pair_out));
+        HIP_TRY(hipMemsetAsync(slot.d_pair_out, 0, 4 * (size_t)np, st));
+        if (ni) {
+            HIP_TRY(hipMemcpyAsync(slot.d_items, slot.h_items, 8 * (size_t)ni, hipMemcpyHostToDevice, st));
+            RTG_TRY(rtg::si_launch_items(s->ctx, st, (const uint32_t *)s->g->d_offsets, (const uint32_t *)s->g->d_adjs,
+                                         slot.d_pairs, slot.d_items, ni, slot.d_pair_out));
+        }
         HIP_TRY(hipMemcpyAsync(slot.h_pair_out, slot.d_pair_out, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
     }
     if (nk) {
         HIP_TRY(hipMemcpyAsync(slot.d_keys, slot.h_keys, 4 * (size_t)nk, hipMemcpyHostToDevice, st));
-        RTG_TRY(rtg::lookup_launch<uint32_t>(s->ctx, st, s->d_array, s->n_array, slot.d_keys, nk, slot.d_lookup_out));
+        RTG_TRY(rtg::lookup_index_launch<uint32_t>(s->index, st, slot.d_keys, nk, slot.d_lookup_out));
         HIP_TRY(hipMemcpyAsync(slot.h_lookup_out, slot.d_lookup_out, 4 * (size_t)nk, hipMemcpyDeviceToHost, st));
     }
     HIP_TRY(hipEventRecord(slot.done, st));
@@ -258,7 +320,15 @@ void dispatcher(rtg_server *s) {
             std::lock_guard<std::mutex> lock(s->mu);
             s->max_busy = std::max<int64_t>(s->max_busy, busy);
         }
-        if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(s->opts.poll_us));
+        if (progress) continue;
+        if (busy > 0) {
+            std::this_thread::yield();  // batches in flight: keep polling their events
+        } else {
+            // idle: sleep until a request arrives (submit notifies) or poll_us passes
+            std::unique_lock<std::mutex> lock(s->mu);
+            if (s->pending.empty() && !s->stop.load(std::memory_order_acquire))
+                s->wake.wait_for(lock, std::chrono::microseconds(s->opts.poll_us));
+        }
     }
 }
 
@@ -271,10 +341,10 @@ void free_slot(Slot &slot, bool device) {
         return;
     }
     if (slot.stream) (void)hipStreamSynchronize(slot.stream);
-    void *host[] = {slot.h_pairs, slot.h_pair_out, slot.h_keys, slot.h_lookup_out};
+    void *host[] = {slot.h_pairs, slot.h_pair_out, slot.h_keys, slot.h_lookup_out, slot.h_items};
     for (void *p : host)
         if (p) (void)hipHostFree(p);
-    void *dev[] = {slot.d_pairs, slot.d_pair_out, slot.d_keys, slot.d_lookup_out, slot.d_si_work};
+    void *dev[] = {slot.d_pairs, slot.d_pair_out, slot.d_keys, slot.d_lookup_out, slot.d_items};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (slot.done) (void)hipEventDestroy(slot.done);
@@ -300,11 +370,12 @@ int alloc_slot(rtg_server *s, Slot &slot) {
     HIP_TRY(hipHostMalloc((void **)&slot.h_pair_out, 4 * (size_t)np, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&slot.h_keys, 4 * (size_t)nk, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&slot.h_lookup_out, 4 * (size_t)nk, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&slot.h_items, 8 * (size_t)s->max_items, hipHostMallocDefault));
     HIP_TRY(hipMalloc((void **)&slot.d_pairs, 8 * (size_t)np));
     HIP_TRY(hipMalloc((void **)&slot.d_pair_out, 4 * (size_t)np));
     HIP_TRY(hipMalloc((void **)&slot.d_keys, 4 * (size_t)nk));
     HIP_TRY(hipMalloc((void **)&slot.d_lookup_out, 4 * (size_t)nk));
-    HIP_TRY(hipMalloc(&slot.d_si_work, rtg::si_batched_work_bytes(np)));
+    HIP_TRY(hipMalloc((void **)&slot.d_items, 8 * (size_t)s->max_items));
     return RTG_OK;
 }
 
@@ -314,7 +385,7 @@ void defaults(rtg_server_opts &o) {
     if (o.max_batch_pairs <= 0) o.max_batch_pairs = 8192;
     if (o.max_batch_keys <= 0) o.max_batch_keys = 1 << 21;
     if (o.mem_budget_bytes <= 0) o.mem_budget_bytes = (int64_t)1 << 30;
-    if (o.poll_us <= 0) o.poll_us = 2;
+    if (o.poll_us <= 0) o.poll_us = 1000;
     if (o.null_latency_us <= 0) o.null_latency_us = 50;
 }
 
@@ -337,9 +408,10 @@ int submit(rtg_server *s, Req *r, uint64_t *req_id) {
         std::lock_guard<std::mutex> lock(s->mu);
         if (s->admitted == 0) s->t_first_arrival = r->t_arrive;
         ++s->admitted;
-        s->live[r->id] = r;
+        s->live.put(r->id, r);
         s->pending.push_back(r);
     }
+    s->wake.notify_one();
     *req_id = r->id;
     return RTG_OK;
 }
@@ -365,8 +437,8 @@ RTG_API int rtg_server_create(rtg_ctx *ctx, const rtg_graph *g, const uint32_t *
     if (!o.null_executor) {
         RTG_TRY(rtg::ctx_enter(ctx));
         if (!g || g->oriented) return rtg_fail(RTG_EINVAL, "rtg_server_create: needs a symmetric graph (built without RTG_PAIRS_ORIENT)");
-        if (n_array < 0 || (n_array > 0 && !d_array) || n_array > INT32_MAX)
-            return rtg_fail(RTG_EINVAL, "rtg_server_create: bad sorted array");
+        if (n_array < 0 || (n_array > 0 && !d_array) || n_array > INT32_MAX || ((uintptr_t)d_array & 15))
+            return rtg_fail(RTG_EINVAL, "rtg_server_create: bad sorted array (size, or not 16-byte aligned)");
     }
     auto *s = new rtg_server();
     s->ctx = ctx;
@@ -374,16 +446,32 @@ RTG_API int rtg_server_create(rtg_ctx *ctx, const rtg_graph *g, const uint32_t *
     s->d_array = d_array;
     s->n_array = n_array;
     s->opts = o;
+    if (!o.null_executor) {
+        // Probe items per batch: 16 per pair on average, and always room for the longest pair.
+        s->max_items = 16 * o.max_batch_pairs + rtg::ceil_div(std::max<int64_t>(g->max_row, 1), 64);
+        s->hoff.resize((size_t)g->n + 1);
+        if (hipMemcpy(s->hoff.data(), g->d_offsets, sizeof(int32_t) * ((size_t)g->n + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+            (void)hipGetLastError();
+            delete s;
+            return rtg_fail(RTG_EHIP, "rtg_server_create: reading the graph offsets failed");
+        }
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        const int rc = rtg::lookup_index_build(ctx, d_array, n_array, 0, &s->index);
+        if (rc != RTG_OK) {
+            delete s;
+            return rc;
+        }
+    }
     s->slots.resize((size_t)o.slots);
     for (Slot &slot : s->slots) {
         const int rc = alloc_slot(s, slot);
         if (rc != RTG_OK) {
             for (Slot &sl : s->slots) free_slot(sl, !o.null_executor);
+            rtg::lookup_index_free(s->index);
             delete s;
             return rc;
         }
     }
-    s->latencies.reserve(1 << 20);
     s->worker = std::thread(dispatcher, s);
     *out = s;
     return RTG_OK;
@@ -397,6 +485,10 @@ RTG_API int rtg_server_submit_pair(rtg_server *s, int32_t u, int32_t v, uint64_t
     r->kind = RTG_REQ_PAIR;
     r->u = u;
     r->v = v;
+    if (!s->opts.null_executor) {
+        const int32_t du = s->hoff[u + 1] - s->hoff[u], dv = s->hoff[v + 1] - s->hoff[v];
+        r->items = (std::min(du, dv) + 63) / 64;
+    }
     return submit(s, r, req_id);
 }
 
@@ -416,15 +508,14 @@ RTG_API int rtg_server_poll(rtg_server *s, uint64_t req_id, int32_t *status, uin
                             int64_t *latency_ns) {
     if (!s || !status) return rtg_fail(RTG_EINVAL, "rtg_server_poll: null argument");
     std::lock_guard<std::mutex> lock(s->mu);
-    auto it = s->live.find(req_id);
-    if (it == s->live.end()) return rtg_fail(RTG_EINVAL, "rtg_server_poll: unknown or already collected request");
-    Req *r = it->second;
+    Req *r = s->live.get(req_id);
+    if (!r) return rtg_fail(RTG_EINVAL, "rtg_server_poll: unknown or already collected request");
     const int32_t st = r->status.load(std::memory_order_acquire);
     *status = st;
     if (st == RTG_REQ_PENDING || st == RTG_REQ_RUNNING) return RTG_OK;
     if (pair_result) *pair_result = r->pair_result;
     if (latency_ns) *latency_ns = r->t_done - r->t_arrive;
-    s->live.erase(it);
+    s->live.erase(req_id, s->next_id.load());
     delete r;  // terminal: collected
     return RTG_OK;
 }
@@ -432,9 +523,8 @@ RTG_API int rtg_server_poll(rtg_server *s, uint64_t req_id, int32_t *status, uin
 RTG_API int rtg_server_cancel(rtg_server *s, uint64_t req_id) {
     if (!s) return rtg_fail(RTG_EINVAL, "rtg_server_cancel: null server");
     std::lock_guard<std::mutex> lock(s->mu);
-    auto it = s->live.find(req_id);
-    if (it == s->live.end()) return rtg_fail(RTG_EINVAL, "rtg_server_cancel: unknown request");
-    Req *r = it->second;
+    Req *r = s->live.get(req_id);
+    if (!r) return rtg_fail(RTG_EINVAL, "rtg_server_cancel: unknown request");
     int32_t expected = RTG_REQ_PENDING;
     if (!r->status.compare_exchange_strong(expected, RTG_REQ_CANCELLED))
         return rtg_fail(RTG_EAGAIN, "rtg_server_cancel: request already running or finished");
@@ -466,7 +556,7 @@ RTG_API int rtg_server_get_stats(rtg_server *s, rtg_server_stats *out) {
         out->mean_batch_unpack_us = 1e3 * s->unpack_ms / (double)s->batches;
         out->mean_batch_slot_us = 1e3 * s->slot_ms / (double)s->batches;
     }
-    lat = s->latencies;
+    lat.assign(s->latencies.begin(), s->latencies.end());
     if (!lat.empty()) {
         double sum = 0;
         int64_t mx = 0;
@@ -511,11 +601,13 @@ RTG_API int rtg_server_destroy(rtg_server *s) {
         std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
     s->stop.store(true);
+    s->wake.notify_all();
     if (s->worker.joinable()) s->worker.join();
     if (!s->opts.null_executor && s->ctx) (void)hipSetDevice(s->ctx->device);
     for (Slot &slot : s->slots) free_slot(slot, !s->opts.null_executor);
+    rtg::lookup_index_free(s->index);
     s->pending.clear();  // every queued request is also in `live`
-    for (auto &kv : s->live) delete kv.second;
+    s->live.for_each([](Req *r) { delete r; });
     delete s;
     return RTG_OK;
 }
@@ -593,27 +685,47 @@ RTG_API int rtg_server_replay(rtg_server *s, const rtg_stream_spec *spec, uint32
     std::vector<char> admitted((size_t)nq, 0);
     const int64_t warm = std::min<int64_t>(spec->warmup, nq);
     int64_t t0 = now_ns(), tbase = 0;
+    int64_t ci = 0;  // oldest request not yet collected
+    // Collects request j if it has finished (done = true), recording its pair result.
+    auto collect = [&](int64_t j, bool &done) -> int {
+        int32_t st = RTG_REQ_PENDING;
+        uint32_t pr = 0;
+        RTG_TRY(rtg_server_poll(s, ids[j], &st, &pr, nullptr));
+        done = st != RTG_REQ_PENDING && st != RTG_REQ_RUNNING;
+        if (done) {
+            if (pair_out && qs[j].kind == RTG_REQ_PAIR) pair_out[j] = pr;
+            admitted[j] = 2;
+        }
+        return RTG_OK;
+    };
     for (int64_t i = 0; i < nq; ++i) {
         if (i == warm) {
             // wait for the warm-up queries, then measure from a clean slate
             for (int64_t j = 0; j < warm; ++j) {
-                if (!admitted[j]) continue;
-                int32_t st = RTG_REQ_PENDING;
-                uint32_t pr = 0;
+                if (admitted[j] != 1) continue;
+                bool done = false;
                 while (true) {
-                    RTG_TRY(rtg_server_poll(s, ids[j], &st, &pr, nullptr));
-                    if (st != RTG_REQ_PENDING && st != RTG_REQ_RUNNING) break;
+                    RTG_TRY(collect(j, done));
+                    if (done) break;
                     std::this_thread::sleep_for(std::chrono::microseconds(20));
                 }
-                if (pair_out && qs[j].kind == RTG_REQ_PAIR) pair_out[j] = pr;
-                admitted[j] = 2;
             }
             RTG_TRY(rtg_server_reset_stats(s));
             t0 = now_ns();  // the measured stream starts now, at its own first arrival time
             tbase = qs[i].t;
         }
         const int64_t due = t0 + qs[i].t - tbase;
-        while (now_ns() < due) {
+        // Until the next arrival, collect finished requests in order, as a client would
+        // (at most one poll every 2 us, none within 5 us of the arrival).
+        for (int64_t now = now_ns(), next_poll = now; now < due; now = now_ns()) {
+            if (now < next_poll || due - now < 5000) continue;
+            next_poll = now + 2000;
+            while (ci < i && admitted[ci] != 1) ++ci;
+            if (ci < i) {
+                bool done = false;
+                RTG_TRY(collect(ci, done));
+                if (done) ++ci;
+            }
         }
         int rc;
         if (qs[i].kind == RTG_REQ_PAIR) rc = rtg_server_submit_pair(s, qs[i].a, qs[i].b, &ids[i]);
@@ -621,17 +733,15 @@ RTG_API int rtg_server_replay(rtg_server *s, const rtg_stream_spec *spec, uint32
         if (rc == RTG_OK) admitted[i] = 1;
         else if (rc != RTG_EAGAIN) return rc;
     }
-    // Collect everything.
-    for (int64_t i = 0; i < nq; ++i) {
+    // Collect the rest.
+    for (int64_t i = ci; i < nq; ++i) {
         if (admitted[i] != 1) continue;
-        int32_t st = RTG_REQ_PENDING;
-        uint32_t pr = 0;
+        bool done = false;
         while (true) {
-            RTG_TRY(rtg_server_poll(s, ids[i], &st, &pr, nullptr));
-            if (st != RTG_REQ_PENDING && st != RTG_REQ_RUNNING) break;
+            RTG_TRY(collect(i, done));
+            if (done) break;
             std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
-        if (pair_out && qs[i].kind == RTG_REQ_PAIR) pair_out[i] = pr;
     }
     if (lookup_sum)
         for (int64_t i = 0; i < nq; ++i) {

@@ -257,6 +257,46 @@ __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, Buckets
     }
 }
 
+// ---- probe items (latency path) ----
+// Number of elements of l[0..nl) below v, nl >= 1, by binary lifting: the trip count depends
+// only on nl, and every probe address is clamped into the list.
+__device__ __forceinline__ uint32_t count_below_lift(const uint32_t *__restrict__ l, uint32_t nl, uint32_t v) {
+    uint32_t base = 0;
+    for (uint32_t step = 1u << (31 - __builtin_clz(nl)); step > 0; step >>= 1) {
+        const uint32_t x = l[min(base + step, nl) - 1];
+        if (base + step <= nl && x < v) base += step;
+    }
+    return base;
+}
+
+// items[i] = {k, first}: one wave takes elements first .. first+63 of pair k's shorter list,
+// one lane per element and one binary search of the longer list each, and adds its count to
+// out[k] (cleared beforehand). Splitting every pair into 64-element items puts all of a
+// batch's probes in flight at once: the launch lasts about one binary search, however
+// skewed the pairs are.
+__global__ __launch_bounds__(kThreads) void si_items(Lists L, const uint2 *__restrict__ pairs,
+                                                     const uint2 *__restrict__ items, int64_t nitems,
+                                                     uint32_t *__restrict__ out) {
+    const int64_t nw = (int64_t)gridDim.x * (kThreads / 64);
+    for (int64_t it = rtgd::uniform_u64((uint64_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)); it < nitems;
+         it += nw) {
+        const uint2 item = items[it];
+        const uint2 p = pairs[item.x];
+        const uint32_t *s, *l;
+        uint32_t ns, nl;
+        short_long(L, p.x, p.y, s, ns, l, nl);
+        const uint32_t i = item.y + __lane_id();
+        uint32_t c = 0;
+        if (i < ns) {
+            const uint32_t v = s[i];
+            const uint32_t b = count_below_lift(l, nl, v);
+            c = (b < nl && l[b] == v) ? 1u : 0u;
+        }
+        c = rtgd::wave_sum_u32(c);
+        if (__lane_id() == 0 && c) atomicAdd(&out[item.x], c);
+    }
+}
+
 __global__ void si_alg_bytes(Lists L, PairSrc P, int64_t npairs, unsigned long long *acc) {
     uint64_t s = 0;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += (int64_t)gridDim.x * blockDim.x) {
@@ -363,6 +403,18 @@ int si_launch_batched(rtg_ctx *ctx, hipStream_t st, const uint32_t *d_off, const
     si_classify<<<grid_for(ctx, npairs, 16), kThreads, 0, st>>>(L, P, npairs, B, 16u, 2048u, d_out);
     const unsigned drain = (unsigned)std::min<int64_t>((int64_t)ctx->cu_count * 8, std::max<int64_t>(npairs, 1));
     si_drain<<<drain, kThreads, 0, st>>>(L, P, B, d_out);
+    HIP_TRY(hipGetLastError());
+    return RTG_OK;
+}
+
+int si_launch_items(rtg_ctx *ctx, hipStream_t st, const uint32_t *d_off, const uint32_t *d_el, const uint32_t *d_pairs,
+                    const uint32_t *d_items, int64_t nitems, uint32_t *d_out) {
+    if (nitems <= 0) return RTG_OK;
+    Lists L{d_off, d_el};
+    const int64_t blocks = ceil_div(nitems, (int64_t)(kThreads / 64));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->cu_count * 16));
+    si_items<<<grid, kThreads, 0, st>>>(L, reinterpret_cast<const uint2 *>(d_pairs), reinterpret_cast<const uint2 *>(d_items),
+                                        nitems, d_out);
     HIP_TRY(hipGetLastError());
     return RTG_OK;
 }

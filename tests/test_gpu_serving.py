@@ -63,6 +63,23 @@ def test_mix_equals_alone_and_oracle(ctx, world):
     srv.close()
 
 
+def test_small_batches_split_on_probe_items(ctx, world):
+    """With 2 pairs per batch the item capacity (16 per pair plus the longest row) splits hub
+    pairs into many batches; results still equal the oracle."""
+    g, off, adj, arr, d_arr = world
+    deg = np.diff(off)
+    hubs = np.argsort(deg)[-40:]
+    uv = [(int(hubs[i]), int(hubs[(i * 3 + 1) % 40])) for i in range(40)] + [(0, 0), (1, 1)]
+    exp = oracle_pairs(off, adj, uv)
+    srv = rtg.Server(ctx, g, d_arr, arr.size, slots=2, max_batch_pairs=2)
+    ids = [srv.submit_pair(u, v) for (u, v) in uv]
+    got = np.array([srv.wait(r)[1] for r in ids], np.uint32)
+    st = srv.stats()
+    srv.close()
+    assert np.array_equal(got, exp)
+    assert st["batches"] >= len(uv) // 2 and st["max_batch_requests"] <= 2
+
+
 def test_replay_stream_matches_oracle(ctx, world):
     g, off, adj, arr, d_arr = world
     srv = rtg.Server(ctx, g, d_arr, arr.size, slots=4)

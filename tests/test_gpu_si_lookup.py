@@ -91,3 +91,99 @@ def test_lookup_signed_reference_interface():
     got = rtg.BSAccessCountFunction(data, q)
     assert np.array_equal(data, ref_sorted)  # sorted in place, as the reference does
     assert np.array_equal(got, exp)
+
+
+# ---- search index (rtg_lookup_index) ---------------------------------------------------------
+def _index_keys(data, rng, signed=False):
+    dt = np.int32 if signed else np.uint32
+    lo, hi = (-(1 << 31), 1 << 31) if signed else (0, 1 << 32)
+    n = data.size
+    parts = [rng.integers(lo, hi, size=4000, dtype=np.int64).astype(dt),
+             np.array([lo, hi - 1, 0, 1], dtype=np.int64).astype(dt)]
+    if n:
+        parts.append(data[rng.integers(0, n, size=4000)])
+        edge = np.array([data[0], data[-1]], dtype=np.int64)
+        parts.append(np.concatenate([edge, edge - 1, edge + 1]).clip(lo, hi - 1).astype(dt))
+    return np.concatenate(parts)
+
+
+# sizes around the node (32) and LDS-top (8192) boundaries and every level count up to 3
+INDEX_SIZES = [0, 1, 31, 32, 33, 8191, 8192, 8193, 8192 * 32, 8192 * 32 + 1, (1 << 20) + 3,
+               8192 * 32 * 32 + 7]
+
+
+@pytest.mark.parametrize("n", INDEX_SIZES)
+def test_lookup_index_u32(ctx, n):
+    rng = np.random.default_rng(1000 + n)
+    data = np.sort(rng.integers(0, 1 << 32, size=n, dtype=np.uint64)).astype(np.uint32)
+    if n > 200:
+        data[n // 3: n // 3 + 100] = data[n // 3]  # a run of duplicates spanning node boundaries
+    keys = _index_keys(data, rng)
+    exp = orc.lookup(data, keys)
+    d_data = ctx.to_device(data if n else np.zeros(1, np.uint32))
+    d_keys = ctx.to_device(keys)
+    with rtg.LookupIndex(ctx, d_data, n) as ix:
+        assert ix.levels == (0 if n <= 8192 else 1 if n <= 8192 * 32 else 2 if n <= 8192 * 32 * 32 else 3)
+        out, _ = ix.run(d_keys, keys.size)
+        assert np.array_equal(out.numpy(), exp)
+        # a second run on the same index, on fewer keys, into the same buffer
+        out2, _ = ix.run(d_keys, keys.size // 2, out)
+        assert np.array_equal(out2.numpy()[: keys.size // 2], exp[: keys.size // 2])
+
+
+@pytest.mark.parametrize("n", [8192 * 32 + 5, 100000])
+def test_lookup_index_max_value_tail(ctx, n):
+    """Array ending in runs of 0xFFFFFFFF (the value the index pads with): first occurrences."""
+    rng = np.random.default_rng(n)
+    data = np.sort(rng.integers(0, 1 << 32, size=n, dtype=np.uint64)).astype(np.uint32)
+    data[-70:] = 0xFFFFFFFF
+    keys = np.concatenate([_index_keys(data, rng), np.full(3, 0xFFFFFFFF, np.uint32)])
+    exp = orc.lookup(data, keys)
+    assert exp[-1] == n - 70
+    d_data = ctx.to_device(data)
+    with rtg.LookupIndex(ctx, d_data, n) as ix:
+        out, _ = ix.run(ctx.to_device(keys), keys.size)
+        assert np.array_equal(out.numpy(), exp)
+
+
+@pytest.mark.parametrize("n", [5000, 300007])
+def test_lookup_index_i32(ctx, n):
+    rng = np.random.default_rng(n)
+    data = np.sort(rng.integers(-(1 << 31), 1 << 31, size=n, dtype=np.int64)).astype(np.int32)
+    data[-3:] = np.iinfo(np.int32).max
+    keys = _index_keys(data, rng, signed=True)
+    exp = orc.lookup(data, keys, signed=True)
+    d_data = ctx.to_device(data)
+    with rtg.LookupIndex(ctx, d_data, n, signed=True) as ix:
+        out, _ = ix.run(ctx.to_device(keys), keys.size)
+        assert np.array_equal(out.numpy(), exp)
+    # same answers as the stateless lookup
+    out_s, _ = rtg.lookup(ctx, d_data, n, ctx.to_device(keys), keys.size, signed=True)
+    assert np.array_equal(out_s.numpy(), exp)
+
+
+def test_lookup_index_rejects_unaligned(ctx):
+    d = ctx.to_device(np.arange(100000, dtype=np.uint32))
+    with pytest.raises(rtg.RTGError):
+        rtg.LookupIndex(ctx, d.ptr + 4, 99999)
+
+
+def test_lookup_index_full_size(ctx):
+    """2^28 sorted uint32 (1 GiB, the serving array): index vs the oracle on 2^20 keys and vs
+    the stateless sampled lookup on 2^22 keys."""
+    n = 1 << 28
+    d_data = rtg.gen_sorted_array_device(ctx, n, seed=1)
+    rng = np.random.default_rng(28)
+    host = d_data.numpy()
+    keys = np.concatenate([rng.integers(0, 1 << 32, size=1 << 19, dtype=np.uint64).astype(np.uint32),
+                           host[rng.integers(0, n, size=1 << 19)]])
+    exp = orc.lookup(host, keys)
+    with rtg.LookupIndex(ctx, d_data, n) as ix:
+        assert ix.levels == 3 and ix.nbytes < n // 7
+        out, _ = ix.run(ctx.to_device(keys), keys.size)
+        assert np.array_equal(out.numpy(), exp)
+        big = rtg.gen_uniform_u32(1 << 22, seed=5)
+        d_big = ctx.to_device(big)
+        a, _ = ix.run(d_big, big.size)
+        b, _ = rtg.lookup(ctx, d_data, n, d_big, big.size)
+        assert np.array_equal(a.numpy(), b.numpy())

@@ -100,3 +100,21 @@ def test_replay_null_executor():
     assert st["completed"] + st["rejected"] + st["cancelled"] == 1900
     assert st["completed"] > 0 and st["p99_us"] > 0
     s.close()
+
+
+def test_request_table_across_chunks():
+    """More than 2^16 requests (the id table's chunk) in flight and collected out of order:
+    every id stays valid until collected, then is unknown."""
+    s = server(slots=4, null_latency_us=100, max_batch_requests=4096)
+    ids = [s.submit_pair(i, i) for i in range(70_000)]
+    order = ids[65_530:65_545] + ids[::-1]
+    seen = set()
+    for r in order:
+        if r in seen:
+            continue
+        seen.add(r)
+        assert s.wait(r)[0] == rtg.REQ_DONE
+    with pytest.raises(rtg.RTGError):
+        s.poll(ids[0])
+    assert s.stats()["completed"] == 70_000
+    s.close()
