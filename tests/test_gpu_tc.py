@@ -226,3 +226,27 @@ def test_full_rmat_s21_ef44(ctx):
     assert st["alg_bytes"] == e["alg_bytes"]
     print("[full] plain wave count", flush=True)
     assert g.tc_count(method=rtg.TC_PLAIN_WAVE)[0] == e["triangles"]
+
+
+def test_full_plcm_n3000000(ctx):
+    """The full C3 workload (power-law configuration model, n = 3,000,000, closing 0.3),
+    generated and preprocessed on the device: CSR hashes, batched and plain counts and the
+    byte model against the golden values."""
+    e = GOLD.get("plcm_n3000000_seed1")
+    if e is None:
+        pytest.skip("golden entry not generated")
+    deg = rtg.gen_powerlaw_degrees(3_000_000, 2.1, 30_000.0, 50.0, seed=1)
+    assert sha(deg.astype("<i4")) == e["degrees_sha256"]
+    print("[full] generating the C3 configuration model on the device", flush=True)
+    dev = rtg.gen_config_model_device(ctx, deg, 0.3, seed=1)
+    g = rtg.DeviceGraph.from_pairs(ctx, dev)
+    dev.free()
+    assert (g.n, g.m, g.max_row) == (e["n"], e["m"], e["max_out_degree"])
+    off, adj = g.download()
+    assert sha(off.astype("<i4")) == e["offsets_sha256"] and sha(adj.astype("<i4")) == e["adjs_sha256"]
+    print("[full] batched count", flush=True)
+    cnt, st = g.tc_count(method=rtg.TC_BATCHED)
+    assert cnt == e["triangles"]
+    assert st["alg_bytes"] == e["alg_bytes"]
+    print("[full] plain wave count", flush=True)
+    assert g.tc_count(method=rtg.TC_PLAIN_WAVE)[0] == e["triangles"]
