@@ -126,6 +126,29 @@ def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
     return out
 
 
+def si_leg(rtg, ctx) -> dict:
+    """Set-intersection microbench: 2^21 Zipf(1.8) lists (lengths clipped at 65,536) of uniform
+    values below 2^24, seed 1, paired (2k, 2k+1) into 2^20 pairs; plain (wave per pair) against
+    batched, median of 5 kernel times each."""
+    lens = rtg.gen_zipf_lengths(1 << 21, 1.8, 65536, seed=1)
+    offs, el = rtg.gen_sorted_lists(lens, 1 << 24, seed=1)
+    pairs = np.arange(1 << 21, dtype=np.uint32)
+    d_off, d_el, d_p = ctx.to_device(offs), ctx.to_device(el), ctx.to_device(pairs)
+    res = {}
+    for name, m in (("plain", rtg.SI_PLAIN), ("batched", rtg.SI_BATCHED)):
+        ts = []
+        for _ in range(5):
+            out, st = rtg.si_pairs(ctx, d_off, d_el, offs.size - 1, d_p, pairs.size // 2, m)
+            ts.append(st["t_kernel_ms"])
+        res[name] = sorted(ts)[2]
+        out.free()
+    for a in (d_off, d_el, d_p):
+        a.free()
+    log(f"set intersection (2^20 Zipf pairs): plain {res['plain']:.3f} ms, batched {res['batched']:.3f} ms")
+    return {"pairs": 1 << 20, "elements": int(el.size), "plain_ms": res["plain"], "batched_ms": res["batched"],
+            "batched_over_plain": res["plain"] / res["batched"], "alg_bytes": st["alg_bytes"]}
+
+
 def read_traffic(kernel: str, workload: str):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     p = ROOT / "profiles" / f"{kernel}_{workload}_pmc.txt"
@@ -151,6 +174,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-si", action="store_true", help="skip the set-intersection microbench")
     ap.add_argument("--serve-queries", type=int, default=1 << 24)
     ap.add_argument("--serve-rate", type=float, default=2.0e6)
     ap.add_argument("--tune-cache", default=str(ROOT / "rt-graph_amd" / "tune_cache.json"),
@@ -220,6 +244,7 @@ def main() -> int:
         cpu = cpu_baseline(off, adj, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.4g} edges/s on {cpu['cores']} threads")
 
+    si = None if args.no_si else si_leg(rtg, ctx)
     serving = None
     if not args.no_serving:
         plan.close()
@@ -240,6 +265,8 @@ def main() -> int:
         "cpu_baseline": cpu,
         "plain_ms": plain, "batched_over_plain": (plain_ms / t_ms) if plain_ms else None,
         "t_build_ms": bst.t_total_ms, "t_plan_ms": plan.plan_stats["t_plan_ms"],
+        "t_end_to_end_ms": bst.t_total_ms + plan.plan_stats["t_plan_ms"] + t_ms,
+        "set_intersection": si,
         "wall_s_timed": wall, "alg_bytes": alg, "merge_bytes": st["merge_bytes"],
         "serving": serving,
     }

@@ -10,11 +10,16 @@
 //                 binary search over the samples in LDS, then over one sample interval in
 //                 HBM. The interval is clamped to the list (the reference's upper bound can
 //                 reach one element past the set, bs.cu:191).
-//   SI_BATCHED    a classify pass writes s = 0 pairs' results directly and appends the other
-//                 pairs to three size buckets; one persistent launch then drains the block
-//                 bucket (block per pair, sampled search as above), the wave bucket (wave per
-//                 pair) and the thread bucket (thread per pair), each through an atomic
-//                 cursor, so that no launch parameter depends on the bucket sizes.
+//   SI_BATCHED    a classify pass gives every pair whose shorter list is longer than 64
+//                 ceil(s/64) items of 64 of its elements (item counts scanned into offsets).
+//                 One launch then runs the items (a wave per item, a lane per element, one
+//                 binary search each, partial counts added into the result) and flat groups
+//                 of 64 consecutive pairs, whose shorter lists of 0..64 elements are
+//                 concatenated over the lanes so that every lane does one binary search per
+//                 round whatever the pairs' lengths. Every search in the launch is one binary
+//                 search deep: neither a long pair nor a run of length-1 pairs serialises a
+//                 chain of searches or leaves lanes idle. Work is assigned statically; no
+//                 counter is shared between waves.
 #include "device.hpp"
 #include "si.hpp"
 
@@ -161,103 +166,28 @@ __global__ __launch_bounds__(kThreads) void si_reference(Lists L, PairSrc P, int
 }
 
 // ---- batched ----
-struct Buckets {
-    uint32_t *list[3];                // pair indices: 0 thread, 1 wave, 2 block
-    unsigned long long *count;        // [0..2] bucket sizes, [3..5] drain cursors
+constexpr uint32_t kFlatMax = 64;  // longest shorter-list handled in flat groups
+
+struct Batch {
+    uint32_t *item_off;     // per pair: item count (s > kFlatMax), scanned in place
+    uint32_t *total_items;  // written by the scan
 };
 
-__device__ __forceinline__ void bucket_append(bool valid, uint32_t k, uint32_t *list, unsigned long long *counter) {
-    const uint64_t mask = __ballot(valid);
-    if (mask == 0) return;
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (valid) list[base + __popcll(mask & rtgd::lanemask_lt())] = k;
-}
-
-__global__ void si_reset(unsigned long long *c) {
-    if (threadIdx.x < 6) c[threadIdx.x] = 0ull;
-}
-
-__global__ __launch_bounds__(kThreads) void si_classify(Lists L, PairSrc P, int64_t npairs, Buckets B,
-                                                        uint32_t thread_max, uint32_t wave_max, uint32_t *__restrict__ out) {
-    const int64_t stride = (int64_t)gridDim.x * kThreads;
-    const int64_t start = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    for (int64_t base = start - (start & 63); base < npairs; base += stride) {
-        const int64_t k = base + (start & 63);
-        int bucket = -1;
-        if (k < npairs) {
-            uint32_t a, b;
-            P.get(k, a, b);
-            const uint32_t la = L.off[a + 1] - L.off[a], lb = L.off[b + 1] - L.off[b];
-            const uint32_t s = la < lb ? la : lb;
-            const uint32_t work = s;  // probes
-            if (s == 0) out[k] = 0u;
-            else bucket = (work <= thread_max) ? 0 : (work <= wave_max ? 1 : 2);
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) bucket_append(bucket == q, (uint32_t)k, B.list[q], &B.count[q]);
-    }
-}
-
-__global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, Buckets B, uint32_t *__restrict__ out) {
-    __shared__ uint32_t samp[kSamples];
-    __shared__ uint32_t red[kThreads / 64];
-    __shared__ unsigned long long slot;
-    // Block bucket: block per pair. Every grab is published through LDS and read back as a
-    // wave-uniform (scalar) value, so the loop below is uniform.
-    const uint64_t n2 = rtgd::uniform_u64(B.count[2]);
-    if (threadIdx.x == 0) slot = atomicAdd(&B.count[5], 1ull);
-    __syncthreads();
-    uint64_t i2 = rtgd::uniform_u64(slot);
-    while (i2 < n2) {
-        const uint32_t k = B.list[2][i2];
+// Per pair: the number of 64-element items of its shorter list when it is longer than
+// kFlatMax (those pairs' results are cleared here and accumulated by the items), else 0.
+__global__ __launch_bounds__(kThreads) void si_classify(Lists L, PairSrc P, int64_t npairs, Batch B,
+                                                        uint32_t *__restrict__ out) {
+    for (int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x; k < npairs; k += (int64_t)gridDim.x * kThreads) {
         uint32_t a, b;
         P.get(k, a, b);
-        const uint32_t c = block_pair(L, a, b, samp, red);  // ends with a barrier
-        if (threadIdx.x == 0) {
-            out[k] = c;
-            slot = atomicAdd(&B.count[5], 1ull);
-        }
-        __syncthreads();
-        i2 = rtgd::uniform_u64(slot);
-    }
-    // Wave bucket: wave per pair, 4 pairs per grab.
-    const uint64_t n1 = rtgd::uniform_u64(B.count[1]);
-    while (true) {
-        uint64_t g = 0;
-        if (__lane_id() == 0) g = atomicAdd(&B.count[4], 4ull);
-        const uint64_t i = rtgd::uniform_u64(g);
-        if (i >= n1) break;
-        const uint64_t e = (i + 4 < n1) ? i + 4 : n1;
-        for (uint64_t j = i; j < e; ++j) {
-            const uint32_t k = B.list[1][j];
-            uint32_t a, b;
-            P.get(k, a, b);
-            const uint32_t c = wave_pair(L, a, b);
-            if (__lane_id() == 0) out[k] = c;
-        }
-    }
-    // Thread bucket: thread per pair, a wave's worth per grab.
-    const uint64_t n0 = rtgd::uniform_u64(B.count[0]);
-    while (true) {
-        uint64_t g = 0;
-        if (__lane_id() == 0) g = atomicAdd(&B.count[3], 64ull);
-        const uint64_t base = rtgd::uniform_u64(g);
-        if (base >= n0) break;
-        const uint64_t i = base + __lane_id();
-        if (i < n0) {
-            const uint32_t k = B.list[0][i];
-            uint32_t a, b;
-            P.get(k, a, b);
-            out[k] = thread_pair(L, a, b);
-        }
+        const uint32_t la = L.off[a + 1] - L.off[a], lb = L.off[b + 1] - L.off[b];
+        const uint32_t sl = la < lb ? la : lb;
+        const uint32_t items = sl > kFlatMax ? (sl + 63u) / 64u : 0u;
+        if (items) out[k] = 0u;
+        B.item_off[k] = items;
     }
 }
 
-// ---- probe items (latency path) ----
 // Number of elements of l[0..nl) below v, nl >= 1, by binary lifting: the trip count depends
 // only on nl, and every probe address is clamped into the list.
 __device__ __forceinline__ uint32_t count_below_lift(const uint32_t *__restrict__ l, uint32_t nl, uint32_t v) {
@@ -269,6 +199,101 @@ __device__ __forceinline__ uint32_t count_below_lift(const uint32_t *__restrict_
     return base;
 }
 
+__device__ __forceinline__ uint32_t member(const uint32_t *__restrict__ l, uint32_t nl, uint32_t v) {
+    const uint32_t b = count_below_lift(l, nl, v);
+    return (b < nl && l[b] == v) ? 1u : 0u;
+}
+
+// Work is assigned statically (no shared queue: a single-address atomic per grab costs more
+// than the grab's work here). Items: wave w takes a contiguous run of items, so it locates
+// its first pair by one binary search over item_off and walks forward. Flat groups: wave w
+// takes pairs 64g .. 64g+63 for g = w, w + waves, ...; pairs whose shorter list is 1..64 long
+// concatenate their elements over the lanes (a lane per element per round, one binary search
+// each), pairs with an empty list get 0, longer pairs are left to the items.
+__global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, Batch B, int64_t npairs,
+                                                     uint32_t *__restrict__ out) {
+    constexpr int kWaves = kThreads / 64;
+    __shared__ uint32_t ex_all[kWaves][65];   // exclusive element offsets of the group's pairs
+    __shared__ uint32_t sh_all[kWaves][64];   // shorter list: start in elements
+    __shared__ uint32_t lo_all[kWaves][64];   // longer list: start in elements
+    __shared__ uint32_t ln_all[kWaves][64];   // longer list: length
+    __shared__ uint32_t cnt_all[kWaves][64];  // per-pair counts
+    const int lane = __lane_id();
+    const int wave = (int)rtgd::uniform_u32(threadIdx.x >> 6);
+    uint32_t *ex = ex_all[wave], *sh = sh_all[wave], *lo = lo_all[wave], *ln = ln_all[wave], *cnt = cnt_all[wave];
+    const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+    const int64_t gw = (int64_t)blockIdx.x * kWaves + wave;
+
+    // Items of the long pairs: item i belongs to the last pair k with item_off[k] <= i and
+    // covers elements 64*(i - item_off[k]) .. +63 of its shorter list.
+    const int64_t nitems = rtgd::uniform_u32(*B.total_items);
+    const int64_t per = (nitems + nwaves - 1) / nwaves;
+    const int64_t i0 = gw * per, i1 = min(nitems, i0 + per);
+    int64_t k = -1;
+    for (int64_t i = i0; i < i1; ++i) {
+        if (k < 0 || (k + 1 < npairs && rtgd::uniform_u32(B.item_off[k + 1]) <= (uint64_t)i)) {
+            int64_t a0 = k + 1, a1 = npairs;  // upper_bound(item_off, i) - 1; item_off[0] = 0
+            while (a0 < a1) {
+                const int64_t mid = (a0 + a1) >> 1;
+                if (rtgd::uniform_u32(B.item_off[mid]) <= (uint64_t)i) a0 = mid + 1;
+                else a1 = mid;
+            }
+            k = a0 - 1;
+        }
+        uint32_t a, b;
+        P.get(k, a, b);
+        const uint32_t *s, *l;
+        uint32_t ns, nl;
+        short_long(L, a, b, s, ns, l, nl);
+        const uint32_t e = 64u * (uint32_t)(i - B.item_off[k]) + (uint32_t)lane;
+        uint32_t c = e < ns ? member(l, nl, s[e]) : 0u;
+        c = rtgd::wave_sum_u32(c);
+        if (lane == 0 && c) atomicAdd(&out[k], c);
+    }
+
+    // Flat groups.
+    const int64_t ngroups = (npairs + 63) / 64;
+    for (int64_t grp = gw; grp < ngroups; grp += nwaves) {
+        const int64_t kk = grp * 64 + lane;
+        const bool in = kk < npairs;
+        uint32_t ns = 0, s0 = 0, l0 = 0, nl = 0;
+        bool mine = false;  // this lane's pair is answered by the group
+        if (in) {
+            uint32_t a, b;
+            P.get(kk, a, b);
+            const uint32_t ba = L.off[a], la = L.off[a + 1] - ba, bb = L.off[b], lb = L.off[b + 1] - bb;
+            if (la <= lb) { s0 = ba; ns = la; l0 = bb; nl = lb; }
+            else { s0 = bb; ns = lb; l0 = ba; nl = la; }
+            mine = ns <= kFlatMax;
+            if (!mine) ns = 0;
+        }
+        const uint32_t incl = (uint32_t)rtgd::wave_inclusive_scan((int32_t)ns);
+        const uint32_t total = rtgd::uniform_u32(__builtin_amdgcn_readlane(incl, 63));
+        ex[lane] = incl - ns;
+        sh[lane] = s0;
+        lo[lane] = l0;
+        ln[lane] = nl;
+        cnt[lane] = 0u;
+        if (lane == 0) ex[64] = total;
+        rtgd::wave_sync_lds();
+        for (uint32_t e0 = 0; e0 < total; e0 += 64) {
+            const uint32_t e = e0 + (uint32_t)lane;
+            if (e < total) {
+                uint32_t p = 0;  // last pair whose first element is at or before e
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1)
+                    if (ex[p + step] <= e) p += step;
+                const uint32_t v = L.el[sh[p] + (e - ex[p])];
+                if (member(L.el + lo[p], ln[p], v)) atomicAdd(&cnt[p], 1u);
+            }
+        }
+        rtgd::wave_sync_lds();
+        if (mine) out[kk] = cnt[lane];
+        rtgd::wave_sync_lds();  // the group's LDS slots are rewritten by the next group
+    }
+}
+
+// ---- probe items (latency path) ----
 // items[i] = {k, first}: one wave takes elements first .. first+63 of pair k's shorter list,
 // one lane per element and one binary search of the longer list each, and adds its count to
 // out[k] (cleared beforehand). Splitting every pair into 64-element items puts all of a
@@ -289,15 +314,14 @@ __global__ __launch_bounds__(kThreads) void si_items(Lists L, const uint2 *__res
         uint32_t c = 0;
         if (i < ns) {
             const uint32_t v = s[i];
-            const uint32_t b = count_below_lift(l, nl, v);
-            c = (b < nl && l[b] == v) ? 1u : 0u;
+            c = member(l, nl, v);
         }
         c = rtgd::wave_sum_u32(c);
         if (__lane_id() == 0 && c) atomicAdd(&out[item.x], c);
     }
 }
 
-__global__ void si_alg_bytes(Lists L, PairSrc P, int64_t npairs, unsigned long long *acc) {
+__global__ __launch_bounds__(kThreads) void si_alg_bytes(Lists L, PairSrc P, int64_t npairs, unsigned long long *acc) {
     uint64_t s = 0;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += (int64_t)gridDim.x * blockDim.x) {
         uint32_t a, b;
@@ -308,8 +332,16 @@ __global__ void si_alg_bytes(Lists L, PairSrc P, int64_t npairs, unsigned long l
         const uint64_t merge = sm + lg, bs = sm * (1 + bits);
         s += 20 + 4 * (merge < bs ? merge : bs);
     }
+    // one atomic per block: same-address atomics from every wave would serialise
+    __shared__ uint64_t part[kThreads / 64];
     s = rtgd::wave_sum_u64(s);
-    if (__lane_id() == 0 && s) atomicAdd(acc, (unsigned long long)s);
+    if (__lane_id() == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kThreads / 64; ++w) t += part[w];
+        if (t) atomicAdd(acc, (unsigned long long)t);
+    }
 }
 
 unsigned grid_for(rtg_ctx *ctx, int64_t n, int per_cu) {
@@ -333,13 +365,16 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
     if (npairs <= 0) return RTG_OK;
     if (npairs >= ((int64_t)1 << 32)) return set_error(RTG_EINVAL, "rtg_si: more than 2^32 pairs");
     Free fb;
-    Buckets B{};
+    Batch B{};
+    void *scan_work = nullptr;
     if (method == RTG_SI_BATCHED) {
-        const size_t bytes = Arena::round(sizeof(uint32_t) * (size_t)npairs) * 3 + 1024;
+        const size_t sw = scan_work_bytes(npairs);
+        const size_t bytes = Arena::round(sizeof(uint32_t) * (size_t)npairs) + Arena::round(sw) + 1024;
         HIP_TRY(hipMalloc(&fb.p, bytes));
         Arena ar{(char *)fb.p, 0, bytes};
-        for (auto &l : B.list) l = ar.take<uint32_t>((size_t)npairs);
-        B.count = ar.take<unsigned long long>(8);
+        B.item_off = ar.take<uint32_t>((size_t)npairs);
+        B.total_items = ar.take<uint32_t>(4);
+        scan_work = ar.take<char>(sw);
     }
     HIP_TRY(hipEventRecord(ctx->ev[0], st));
     switch (method) {
@@ -350,9 +385,9 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
             si_reference<<<(unsigned)std::min<int64_t>(npairs, (int64_t)ctx->cu_count * 8), kThreads, 0, st>>>(L, P, npairs, d_out);
             break;
         case RTG_SI_BATCHED: {
-            si_reset<<<1, 64, 0, st>>>(B.count);
-            si_classify<<<grid_for(ctx, npairs, 16), kThreads, 0, st>>>(L, P, npairs, B, 16u, 2048u, d_out);
-            si_drain<<<(unsigned)((int64_t)ctx->cu_count * 8), kThreads, 0, st>>>(L, P, B, d_out);
+            si_classify<<<grid_for(ctx, npairs, 16), kThreads, 0, st>>>(L, P, npairs, B, d_out);
+            RTG_TRY(scan_exclusive_u32_w(ctx, B.item_off, B.item_off, npairs, B.total_items, scan_work));
+            si_drain<<<(unsigned)((int64_t)ctx->cu_count * 8), kThreads, 0, st>>>(L, P, B, npairs, d_out);
             break;
         }
         default:
@@ -364,17 +399,18 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
         void *s = nullptr;
         RTG_TRY(ctx_scratch(ctx, 256, &s));
         HIP_TRY(hipMemsetAsync(s, 0, sizeof(unsigned long long), st));
-        si_alg_bytes<<<grid_for(ctx, npairs, 8), kThreads, 0, st>>>(L, P, npairs, (unsigned long long *)s);
+        si_alg_bytes<<<grid_for(ctx, npairs, 2), kThreads, 0, st>>>(L, P, npairs, (unsigned long long *)s);
         HIP_TRY(hipGetLastError());
-        unsigned long long ab = 0, cnt[3] = {0, 0, 0};
+        unsigned long long ab = 0;
+        uint32_t nitems = 0;
         HIP_TRY(hipMemcpyAsync(&ab, s, sizeof(ab), hipMemcpyDeviceToHost, st));
-        if (method == RTG_SI_BATCHED) HIP_TRY(hipMemcpyAsync(cnt, B.count, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        if (method == RTG_SI_BATCHED) HIP_TRY(hipMemcpyAsync(&nitems, B.total_items, sizeof(nitems), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         stats->t_kernel_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
         stats->alg_bytes = ab;
-        if (method == RTG_SI_BATCHED) {
-            stats->nbuckets = 3;
-            for (int q = 0; q < 3; ++q) stats->bucket_items[q] = (int64_t)cnt[q];
+        if (method == RTG_SI_BATCHED) {  // 64-element items of the pairs answered by items
+            stats->nbuckets = 1;
+            stats->bucket_items[0] = (int64_t)nitems;
         }
     } else {
         HIP_TRY(hipStreamSynchronize(st));
@@ -385,27 +421,6 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
 }  // namespace
 
 namespace rtg {
-
-size_t si_batched_work_bytes(int64_t npairs) {
-    return Arena::round(sizeof(uint32_t) * (size_t)std::max<int64_t>(npairs, 1)) * 3 + 1024;
-}
-
-int si_launch_batched(rtg_ctx *ctx, hipStream_t st, const uint32_t *d_off, const uint32_t *d_el,
-                      const uint32_t *d_pairs, int64_t npairs, void *work, uint32_t *d_out) {
-    if (npairs <= 0) return RTG_OK;
-    Lists L{d_off, d_el};
-    PairSrc P{d_pairs, 0, 1};
-    Arena ar{(char *)work, 0, si_batched_work_bytes(npairs)};
-    Buckets B{};
-    for (auto &l : B.list) l = ar.take<uint32_t>((size_t)npairs);
-    B.count = ar.take<unsigned long long>(8);
-    si_reset<<<1, 64, 0, st>>>(B.count);
-    si_classify<<<grid_for(ctx, npairs, 16), kThreads, 0, st>>>(L, P, npairs, B, 16u, 2048u, d_out);
-    const unsigned drain = (unsigned)std::min<int64_t>((int64_t)ctx->cu_count * 8, std::max<int64_t>(npairs, 1));
-    si_drain<<<drain, kThreads, 0, st>>>(L, P, B, d_out);
-    HIP_TRY(hipGetLastError());
-    return RTG_OK;
-}
 
 int si_launch_items(rtg_ctx *ctx, hipStream_t st, const uint32_t *d_off, const uint32_t *d_el, const uint32_t *d_pairs,
                     const uint32_t *d_items, int64_t nitems, uint32_t *d_out) {

@@ -210,12 +210,6 @@ __device__ __forceinline__ void table_seed(Table &c, int r) {
     c.k1 |= 1u;
 }
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Block-wide build of the owner table from adj[xb .. xb+d). Returns the table (uniform).
 template <int kThreads>
 __device__ Table build_block_table(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d,
@@ -248,17 +242,17 @@ __device__ Table build_wave_table(uint32_t *tab, const int32_t *__restrict__ adj
     for (int r = 0; r < kRetries; ++r) {
         table_seed(c, r);
         for (uint32_t i = lane; i < 2u * c.t; i += 64) tab[i] = kEmpty;
-        wave_sync_lds();
+        rtgd::wave_sync_lds();
         bool ok = true;
         for (int32_t i = lane; i < d; i += 64) ok &= cuckoo_insert(c, (uint32_t)adj[xb + i]);
-        wave_sync_lds();
+        rtgd::wave_sync_lds();
         if (__ballot(!ok) == 0ull) return c;
     }
     c.linear = 1u;
     for (uint32_t i = lane; i < 2u * c.t; i += 64) tab[i] = kEmpty;
-    wave_sync_lds();
+    rtgd::wave_sync_lds();
     for (int32_t i = lane; i < d; i += 64) linear_insert(c, (uint32_t)adj[xb + i]);
-    wave_sync_lds();
+    rtgd::wave_sync_lds();
     return c;
 }
 
@@ -317,7 +311,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
                 st[ci] = sp.x;
             }
             if (lane == 0) ex[ns] = total;
-            wave_sync_lds();
+            rtgd::wave_sync_lds();
             int k = 0;
             for (int32_t jb = 0; jb < total; jb += 64 * U) {
                 uint32_t x[U];
@@ -334,7 +328,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
                 for (int u = 0; u < U; ++u)
                     if (x[u] != kEmpty) hits += probe<kLinear>(c, x[u]);
             }
-            wave_sync_lds();
+            rtgd::wave_sync_lds();
         }
     }
     return hits;
@@ -405,7 +399,7 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 const Table c = build_wave_table(tab, a.adj, xb, d);
                 if (c.linear) hits += stream_spans<true, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
                 else hits += stream_spans<false, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
-                wave_sync_lds();
+                rtgd::wave_sync_lds();
             }
         }
         __syncthreads();  // every wave is done with the table, item_slot and next_batch
