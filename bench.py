@@ -58,32 +58,51 @@ def golden_triangles(workload: str):
         return None
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(off: np.ndarray, adj: np.ndarray, target_s: float) -> dict:
-    """Times the oracle's 1A2 count on a sample of edge ranges spread over the CSR."""
+    """Times the oracle's 1A2 count on a sample of edge ranges spread over the CSR: all host
+    threads on a sample of about target_s seconds, then one thread on about target_s / 3."""
     sys.path.insert(0, str(ROOT))
     from oracle import oracle as orc  # checker / CPU baseline only
     m = adj.size
     threads = orc.max_threads()
     nr = 64
-    # calibration on a 0.2% sample
-    def run(frac: float):
+
+    def run(frac: float, nthreads: int):
         per = max(1, int(m * frac / nr))
         t0 = time.perf_counter()
-        tot = 0
         for r in range(nr):
             b = (m * r) // nr
-            tot += orc.tc_count_1a2(off, adj, b, min(m, b + per), threads)
+            orc.tc_count_1a2(off, adj, b, min(m, b + per), nthreads)
         return per * nr, time.perf_counter() - t0
-    e0, t0 = run(0.002)
-    rate = e0 / max(t0, 1e-6)
-    frac = min(1.0, max(0.0005, target_s * rate / m))
-    edges, secs = run(frac)
-    if secs < 0.5 * target_s and frac < 1.0:  # small samples overestimate the rate
-        frac = min(1.0, frac * target_s / max(secs, 1e-3))
-        edges, secs = run(frac)
+
+    def sized(target: float, nthreads: int):
+        e0, t0 = run(0.002 / max(1, threads // nthreads), nthreads)  # calibration
+        frac = min(1.0, max(0.0002, target * e0 / max(t0, 1e-6) / m))
+        edges, secs = run(frac, nthreads)
+        for _ in range(3):  # small samples misjudge the rate: resize until the sample is long enough
+            if secs >= 0.67 * target or frac >= 1.0:
+                break
+            frac = min(1.0, frac * target / max(secs, 1e-3))
+            edges, secs = run(frac, nthreads)
+        return edges, secs
+
+    edges, secs = sized(target_s, threads)
+    e1, s1 = sized(target_s / 3, 1)
     return {"value": edges / secs, "unit": "edges/s", "cores": threads, "kind": "port",
+            "single_thread_value": e1 / s1, "cpu_model": cpu_model(),
             "sample": f"1A2 (ListIntersectionKernel restated, OpenMP) over {nr} edge ranges spread across the "
-                      f"CSR, {edges} of {m} oriented edges, {secs:.1f} s"}
+                      f"CSR, {edges} of {m} oriented edges, {secs:.1f} s on {threads} threads; single thread "
+                      f"{e1} edges, {s1:.1f} s"}
 
 
 def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
@@ -169,7 +188,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="rmat_s21_ef44", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
