@@ -126,7 +126,7 @@ def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
         ix.run(d_keys, nk)
         lk_ms = sorted(ix.run(d_keys, nk)[1] for _ in range(5))[2]
     d_keys.free()
-    srv = rtg.Server(ctx, g, d_arr, 1 << 28, slots=4)
+    srv = rtg.Server(ctx, g, d_arr, 1 << 28, slots=2)
     warm = min(1 << 16, nqueries // 8)
     st, _, _ = srv.replay(rate_qps=rate, nqueries=nqueries + warm, warmup=warm, pair_frac=0.7, max_keys=4096, seed=1)
     srv.close()
@@ -139,7 +139,8 @@ def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
            **{k: st[k] for k in ("offered_qps", "throughput_qps", "p50_us", "p99_us", "p999_us", "mean_us", "max_us",
                                   "admitted", "rejected", "completed", "cancelled", "failed", "batches",
                                   "mean_batch_requests", "max_busy_slots", "mean_batch_device_us",
-                                  "mean_batch_pack_us", "mean_batch_unpack_us", "mean_batch_slot_us")}}
+                                  "mean_batch_pack_us", "mean_batch_unpack_us", "mean_batch_slot_us",
+                                  "max_batch_device_us", "max_batch_slot_us")}}
     g.close()
     d_arr.free()
     return out

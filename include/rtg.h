@@ -231,7 +231,7 @@ int rtg_lookup_index_destroy(rtg_lookup_index *ix);
 typedef struct rtg_server rtg_server;
 
 typedef struct rtg_server_opts {
-    int32_t slots;              /* concurrent batches (streams); default 4                      */
+    int32_t slots;              /* concurrent batches (streams); default 2                      */
     int32_t max_batch_requests; /* requests per batch; default 8192                             */
     int64_t max_batch_pairs;    /* pair queries per batch; default 8192                         */
     int64_t max_batch_keys;     /* lookup keys per batch; default 2^21                          */
@@ -253,6 +253,8 @@ typedef struct rtg_server_stats {
     double mean_batch_pack_us;    /* host time packing a batch into staging buffers            */
     double mean_batch_unpack_us;  /* host time delivering a batch's results                    */
     double mean_batch_slot_us;    /* launch -> results delivered                               */
+    double max_batch_device_us;   /* longest batch on the device                               */
+    double max_batch_slot_us;     /* longest batch from launch to results delivered            */
     char last_error[128];
 } rtg_server_stats;
 

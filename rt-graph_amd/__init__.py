@@ -688,6 +688,7 @@ class ServerStats(C.Structure):
                 ("throughput_qps", C.c_double), ("offered_qps", C.c_double),
                 ("mean_batch_device_us", C.c_double), ("mean_batch_pack_us", C.c_double),
                 ("mean_batch_unpack_us", C.c_double), ("mean_batch_slot_us", C.c_double),
+                ("max_batch_device_us", C.c_double), ("max_batch_slot_us", C.c_double),
                 ("last_error", C.c_char * 128)]
 
     def as_dict(self) -> dict:
