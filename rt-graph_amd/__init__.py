@@ -31,7 +31,7 @@ __all__ = [
     "read_dataset", "write_dataset", "TC_PLAIN_THREAD", "TC_PLAIN_WAVE", "TC_BATCHED", "TC_2A1",
     "SI_PLAIN", "SI_BATCHED", "SI_REFERENCE", "gen_ba", "gen_rmat", "gen_powerlaw_degrees",
     "gen_config_model", "gen_zipf_lengths", "gen_sorted_lists", "gen_sorted_array",
-    "gen_uniform_u32", "read_binary_pairs", "lib_path",
+    "gen_uniform_u32", "read_binary_pairs", "read_text_pairs", "read_mtx_pairs", "lib_path",
 ]
 
 _HERE = Path(__file__).resolve().parent
@@ -343,6 +343,27 @@ def read_binary_pairs(path: str | os.PathLike) -> np.ndarray:
     return np.fromfile(path, dtype="<i4", count=(size // 8) * 2).astype(np.int32, copy=False)
 
 
+def _read_pairs(fn, path) -> np.ndarray:
+    p = str(path).encode()
+    n = C.c_int64()
+    _check(fn(p, None, C.byref(n)))
+    out = np.empty(2 * n.value, dtype=np.int32)
+    _check(fn(p, _ptr(out) if out.size else None, C.byref(n)))
+    return out[: 2 * n.value]
+
+
+def read_text_pairs(path: str | os.PathLike) -> np.ndarray:
+    """rtg_read_text_pairs: "src dst" per line; lines that do not start with two int32 values are
+    skipped with a message (Graph::Txt2Bin, tc/graph/graph.cpp:179-214)."""
+    return _read_pairs(_lib.rtg_read_text_pairs, path)
+
+
+def read_mtx_pairs(path: str | os.PathLike) -> np.ndarray:
+    """rtg_read_mtx_pairs: Matrix Market coordinate entries as 0-based pairs, symmetric files
+    expanded (bfs/graph/graph.cpp:17-122)."""
+    return _read_pairs(_lib.rtg_read_mtx_pairs, path)
+
+
 class Graph:
     """Mirror of the reference's Graph (tc/graph/include/graph.h:10-30).
 
@@ -396,18 +417,9 @@ class Graph:
     @staticmethod
     def Txt2Bin(txt_file, bin_file) -> None:
         """Text edge list ("a b" per line) to binary int32 pairs (graph.cpp:179-214)."""
-        out = []
-        with open(txt_file) as f:
-            for line in f:
-                parts = line.split()
-                if len(parts) >= 2:
-                    try:
-                        out.append((int(parts[0]), int(parts[1])))
-                        continue
-                    except ValueError:
-                        pass
-                print(f"Error parsing line: {line.rstrip()}", file=sys.stderr)
-        np.asarray(out, dtype="<i4").reshape(-1).tofile(bin_file)
+        pairs = read_text_pairs(txt_file)
+        _check(_lib.rtg_write_binary_pairs(str(bin_file).encode(), _ptr(pairs) if pairs.size else None,
+                                           C.c_int64(pairs.size // 2)))
         print("Edge list has been successfully converted to binary.")
 
 

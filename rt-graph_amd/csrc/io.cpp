@@ -12,6 +12,7 @@
 #include "rtg_internal.hpp"
 
 #include <cctype>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -27,10 +28,10 @@ int read_text_pairs(const char *path, std::vector<int32_t> &out) {
     std::string line;
     while (std::getline(in, line)) {
         std::stringstream ss(line);
-        long long a, b;
+        int32_t a, b;  // as the reference's `int` extraction: an out-of-range value fails the line
         if (ss >> a >> b) {
-            out.push_back((int32_t)a);
-            out.push_back((int32_t)b);
+            out.push_back(a);
+            out.push_back(b);
         } else if (!line.empty() && line[0] != '#' && line[0] != '%') {
             std::fprintf(stderr, "Error parsing line: %s\n", line.c_str());
         }
@@ -57,11 +58,17 @@ int read_mtx_pairs(const char *path, std::vector<int32_t> &out, int32_t *rows_ou
     const bool symmetric = banner.find("symmetric") != std::string::npos;
     if (!pattern && !real) return rtg_fail(RTG_EIO, "Unrecognized matrix market format type");
     long long rows = 0, cols = 0, nnz = 0;
+    bool have_size = false;
     while (std::fgets(buf, sizeof(buf), f)) {
         if (buf[0] == '%') continue;
-        if (std::sscanf(buf, "%lld %lld %lld", &rows, &cols, &nnz) == 3) break;
+        if (std::sscanf(buf, "%lld %lld %lld", &rows, &cols, &nnz) == 3) {
+            have_size = true;
+            break;
+        }
         return rtg_fail(RTG_EIO, "Could not read file info (M, N, NNZ)");
     }
+    if (!have_size || rows < 0 || cols < 0 || nnz < 0 || rows > INT32_MAX || cols > INT32_MAX)
+        return rtg_fail(RTG_EIO, "Could not read file info (M, N, NNZ)");
     out.reserve((size_t)nnz * (symmetric ? 4 : 2));
     for (long long i = 0; i < nnz; ++i) {
         long long r = 0, c = 0;
@@ -69,6 +76,9 @@ int read_mtx_pairs(const char *path, std::vector<int32_t> &out, int32_t *rows_ou
         int got = pattern ? std::fscanf(f, " %lld %lld", &r, &c) : std::fscanf(f, " %lld %lld %lf", &r, &c, &w);
         if (got != (pattern ? 2 : 3)) return rtg_fail(RTG_EIO, "Could not read edge from market file");
         if (r == 0 || c == 0) return rtg_fail(RTG_EIO, "Market file is zero-indexed");
+        // The reference stores such entries unchecked (bfs/graph/graph.cpp:53-54, 74-75) and
+        // then indexes its CSR with them; they are rejected here.
+        if (r < 0 || c < 0 || r > rows || c > cols) return rtg_fail(RTG_EIO, "Market file entry outside the matrix");
         out.push_back((int32_t)(r - 1));
         out.push_back((int32_t)(c - 1));
         if (symmetric && r != c) {
@@ -85,7 +95,7 @@ int deliver(const std::vector<int32_t> &v, int32_t *pairs, int64_t *npairs) {
     const int64_t n = (int64_t)v.size() / 2;
     if (pairs) {
         if (*npairs < n) return rtg_fail(RTG_EINVAL, "pairs buffer too small");
-        std::memcpy(pairs, v.data(), sizeof(int32_t) * v.size());
+        if (!v.empty()) std::memcpy(pairs, v.data(), sizeof(int32_t) * v.size());
     }
     *npairs = n;
     return RTG_OK;

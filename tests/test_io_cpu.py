@@ -1,0 +1,71 @@
+"""Host-side ingest of the reference's file formats (no GPU): binary pairs
+(tc/graph/graph.cpp:23-39), text edge lists (Txt2Bin, graph.cpp:179-214) and Matrix Market
+coordinate files (bfs/graph/graph.cpp:17-122), on hand-written fixtures and error cases."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_rtg
+
+rtg = load_rtg()
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_bytes(text.encode() if isinstance(text, str) else text)
+    return p
+
+
+def test_binary_pairs_round_trip_and_odd_sizes(tmp_path):
+    pairs = np.array([1, 2, 3, 4, -1, 7], np.int32)
+    p = tmp_path / "g.bin"
+    pairs.astype("<i4").tofile(p)
+    assert np.array_equal(rtg.read_binary_pairs(p), pairs)
+    # a trailing partial pair is ignored: edges = file_size / 8
+    write(tmp_path, "odd.bin", pairs.astype("<i4").tobytes() + b"\x01\x02\x03")
+    assert np.array_equal(rtg.read_binary_pairs(tmp_path / "odd.bin"), pairs)
+    assert rtg.read_binary_pairs(write(tmp_path, "e.bin", b"")).size == 0
+
+
+def test_text_pairs_like_txt2bin(tmp_path):
+    p = write(tmp_path, "g.txt", "1 2\n# comment\ngarbage\n3\n4 5 6\n\n99999999999 1\n-5 7\n 8\t9\n")
+    # lines that do not begin with two int32 values are skipped; extra columns are ignored
+    assert rtg.read_text_pairs(p).tolist() == [1, 2, 4, 5, -5, 7, 8, 9]
+    assert rtg.read_text_pairs(write(tmp_path, "e.txt", "")).size == 0
+    out = tmp_path / "g.bin"
+    rtg.Graph.Txt2Bin(p, out)
+    assert rtg.read_binary_pairs(out).tolist() == [1, 2, 4, 5, -5, 7, 8, 9]
+
+
+def test_mtx_pattern_symmetric_expands_off_diagonal(tmp_path):
+    p = write(tmp_path, "s.mtx", "%%MatrixMarket matrix coordinate pattern symmetric\n% c\n3 3 3\n1 2\n2 3\n3 3\n")
+    # (r, c) then (c, r) for off-diagonal entries, diagonal once, 1-based -> 0-based
+    assert rtg.read_mtx_pairs(p).tolist() == [0, 1, 1, 0, 1, 2, 2, 1, 2, 2]
+
+
+def test_mtx_real_general_reads_nnz_entries(tmp_path):
+    p = write(tmp_path, "r.mtx", "%%MatrixMarket matrix coordinate real general\n3 3 2\n1 2 0.5\n3 1 -2\n2 2 9\n")
+    assert rtg.read_mtx_pairs(p).tolist() == [0, 1, 2, 0]  # the header's nnz = 2 entries
+
+
+@pytest.mark.parametrize("text", [
+    "%%MatrixMarket matrix coordinate real general\n3 3 2\n1 2 0.5\n",          # fewer entries than nnz
+    "%%MatrixMarket matrix coordinate integer general\n2 2 1\n0 1 1\n",         # zero-indexed
+    "%%MatrixMarket matrix coordinate pattern general\n2 2 1\n3 1\n",           # outside the matrix
+    "%%MatrixMarket matrix coordinate pattern general\n2 2 1\n" + "1" * 3000 + " 2\n",
+    "%%MatrixMarket matrix array real general\n2 2\n1\n2\n3\n4\n",              # dense array
+    "%%MatrixMarket matrix coordinate complex general\n2 2 1\n1 1 1 1\n",       # unsupported field
+    "%%MatrixMarket matrix coordinate pattern general\n",                       # no size line
+    "not a banner\n1 1 1\n1 1\n",
+], ids=["short", "zero-indexed", "outside", "overflow", "array", "complex", "no-size", "no-banner"])
+def test_mtx_errors(tmp_path, text):
+    with pytest.raises(rtg.RTGError) as e:
+        rtg.read_mtx_pairs(write(tmp_path, "bad.mtx", text))
+    assert e.value.code == 4  # RTG_EIO
+
+
+def test_missing_files(tmp_path):
+    for fn in (rtg.read_text_pairs, rtg.read_mtx_pairs):
+        with pytest.raises(rtg.RTGError):
+            fn(tmp_path / "missing")
