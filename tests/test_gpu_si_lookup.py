@@ -201,3 +201,20 @@ def test_lookup_index_full_size(ctx):
         a, _ = ix.run(d_big, big.size)
         b, _ = rtg.lookup(ctx, d_data, n, d_big, big.size)
         assert np.array_equal(a.numpy(), b.numpy())
+
+
+@pytest.mark.parametrize("n,q", [(300_000, 40_000), (300_000, 2_000), (5_000, 3)])
+def test_lookup_stateless_both_forms(ctx, n, q):
+    """rtg_lookup_u32 searches through a transient index when the keys are many (q >= n/128) or
+    the array is small, and by the sampled kernel otherwise or when the array is not 16-byte
+    aligned; every form matches the oracle."""
+    rng = np.random.default_rng(n + q)
+    data = np.sort(rng.integers(0, 1 << 32, size=n + 1, dtype=np.uint64)).astype(np.uint32)
+    keys = np.concatenate([rng.integers(0, 1 << 32, size=q, dtype=np.uint64).astype(np.uint32),
+                           data[rng.integers(0, n + 1, size=q)]])
+    d_data, d_keys = ctx.to_device(data), ctx.to_device(keys)
+    out, _ = rtg.lookup(ctx, d_data, n + 1, d_keys, keys.size)
+    assert np.array_equal(out.numpy(), orc.lookup(data, keys))
+    # the same array seen from its second element: 4-byte aligned only
+    out2, _ = rtg.lookup(ctx, d_data.ptr + 4, n, d_keys, keys.size)
+    assert np.array_equal(out2.numpy(), orc.lookup(data[1:], keys))
