@@ -20,6 +20,7 @@
 #include "tc_kernels.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -598,6 +599,30 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     return RTG_OK;
 }
 
+// Indices 0 .. key.size()-1 in descending order of key, keys within 1/16 of an octave of each
+// other (and ties) in index order: a counting sort on 16*log2(key), linear in the count. The
+// work queue only needs its heaviest items roughly first.
+std::vector<int64_t> order_heaviest_first(const std::vector<uint64_t> &key) {
+    constexpr int kSub = 16, kBuckets = 1 + 64 * kSub;
+    const auto bucket = [](uint64_t k) -> int {
+        if (k == 0) return 0;
+        const int e = 63 - __builtin_clzll(k);
+        const int frac = e >= 4 ? (int)((k >> (e - 4)) & 15u) : (int)((k << (4 - e)) & 15u);
+        return 1 + e * kSub + frac;
+    };
+    std::vector<int64_t> start(kBuckets, 0);
+    for (uint64_t k : key) ++start[bucket(k)];
+    int64_t acc = 0;
+    for (int b = kBuckets - 1; b >= 0; --b) {
+        const int64_t c = start[b];
+        start[b] = acc;
+        acc += c;
+    }
+    std::vector<int64_t> out(key.size());
+    for (size_t i = 0; i < key.size(); ++i) out[start[bucket(key[i])]++] = (int64_t)i;
+    return out;
+}
+
 // Tune-dependent part: table sizes, the work queue (heaviest first) and the kernel instance.
 int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
     using namespace rtg;
@@ -642,13 +667,17 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
         }
     }
     // Heaviest first.
-    std::vector<int64_t> order(items.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int64_t l, int64_t r) { return item_cost[l] > item_cost[r]; });
     std::vector<TcItem> queue;
     queue.reserve(items.size() + small.size() / 4 + 1);
-    for (int64_t i : order) queue.push_back(items[i]);
-    std::stable_sort(small.begin(), small.end(), [](const auto &l, const auto &r) { return l.first > r.first; });
+    for (int64_t i : order_heaviest_first(item_cost)) queue.push_back(items[i]);
+    {
+        std::vector<uint64_t> sw(small.size());
+        for (size_t i = 0; i < small.size(); ++i) sw[i] = small[i].first;
+        const std::vector<int64_t> so = order_heaviest_first(sw);
+        std::vector<std::pair<uint64_t, int32_t>> sorted(small.size());
+        for (size_t i = 0; i < small.size(); ++i) sorted[i] = small[so[i]];
+        small.swap(sorted);
+    }
     std::vector<int32_t> small_list(small.size());
     for (size_t i = 0; i < small.size(); ++i) small_list[i] = small[i].second;
     // Wave items: groups of `waves` or more consecutive small owners with bounded total work.
@@ -803,7 +832,12 @@ int autotune(rtg_ctx *ctx, rtg_tc_plan *p, BatchTune *best) {
 // Chooses the tune for a batched plan: RTG_TC_TUNE, else the cache entry, else the
 // autotuner (when asked for), else the defaults; then builds the work queue for it.
 int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p) {
+    using Clk = std::chrono::steady_clock;
+    const auto ms_since = [](Clk::time_point t) { return std::chrono::duration<double, std::milli>(Clk::now() - t).count(); };
+    auto t0 = Clk::now();
     RTG_TRY(plan_graph(ctx, p));
+    p->t_plan_graph_ms = ms_since(t0);
+    t0 = Clk::now();
     BatchTune tune;
     const char *env = std::getenv("RTG_TC_TUNE");
     const std::string key = tune_key(ctx, p);
@@ -828,7 +862,11 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p) {
             RTG_TRY(write_tune_cache(p->opts.tune_cache, cache));
         }
     }
-    return plan_items(ctx, p, tune);
+    p->t_plan_tune_ms = ms_since(t0);
+    t0 = Clk::now();
+    const int rc = plan_items(ctx, p, tune);
+    p->t_plan_items_ms = ms_since(t0);
+    return rc;
 }
 
 int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
@@ -973,7 +1011,9 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
     if (p->opts.method == RTG_TC_BATCHED)
         d += "," + p->tune + ",grid=" + std::to_string(p->grid) + ",table_slots=" + std::to_string(p->max_table) +
              ",wave_owner_max=" + std::to_string(p->wave_owner_max) + ",source=" +
-             (p->autotuned ? "autotune" : (p->tune_from_cache ? "cache" : (std::getenv("RTG_TC_TUNE") ? "env" : "default")));
+             (p->autotuned ? "autotune" : (p->tune_from_cache ? "cache" : (std::getenv("RTG_TC_TUNE") ? "env" : "default"))) +
+             ",plan_graph_ms=" + std::to_string(p->t_plan_graph_ms) + ",plan_tune_ms=" + std::to_string(p->t_plan_tune_ms) +
+             ",plan_items_ms=" + std::to_string(p->t_plan_items_ms);
     std::snprintf(buf, len, "%s", d.c_str());
     return RTG_OK;
 }

@@ -46,6 +46,7 @@ struct rtg_tc_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double t_plan_ms = 0.0;
+    double t_plan_graph_ms = 0.0, t_plan_tune_ms = 0.0, t_plan_items_ms = 0.0;  // host wall time of the stages
     // batched planning state
     std::vector<int32_t> hoff, hcnt;        // host copies: row offsets, partner-list offsets
     std::vector<unsigned long long> hwork;  // streamed elements per owner
