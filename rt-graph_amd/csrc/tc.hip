@@ -148,6 +148,7 @@ __global__ __launch_bounds__(kPlainThreads) void tc_per_edge_1a2(
 // probing over the same slots.
 
 constexpr int kRetries = 4;
+constexpr int32_t kMaxTableSlots = 150 * 1024 / 4;  // LDS table slots one workgroup may use
 constexpr int kEvict = 64;
 
 // Slots per half for an out-list of length d.
@@ -257,10 +258,36 @@ __device__ Table build_wave_table(uint32_t *tab, const int32_t *__restrict__ adj
     return c;
 }
 
-template <bool kLinear>
+// Probe modes: the owner's list in a cuckoo table or in a linear-probing table.
+constexpr int kCuckoo = 0, kLinearProbe = 1;
+
+template <int kMode>
 __device__ __forceinline__ uint32_t probe(const Table &c, uint32_t x) {
-    if constexpr (kLinear) return linear_has(c, x);
+    if constexpr (kMode == kLinearProbe) return linear_has(c, x);
     else return cuckoo_has(c, x);
+}
+
+// Owners too long for any LDS table: every element of the partner lists wspan[p0 .. p1) is
+// binary-searched in the owner's sorted list own[0 .. d) in HBM. Kept out of line and compact
+// (no unrolling): it is rare, and a second copy of the streaming loop would only crowd the
+// instruction cache of the common path.
+__device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
+                                               int64_t p0, int64_t p1, const int32_t *__restrict__ own, int32_t d) {
+    uint32_t hits = 0;
+    for (int64_t pi = p0; pi < p1; ++pi) {
+        const int2 sp = wspan[pi];
+        for (int32_t j = __lane_id(); j < sp.y; j += 64) {
+            const int32_t x = adj[sp.x + j];
+            int32_t lo = 0, hi = d;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (own[mid] < x) lo = mid + 1;
+                else hi = mid;
+            }
+            hits += (lo < d && own[lo] == x) ? 1u : 0u;
+        }
+    }
+    return hits;
 }
 
 // Streams the out-lists of partner spans wspan[p0 .. p1) ({start, length} in adj) through the
@@ -269,7 +296,7 @@ __device__ __forceinline__ uint32_t probe(const Table &c, uint32_t x) {
 // reading element j (coalesced, no index arithmetic beyond base + j); shorter ones are
 // concatenated: their lengths are scanned into the wave's LDS slots and each lane walks a
 // cursor, so that lane j of a round handles element j of the concatenation.
-template <bool kLinear, int U>
+template <int kMode, int U>
 __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                  int64_t p0, int64_t p1, const Table &c, int32_t long_min,
                                                  int32_t *ex, int32_t *st) {
@@ -296,7 +323,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if (x[u] != kEmpty) hits += probe<kLinear>(c, x[u]);
+                    if (x[u] != kEmpty) hits += probe<kMode>(c, x[u]);
             }
         }
         const bool is_short = !is_long && sp.y > 0;
@@ -327,7 +354,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u)
-                    if (x[u] != kEmpty) hits += probe<kLinear>(c, x[u]);
+                    if (x[u] != kEmpty) hits += probe<kMode>(c, x[u]);
             }
             rtgd::wave_sync_lds();
         }
@@ -388,8 +415,22 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
-                if (c.linear) hits += stream_spans<true, U>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
-                else hits += stream_spans<false, U>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
+                if (c.linear) hits += stream_spans<kLinearProbe, U>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
+                else hits += stream_spans<kCuckoo, U>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
+            }
+        } else if (it.kind == 2) {
+            // Search item: an owner whose list is too long for an LDS table; partners
+            // [it.b, it.c) are binary-searched in its sorted list in HBM.
+            const int32_t x = it.a;
+            const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
+            while (true) {
+                int32_t g = 0;
+                if (lane == 0) g = atomicAdd(&next_batch, a.grab);
+                const int32_t b = (int32_t)rtgd::uniform_u32((uint32_t)g);
+                const int64_t p0 = (int64_t)it.b + b;
+                if (p0 >= it.c) break;
+                const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
+                hits += stream_search(a.adj, a.wspan, p0, p1, a.adj + xb, d);
             }
         } else {
             // Wave items: small owners small_list[it.b, it.c), one per wave at a time.
@@ -398,8 +439,8 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 const int32_t x = a.small_list[s];
                 const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
                 const Table c = build_wave_table(tab, a.adj, xb, d);
-                if (c.linear) hits += stream_spans<true, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
-                else hits += stream_spans<false, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
+                if (c.linear) hits += stream_spans<kLinearProbe, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
+                else hits += stream_spans<kCuckoo, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
                 rtgd::wave_sync_lds();
             }
         }
@@ -515,13 +556,14 @@ struct BatchTune {
     int32_t wave_owner_max = -1;   // -1: as large as the wave's table share allows
     int32_t long_min = 128;
     int32_t grab = 64;
+    int32_t table_cap = 0;         // LDS table slots per block (0: what a workgroup can hold)
 };
 
 std::string tune_string(const BatchTune &t) {
     return "threads=" + std::to_string(t.block_threads) + ",unroll=" + std::to_string(t.unroll) +
            ",item=" + std::to_string(t.item_work) + ",wave_item=" + std::to_string(t.wave_item_work) +
            ",wave_max=" + std::to_string(t.wave_owner_max) + ",long=" + std::to_string(t.long_min) +
-           ",grab=" + std::to_string(t.grab);
+           ",grab=" + std::to_string(t.grab) + (t.table_cap > 0 ? ",table_cap=" + std::to_string(t.table_cap) : "");
 }
 
 void tune_from_string(BatchTune &t, const std::string &s) {
@@ -541,6 +583,7 @@ void tune_from_string(BatchTune &t, const std::string &s) {
             else if (k == "wave_max") t.wave_owner_max = (int32_t)v;
             else if (k == "long") t.long_min = (int32_t)std::max<long long>(1, v);
             else if (k == "grab") t.grab = (int32_t)std::max<long long>(1, std::min<long long>(64, v));
+            else if (k == "table_cap") t.table_cap = (int32_t)std::max<long long>(0, std::min<long long>(kMaxTableSlots, v));
         }
         pos = e + 1;
     }
@@ -635,12 +678,23 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
     if (threads != 256 && threads != 512) return set_error(RTG_EINVAL, "batched TC: threads must be 256 or 512");
     const int waves = threads / 64;
     const int32_t max_d = g->max_row;
-    // Block table: both cuckoo halves of the largest owner, at least 2048 slots, a multiple of
-    // 64 * waves so that the wave-private tables split it evenly.
-    int32_t max_table = std::max<int32_t>(2048, 2 * (int32_t)cuckoo_half(max_d));
-    max_table = (max_table + 64 * waves - 1) / (64 * waves) * (64 * waves);
-    if ((size_t)max_table * 4 > 150 * 1024)
-        return set_error(RTG_EUNSUPPORTED, "batched TC: out-degree " + std::to_string(max_d) + " needs more LDS than a workgroup has");
+    // Block table: both cuckoo halves of the largest owner that fits, at least 2048 slots, a
+    // multiple of 64 * waves so that the wave-private tables split it evenly, at most the cap.
+    // Owners whose table would exceed the cap become search items.
+    const int32_t quantum = 64 * waves;
+    const int32_t cap = (tune.table_cap > 0 ? tune.table_cap : kMaxTableSlots) / quantum * quantum;
+    int32_t fit_block = 0;  // largest out-degree whose table fits the cap
+    {
+        int32_t lo = 0, hi = std::max<int32_t>(max_d, 0);
+        while (lo < hi) {
+            const int32_t mid = lo + (hi - lo + 1) / 2;
+            if (2 * (int64_t)cuckoo_half(mid) <= cap) lo = mid;
+            else hi = mid - 1;
+        }
+        fit_block = lo;
+    }
+    int32_t max_table = std::max<int32_t>(std::min<int32_t>(2048, cap), 2 * (int32_t)cuckoo_half(fit_block));
+    max_table = std::min<int32_t>(cap, (max_table + quantum - 1) / quantum * quantum);
     const int32_t wave_table = max_table / waves;
     int32_t fit = 0;  // largest out-degree whose table fits a wave's share
     while (2 * (int32_t)cuckoo_half(fit + 1) <= wave_table) ++fit;
@@ -648,6 +702,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
 
     std::vector<TcItem> items;
     std::vector<std::pair<uint64_t, int32_t>> small;  // (work, owner)
+    p->n_search_items = 0;
     std::vector<uint64_t> item_cost;
     for (int32_t x = 0; x < n; ++x) {
         const int32_t c = hcnt[x + 1] - hcnt[x];
@@ -660,10 +715,13 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
         }
         int64_t chunks = std::max<int64_t>(1, (int64_t)((w + tune.item_work / 2) / tune.item_work));
         chunks = std::min<int64_t>(chunks, c);
+        const bool search = d > fit_block;  // no LDS table: binary searches, ~log2(d) reads each
+        const uint64_t per = search ? (uint64_t)(32 - __builtin_clz((uint32_t)d)) / 2 : 1;
         for (int64_t k = 0; k < chunks; ++k) {
             const int32_t b = hcnt[x] + (int32_t)(c * k / chunks), e = hcnt[x] + (int32_t)(c * (k + 1) / chunks);
-            items.push_back(TcItem{0, x, b, e});
-            item_cost.push_back(w / chunks + (uint64_t)d);
+            items.push_back(TcItem{search ? 2 : 0, x, b, e});
+            item_cost.push_back(w / chunks * per + (uint64_t)d);
+            p->n_search_items += search ? 1 : 0;
         }
     }
     // Heaviest first.
@@ -1012,6 +1070,7 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
         d += "," + p->tune + ",grid=" + std::to_string(p->grid) + ",table_slots=" + std::to_string(p->max_table) +
              ",wave_owner_max=" + std::to_string(p->wave_owner_max) + ",source=" +
              (p->autotuned ? "autotune" : (p->tune_from_cache ? "cache" : (std::getenv("RTG_TC_TUNE") ? "env" : "default"))) +
+             ",search_items=" + std::to_string(p->n_search_items) +
              ",plan_graph_ms=" + std::to_string(p->t_plan_graph_ms) + ",plan_tune_ms=" + std::to_string(p->t_plan_tune_ms) +
              ",plan_items_ms=" + std::to_string(p->t_plan_items_ms);
     std::snprintf(buf, len, "%s", d.c_str());

@@ -51,6 +51,7 @@ struct rtg_tc_plan {
     std::vector<int32_t> hoff, hcnt;        // host copies: row offsets, partner-list offsets
     std::vector<unsigned long long> hwork;  // streamed elements per owner
     int64_t n_block_items = 0;
+    int64_t n_search_items = 0;  // block items of owners too long for an LDS table
     std::string tune;                       // chosen configuration
     bool autotuned = false, tune_from_cache = false;
 };

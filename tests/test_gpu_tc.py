@@ -161,6 +161,23 @@ def test_every_candidate_configuration(ctx, monkeypatch):
         p.close()
 
 
+@pytest.mark.parametrize("cap", [2048, 64])
+def test_owners_beyond_the_lds_table(ctx, monkeypatch, cap):
+    """Owners whose cuckoo table would exceed the LDS cap are binary-searched in HBM instead
+    (search items); a cap of 64 slots sends every owner there."""
+    for name in ("hub", "rmat_s16", "plcm"):
+        off, adj, t = oracle_csr(name)
+        g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+        monkeypatch.setenv("RTG_TC_TUNE", f"threads=256,unroll=8,item=65536,wave_item=4096,long=128,grab=16,table_cap={cap}")
+        p = g.tc_plan(method=rtg.TC_BATCHED)
+        d = p.describe()
+        assert int(d["table_slots"]) <= cap, (name, d)
+        if name == "hub" or cap == 64:  # out-degree 2500 needs 6272 slots
+            assert int(d["search_items"]) > 0, (name, d)
+        assert p.run()[0] == t, name
+        p.close()
+
+
 def test_symmetric_csr(ctx):
     p = GRAPHS["gnm_dups"]
     rp, nodes = orc.relabel(p)
