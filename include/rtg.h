@@ -240,7 +240,8 @@ typedef struct rtg_server_opts {
     int32_t poll_us;            /* dispatcher wait when idle (submits wake it); default 1000    */
     int32_t null_executor;      /* no device: batches complete after null_latency_us, no results */
     int32_t null_latency_us;
-    int32_t reserved;
+    int32_t copy_staging;       /* 1: stage requests/results through device buffers with copies;
+                                   0 (default): kernels read and write pinned host memory     */
 } rtg_server_opts;
 
 typedef struct rtg_server_stats {

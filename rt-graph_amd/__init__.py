@@ -688,7 +688,7 @@ class ServerOpts(C.Structure):
     _fields_ = [("slots", C.c_int32), ("max_batch_requests", C.c_int32), ("max_batch_pairs", C.c_int64),
                 ("max_batch_keys", C.c_int64), ("mem_budget_bytes", C.c_int64), ("timeout_us", C.c_int64),
                 ("poll_us", C.c_int32), ("null_executor", C.c_int32), ("null_latency_us", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("copy_staging", C.c_int32)]
 
 
 class ServerStats(C.Structure):

@@ -6,9 +6,11 @@
 // slots) and queued. A dispatcher thread owns `slots` execution slots, each with its own HIP
 // stream, event, pinned staging buffers and device buffers. Whenever a slot is free and
 // requests are waiting, the dispatcher packs as many as the slot holds into one batch and
-// launches it at once — copies in, the probe-item intersection kernel, the indexed lookup
-// kernel, copies out — so that requests arriving while earlier batches run join the next
-// free slot and their work overlaps on the device. Completed batches are detected by event
+// launches it at once — the probe-item intersection kernel and the indexed lookup kernel,
+// which read the packed requests from the slot's pinned host buffers and write the results
+// back there (zero-copy; `copy_staging` stages through device buffers with copies instead) —
+// so that requests arriving while earlier batches run join the next free slot and their work
+// overlaps on the device. Completed batches are detected by event
 // queries; each request records arrival and completion times on the host steady clock.
 //
 // Results do not depend on how requests are batched: each pair and each key is computed
@@ -66,6 +68,10 @@ struct Slot {
     int32_t *h_lookup_out = nullptr;
     uint32_t *d_pairs = nullptr, *d_pair_out = nullptr, *d_keys = nullptr, *d_items = nullptr;
     int32_t *d_lookup_out = nullptr;
+    // device addresses of the pinned host buffers (zero-copy mode)
+    const uint32_t *z_pairs = nullptr, *z_items = nullptr, *z_keys = nullptr;
+    uint32_t *z_pair_out = nullptr;
+    int32_t *z_lookup_out = nullptr;
     std::vector<Req *> reqs;
     bool busy = false;
     int64_t fake_deadline = 0;
@@ -190,6 +196,16 @@ void take_batch(rtg_server *s, Slot &slot, int64_t now) {
     }
 }
 
+__global__ void fill_zero_u32(uint32_t *__restrict__ p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+__global__ void copy_u32(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+unsigned small_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(rtg::ceil_div(n, 256), 64)); }
+
 int launch_slot(rtg_server *s, Slot &slot) {
     int64_t np = 0, nk = 0, ni = 0;
     const int64_t t_pack = now_ns();
@@ -217,6 +233,24 @@ int launch_slot(rtg_server *s, Slot &slot) {
     }
     hipStream_t st = slot.stream;
     HIP_TRY(hipEventRecord(slot.start, st));
+    if (!s->opts.copy_staging) {
+        // Zero-copy: the kernels read the packed requests from pinned host memory and write the
+        // lookup results there, so a batch is kernels only on one queue (copy engines and the
+        // waits between them and the compute queue are not involved). Pair counts accumulate
+        // in device memory (atomics) and a small kernel writes them out.
+        if (np) {
+            fill_zero_u32<<<small_grid(np), 256, 0, st>>>(slot.d_pair_out, np);
+            HIP_TRY(hipGetLastError());
+            if (ni)
+                RTG_TRY(rtg::si_launch_items(s->ctx, st, (const uint32_t *)s->g->d_offsets, (const uint32_t *)s->g->d_adjs,
+                                             slot.z_pairs, slot.z_items, ni, slot.d_pair_out));
+            copy_u32<<<small_grid(np), 256, 0, st>>>(slot.d_pair_out, slot.z_pair_out, np);
+            HIP_TRY(hipGetLastError());
+        }
+        if (nk) RTG_TRY(rtg::lookup_index_launch<uint32_t>(s->index, st, slot.z_keys, nk, slot.z_lookup_out));
+        HIP_TRY(hipEventRecord(slot.done, st));
+        return RTG_OK;
+    }
     if (np) {
         HIP_TRY(hipMemcpyAsync(slot.d_pairs, slot.h_pairs, 8 * (size_t)np, hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemsetAsync(slot.d_pair_out, 0, 4 * (size_t)np, st));
@@ -374,6 +408,19 @@ int alloc_slot(rtg_server *s, Slot &slot) {
     HIP_TRY(hipHostMalloc((void **)&slot.h_keys, 4 * (size_t)nk, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&slot.h_lookup_out, 4 * (size_t)nk, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&slot.h_items, 8 * (size_t)s->max_items, hipHostMallocDefault));
+    {
+        void *z = nullptr;
+        HIP_TRY(hipHostGetDevicePointer(&z, slot.h_pairs, 0));
+        slot.z_pairs = (const uint32_t *)z;
+        HIP_TRY(hipHostGetDevicePointer(&z, slot.h_items, 0));
+        slot.z_items = (const uint32_t *)z;
+        HIP_TRY(hipHostGetDevicePointer(&z, slot.h_keys, 0));
+        slot.z_keys = (const uint32_t *)z;
+        HIP_TRY(hipHostGetDevicePointer(&z, slot.h_pair_out, 0));
+        slot.z_pair_out = (uint32_t *)z;
+        HIP_TRY(hipHostGetDevicePointer(&z, slot.h_lookup_out, 0));
+        slot.z_lookup_out = (int32_t *)z;
+    }
     HIP_TRY(hipMalloc((void **)&slot.d_pairs, 8 * (size_t)np));
     HIP_TRY(hipMalloc((void **)&slot.d_pair_out, 4 * (size_t)np));
     HIP_TRY(hipMalloc((void **)&slot.d_keys, 4 * (size_t)nk));
@@ -383,9 +430,10 @@ int alloc_slot(rtg_server *s, Slot &slot) {
 }
 
 void defaults(rtg_server_opts &o) {
-    // Two slots overlap one batch's copies and host work with the other's kernels. With three or
-    // more concurrent batch streams, measured runs on MI355X showed sporadic batches stalled 7-11
-    // ms on the device; with two, no batch exceeded 0.6 ms at 2e6 queries/s.
+    // Two slots overlap one batch's host work with the other's kernels. (With copy staging,
+    // batches whose copies and kernels alternate between the copy engines and the compute queue
+    // stalled for up to 7-11 ms now and then, more often with more slots; zero-copy batches
+    // never exceeded 0.4 ms of device time at 2e6 queries/s.)
     if (o.slots <= 0) o.slots = 2;
     if (o.max_batch_requests <= 0) o.max_batch_requests = 8192;
     if (o.max_batch_pairs <= 0) o.max_batch_pairs = 8192;

@@ -29,7 +29,8 @@ def oracle_pairs(off, adj, uv):
     return orc.si_pairs(off.astype(np.uint32), adj.astype(np.uint32), np.asarray(uv, np.uint32).reshape(-1))
 
 
-def test_mix_equals_alone_and_oracle(ctx, world):
+@pytest.mark.parametrize("staging", [0, 1], ids=["zero_copy", "copy_staging"])
+def test_mix_equals_alone_and_oracle(ctx, world, staging):
     g, off, adj, arr, d_arr = world
     rng = np.random.default_rng(0)
     n = g.n
@@ -41,7 +42,7 @@ def test_mix_equals_alone_and_oracle(ctx, world):
                                arr[rng.integers(0, arr.size, size=int(k))]]) for k in rng.integers(1, 2048, size=60)]
     exp_pairs = oracle_pairs(off, adj, uv)
     exp_lk = [orc.lookup(arr, k) for k in lookups]
-    srv = rtg.Server(ctx, g, d_arr, arr.size, slots=4)
+    srv = rtg.Server(ctx, g, d_arr, arr.size, slots=4, copy_staging=staging)
     # the mix: everything in flight together, interleaved
     pid = [srv.submit_pair(u, v) for (u, v) in uv]
     lid = [srv.submit_lookup(k) for k in lookups]
