@@ -267,3 +267,24 @@ def test_full_plcm_n3000000(ctx):
     assert st["alg_bytes"] == e["alg_bytes"]
     print("[full] plain wave count", flush=True)
     assert g.tc_count(method=rtg.TC_PLAIN_WAVE)[0] == e["triangles"]
+
+
+def test_bs_tc_command_line(tmp_path):
+    """The reference's driver contract, bs_tc <graph_file> <device_id> (tc/bs_tc/main.cu:6-20),
+    on the same graph as binary pairs, a text edge list and a Matrix Market file."""
+    import subprocess
+    cli = Path(rtg.lib_path()).with_name("bs_tc")
+    assert cli.exists(), "rt-graph_amd/bs_tc is not built"
+    pairs = GRAPHS["rmat_s12"]
+    off, adj, t = oracle_csr("rmat_s12")
+    p = pairs.reshape(-1, 2)
+    pairs.astype("<i4").tofile(tmp_path / "g.bin")
+    (tmp_path / "g.txt").write_text("".join(f"{a} {b}\n" for a, b in p))
+    ids = p.astype(np.int64) + 1
+    n = int(ids.max())
+    (tmp_path / "g.mtx").write_text(f"%%MatrixMarket matrix coordinate pattern general\n{n} {n} {len(ids)}\n"
+                                    + "".join(f"{a} {b}\n" for a, b in ids))
+    for name in ("g.bin", "g.txt", "g.mtx"):
+        r = subprocess.run([str(cli), str(tmp_path / name), "0"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (name, r.stdout, r.stderr)
+        assert f"counting result = {t & 0xFFFFFFFF}" in r.stdout, (name, r.stdout)
