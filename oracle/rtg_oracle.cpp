@@ -3,7 +3,7 @@
 // by __graft_entry__.smoke() as the checker, and by bench.py's cpu_baseline leg as the timed
 // CPU baseline. Nothing in the product links or calls this library.
 //
-// The reference path cannot be compiled here (CUDA + Thrust, no nvcc), so each
+// The reference path cannot be compiled here (it is CUDA + Thrust and there is no nvcc), so each
 // function below restates one reference function and cites the file:line it follows.
 //
 // Pinning (tests/test_oracle.py): closed forms (K_n, wheels, friendship graphs, trees,

@@ -35,7 +35,7 @@ def check_graph(entry, pairs):
     assert orc.tc_edge_model(off, adj)["alg_bytes"] == entry["alg_bytes"]
 
 
-def test_ba_c1():
+def test_ba_100000():
     check_graph(GOLD["ba_n100000_m5_seed1"], rtg.gen_ba(100_000, 5, 1))
 
 

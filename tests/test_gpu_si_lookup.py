@@ -31,16 +31,16 @@ def test_si_pairs_zipf(ctx, method):
     assert np.array_equal(out.numpy(), exp)
 
 
-def test_si_pairs_full_c2(ctx):
-    """C2 at full size: 2^21 Zipf(1.8) lists of uniform values below 2^24 (about 59M elements),
-    paired into 2^20 pairs; every method against the oracle."""
-    print("[c2] generating 2^21 lists", flush=True)
+def test_si_pairs_full_zipf_2e20(ctx):
+    """The set-intersection microbench at full size: 2^21 Zipf(1.8) lists of uniform values
+    below 2^24 (about 59M elements) paired into 2^20 pairs; every method against the oracle."""
+    print("[zipf 2^20 pairs] generating 2^21 lists", flush=True)
     offs, el, pairs = zipf_workload(1 << 21, 1)
     assert 5.0e7 < el.size < 7.0e7
     exp = orc.si_pairs(offs, el, pairs)
     d_off, d_el, d_p = ctx.to_device(offs), ctx.to_device(el), ctx.to_device(pairs)
     for method in SI_METHODS:
-        print(f"[c2] method {method}", flush=True)
+        print(f"[zipf 2^20 pairs] method {method}", flush=True)
         out, st = rtg.si_pairs(ctx, d_off, d_el, offs.size - 1, d_p, pairs.size // 2, method)
         assert np.array_equal(out.numpy(), exp), method
 

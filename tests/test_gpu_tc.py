@@ -34,7 +34,7 @@ def _graphs():
                           dtype=np.int32).reshape(-1)
     e = rng.integers(0, 3000, size=(60000, 2)).astype(np.int32)  # duplicates and self-loops
     g["gnm_dups"] = e.reshape(-1)
-    g["ba_c1"] = rtg.gen_ba(100_000, 5, 1)
+    g["ba_100000"] = rtg.gen_ba(100_000, 5, 1)
     g["rmat_s12"] = rtg.gen_rmat(12, 16 << 12, seed=1)
     g["rmat_s16"] = rtg.gen_rmat(16, 44 << 16, seed=1)
     deg = rtg.gen_powerlaw_degrees(20_000, 2.1, 2_000.0, 20.0, seed=1)
@@ -87,7 +87,7 @@ def test_count_matches_oracle(ctx, name, method):
     assert st["edges"] == adj.size
 
 
-def test_golden_counts_c1(ctx):
+def test_golden_counts_ba_100000(ctx):
     e = GOLD["ba_n100000_m5_seed1"]
     g = rtg.DeviceGraph.from_pairs(ctx, rtg.gen_ba(100_000, 5, 1))
     off, adj = g.download()
@@ -228,7 +228,8 @@ def test_reference_interface(capsys):
 
 
 def test_full_rmat_s21_ef44(ctx):
-    """The full C4 workload against its golden count, CSR hashes and byte model."""
+    """The benchmark graph (R-MAT scale 21, edge factor 44) against its golden count, CSR hashes
+    and byte model."""
     e = GOLD["rmat_s21_ef44_seed1"]
     print("[full] generating R-MAT s21 ef44 on the device", flush=True)
     dev = rtg.gen_rmat_device(ctx, 21, 44 << 21, seed=1)
@@ -246,7 +247,7 @@ def test_full_rmat_s21_ef44(ctx):
 
 
 def test_full_plcm_n3000000(ctx):
-    """The full C3 workload (power-law configuration model, n = 3,000,000, closing 0.3),
+    """The full power-law configuration model (n = 3,000,000, closing 0.3, the serving graph),
     generated and preprocessed on the device: CSR hashes, batched and plain counts and the
     byte model against the golden values."""
     e = GOLD.get("plcm_n3000000_seed1")
@@ -254,7 +255,7 @@ def test_full_plcm_n3000000(ctx):
         pytest.skip("golden entry not generated")
     deg = rtg.gen_powerlaw_degrees(3_000_000, 2.1, 30_000.0, 50.0, seed=1)
     assert sha(deg.astype("<i4")) == e["degrees_sha256"]
-    print("[full] generating the C3 configuration model on the device", flush=True)
+    print("[full] generating the 3M-vertex configuration model on the device", flush=True)
     dev = rtg.gen_config_model_device(ctx, deg, 0.3, seed=1)
     g = rtg.DeviceGraph.from_pairs(ctx, dev)
     dev.free()
