@@ -349,6 +349,8 @@ void dispatcher(rtg_server *s) {
                     std::lock_guard<std::mutex> lock(s->mu);
                     s->error = rtg_last_error();
                 }
+                // whatever was enqueued before the failure still reads the slot's buffers
+                if (!s->opts.null_executor) (void)hipStreamSynchronize(slot.stream);
                 complete_slot(s, slot, false);
                 --busy;
             }
