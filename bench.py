@@ -169,6 +169,20 @@ def si_leg(rtg, ctx) -> dict:
             "batched_over_plain": res["plain"] / res["batched"], "alg_bytes": st["alg_bytes"]}
 
 
+def copy_peak_gbs(rtg, ctx, nbytes: int = 1 << 30) -> float:
+    """Measured device-to-device copy rate (bytes read + bytes written per second, GB/s), best of 5."""
+    a, b = rtg.DeviceArray(ctx, (nbytes,), np.uint8), rtg.DeviceArray(ctx, (nbytes,), np.uint8)
+    best = float("inf")
+    for _ in range(6):
+        t0 = time.perf_counter()
+        rtg._check(rtg._lib.rtg_memcpy_d2d(ctx.handle, rtg.C.c_void_p(b.ptr), rtg.C.c_void_p(a.ptr),
+                                           rtg.C.c_size_t(nbytes)))
+        best = min(best, time.perf_counter() - t0)
+    a.free()
+    b.free()
+    return 2 * nbytes / best / 1e9
+
+
 def read_traffic(kernel: str, workload: str):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
     p = ROOT / "profiles" / f"{kernel}_{workload}_pmc.txt"
@@ -272,6 +286,10 @@ def main() -> int:
         serving = serving_leg(rtg, ctx, args.serve_rate, args.serve_queries)
 
     traffic = read_traffic("tc_batched", args.workload)
+    copy_gbs = copy_peak_gbs(rtg, ctx)
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
+    # the build's model traffic: 8-byte keys read and written per radix pass, 16 B per raw pair
+    build_bytes = 16 * bst.raw_pairs * max(1, bst.sort_passes)
     out = {
         "metric": METRIC, "value": value, "unit": "edges/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t_ms, "higher_is_better": True, "scaling": "weak",
@@ -281,10 +299,11 @@ def main() -> int:
                    "method": "batched: owner-grouped two-choice cuckoo LDS tables, persistent work queue, hipGraph",
                    "plan": desc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "measured_copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "plain_ms": plain, "batched_over_plain": (plain_ms / t_ms) if plain_ms else None,
-        "t_build_ms": bst.t_total_ms, "t_plan_ms": plan.plan_stats["t_plan_ms"],
+        "t_build_ms": bst.t_total_ms, "build_model_bytes": build_bytes,
+        "build_gbs": build_bytes / (bst.t_total_ms * 1e-3) / 1e9, "t_plan_ms": plan.plan_stats["t_plan_ms"],
         "t_end_to_end_ms": bst.t_total_ms + plan.plan_stats["t_plan_ms"] + t_ms,
         "set_intersection": si,
         "wall_s_timed": wall, "alg_bytes": alg, "merge_bytes": st["merge_bytes"],
