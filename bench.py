@@ -166,7 +166,38 @@ def si_leg(rtg, ctx) -> dict:
         a.free()
     log(f"set intersection (2^20 Zipf pairs): plain {res['plain']:.3f} ms, batched {res['batched']:.3f} ms")
     return {"pairs": 1 << 20, "elements": int(el.size), "plain_ms": res["plain"], "batched_ms": res["batched"],
-            "batched_over_plain": res["plain"] / res["batched"], "alg_bytes": st["alg_bytes"]}
+            "batched_over_plain": res["plain"] / res["batched"], "alg_bytes": st["alg_bytes"],
+            "tile": si_tile_leg(rtg, ctx)}
+
+
+def si_tile_leg(rtg, ctx) -> dict:
+    """The reference's set-intersection dataset shape (its generator with length 100,000,
+    skew ratio 1, selectivity 0.1, density 0.1; 1000 A sets against one B set; seed 1) in tile
+    mode: median of 5 kernel times for every method, and the strategy SI_AUTO keeps."""
+    ds = rtg.gen_si_dataset(1000, 1, 100000, 1.0, 0.1, 0.1, seed=1)
+    d_off, d_el = ctx.to_device(ds.set_offsets), ctx.to_device(ds.elements)
+    out = rtg.DeviceArray(ctx, (1000,), np.uint32)
+    names = {rtg.SI_PLAIN: "plain", rtg.SI_REFERENCE: "reference", rtg.SI_BATCHED: "batched_search",
+             rtg.SI_MERGE: "merge", rtg.SI_HASH: "hash", rtg.SI_BITMAP: "bitmap", rtg.SI_AUTO: "auto"}
+    ms, alg, auto = {}, 0, None
+    for m, name in names.items():
+        ts = []
+        for _ in range(5):
+            _, st = rtg.si_tile(ctx, d_off, d_el, 1000, 1, m, d_out=out)
+            ts.append(st["t_kernel_ms"])
+            if m == rtg.SI_AUTO:
+                auto = st["strategy"]
+        if not np.all(out.numpy() == ds.intersection):
+            raise RuntimeError(f"set-intersection tile: method {name} gave a wrong count")
+        ms[name] = sorted(ts)[2]
+        alg = st["alg_bytes"]
+    for a in (out, d_el, d_off):
+        a.free()
+    best = min(v for k, v in ms.items() if k != "auto")
+    log("set intersection tile (1000 x 1 sets of 100,000): " + ", ".join(f"{k} {v:.3f} ms" for k, v in ms.items()))
+    return {"workload": "1000 A sets x 1 B set, length 100000, selectivity 0.1, density 0.1", "ms": ms,
+            "auto_strategy": auto, "alg_bytes": alg, "alg_gbs_best": alg / best / 1e6,
+            "elements": int(ds.elements.size)}
 
 
 def copy_peak_gbs(rtg, ctx, nbytes: int = 1 << 30) -> float:

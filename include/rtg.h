@@ -169,17 +169,30 @@ int rtg_tc_per_edge(rtg_ctx *ctx, const rtg_graph *g, int32_t form, uint32_t *d_
 
 /* ---- sorted-set intersection ---------------------------------------------------------- */
 #define RTG_SI_PLAIN 0     /* one wave per pair, binary search of the shorter in the longer    */
-#define RTG_SI_BATCHED 1   /* size-bucketed, per-bucket strategy                               */
+#define RTG_SI_BATCHED 1   /* one launch: pairs with a shorter list of <= 64 elements in flat
+                              groups, longer ones as 64-element search items                  */
 #define RTG_SI_REFERENCE 2 /* block per pair with an LDS splitter sample, the BlockBasedOBS
                               scheme (set_intersection/baseline/bs.cu:138-209)                */
+/* Batched variants whose "balanced" long pairs (s + l <= s * (1 + ceil(log2(l + 1))) for
+   shorter/longer lengths s, l) take another strategy; skewed ones keep the search items.     */
+#define RTG_SI_MERGE 3     /* merge-path segments staged in LDS (baseline/ip.cu:138-287)       */
+#define RTG_SI_HASH 4      /* LDS hash table per 1024-element chunk (baseline/hi.cu:134-195)  */
+#define RTG_SI_BITMAP 5    /* LDS bitmap per 65,536-value window for dense lists, merge-path
+                              segments for the sparser ones (baseline/bi_dynamic.cu:134-169)  */
+#define RTG_SI_AUTO 6      /* times BATCHED, MERGE, HASH and BITMAP once per argument set
+                              (offsets, elements, pairs pointers and sizes) and keeps the
+                              fastest; t_plan_ms holds the tuning time when it ran            */
 
 typedef struct rtg_si_stats {
     double t_kernel_ms;
     double t_plan_ms;
     uint64_t alg_bytes;  /* sum over pairs of 20 + list_bytes(|A|,|B|)                         */
     int32_t nbuckets;
-    int32_t reserved;
-    int64_t bucket_items[RTG_TC_MAX_BUCKETS];
+    int32_t strategy;    /* long-pair strategy run: 0 search, 1 merge, 2 hash, 3 bitmap;
+                            -1 for PLAIN and REFERENCE                                         */
+    int64_t bucket_items[RTG_TC_MAX_BUCKETS]; /* batched: units of each strategy (index as
+                            strategy): 64-element search items, merge segments of 1024 steps,
+                            hash chunks, bitmap windows                                        */
 } rtg_si_stats;
 
 /* Lists are sorted, duplicate-free uint32 runs elements[offsets[i] .. offsets[i+1]).

@@ -29,7 +29,8 @@ __all__ = [
     "RTGError", "Context", "DeviceArray", "DeviceGraph", "TcPlan", "LookupIndex", "Graph", "GraphInfo",
     "CountOnGPU", "BinarySearchBasedIntersection", "BSAccessCountFunction", "Dataset",
     "read_dataset", "write_dataset", "TC_PLAIN_THREAD", "TC_PLAIN_WAVE", "TC_BATCHED", "TC_2A1",
-    "SI_PLAIN", "SI_BATCHED", "SI_REFERENCE", "gen_ba", "gen_rmat", "gen_powerlaw_degrees",
+    "SI_PLAIN", "SI_BATCHED", "SI_REFERENCE", "SI_MERGE", "SI_HASH", "SI_BITMAP", "SI_AUTO",
+    "SI_METHODS", "gen_si_dataset", "gen_ba", "gen_rmat", "gen_powerlaw_degrees",
     "gen_config_model", "gen_zipf_lengths", "gen_sorted_lists", "gen_sorted_array",
     "gen_uniform_u32", "read_binary_pairs", "read_text_pairs", "read_mtx_pairs", "lib_path",
 ]
@@ -38,7 +39,9 @@ _HERE = Path(__file__).resolve().parent
 _LIB_PATH = _HERE / "librtg.so"
 
 TC_PLAIN_THREAD, TC_PLAIN_WAVE, TC_BATCHED, TC_2A1 = 0, 1, 2, 3
-SI_PLAIN, SI_BATCHED, SI_REFERENCE = 0, 1, 2
+SI_PLAIN, SI_BATCHED, SI_REFERENCE, SI_MERGE, SI_HASH, SI_BITMAP, SI_AUTO = 0, 1, 2, 3, 4, 5, 6
+SI_METHODS = (SI_PLAIN, SI_BATCHED, SI_REFERENCE, SI_MERGE, SI_HASH, SI_BITMAP, SI_AUTO)
+SI_STRATEGIES = {-1: "none", 0: "search", 1: "merge", 2: "hash", 3: "bitmap"}
 PAIRS_RELABEL, PAIRS_ORIENT, PAIRS_DEVICE = 0x1, 0x2, 0x4
 TC_MAX_BUCKETS = 8
 
@@ -117,8 +120,13 @@ class TcStats(C.Structure):
 
 class SiStats(C.Structure):
     _fields_ = [("t_kernel_ms", C.c_double), ("t_plan_ms", C.c_double), ("alg_bytes", C.c_uint64),
-                ("nbuckets", C.c_int32), ("reserved", C.c_int32),
+                ("nbuckets", C.c_int32), ("strategy", C.c_int32),
                 ("bucket_items", C.c_int64 * TC_MAX_BUCKETS)]
+
+    def as_dict(self) -> dict:
+        return {"t_kernel_ms": self.t_kernel_ms, "t_plan_ms": self.t_plan_ms, "alg_bytes": self.alg_bytes,
+                "strategy": SI_STRATEGIES.get(self.strategy, str(self.strategy)),
+                "units": {SI_STRATEGIES[q]: int(self.bucket_items[q]) for q in range(self.nbuckets)}}
 
 
 # ---- context and device memory --------------------------------------------------------------
@@ -480,17 +488,18 @@ def si_pairs(ctx: Context, d_offsets: DeviceArray, d_elements: DeviceArray, nlis
     st = SiStats()
     _check(_lib.rtg_si_pairs(ctx.handle, _ptr(d_offsets), _ptr(d_elements), C.c_int64(nlists), _ptr(d_pairs),
                              C.c_int64(npairs), C.c_int32(method), _ptr(d_out), C.byref(st)))
-    return d_out, {"t_kernel_ms": st.t_kernel_ms, "t_plan_ms": st.t_plan_ms, "alg_bytes": st.alg_bytes,
-                   "bucket_items": list(st.bucket_items[: st.nbuckets])}
+    return d_out, st.as_dict()
 
 
 def si_tile(ctx: Context, d_offsets: DeviceArray, d_elements: DeviceArray, num_a: int, num_b: int,
-            method: int = SI_BATCHED):
-    out = DeviceArray(ctx, (num_a * num_b,), np.uint32)
+            method: int = SI_BATCHED, d_out: DeviceArray | None = None):
+    """rtg_si_tile: counts[a * num_b + b] = |A_a & B_b| for sets [0, num_a) and [num_a, num_a + num_b)."""
+    if d_out is None:
+        d_out = DeviceArray(ctx, (num_a * num_b,), np.uint32)
     st = SiStats()
     _check(_lib.rtg_si_tile(ctx.handle, _ptr(d_offsets), _ptr(d_elements), C.c_uint32(num_a), C.c_uint32(num_b),
-                            C.c_int32(method), _ptr(out), C.byref(st)))
-    return out, {"t_kernel_ms": st.t_kernel_ms, "alg_bytes": st.alg_bytes}
+                            C.c_int32(method), _ptr(d_out), C.byref(st)))
+    return d_out, st.as_dict()
 
 
 def BinarySearchBasedIntersection(dataset: Dataset, blocks: int = 0, block_size: int = 512,
@@ -665,6 +674,42 @@ def gen_sorted_lists(lengths: np.ndarray, value_range: int = 1 << 24, seed: int 
     _check(_lib.rtg_gen_sorted_lists(C.c_int64(lengths.size), _ptr(offsets), C.c_uint32(value_range),
                                      C.c_uint64(seed), _ptr(elems)))
     return offsets, elems
+
+
+def gen_si_dataset(num_a: int, num_b: int, length: int, skew_ratio: float = 1.0, selectivity: float = 0.1,
+                   density: float = 0.1, seed: int = 1) -> Dataset:
+    """The reference's uniform set-intersection dataset (set_intersection/data_generator/
+    data.cpp:21-74, header fields as in its Writer, :344-373): every A set is the same sorted
+    set of len_a = length distinct values below max_element = (len_a + len_b - len_c) / density
+    (at most 2^31 - 1); every B set holds the len_c smallest values of that set plus
+    len_b - len_c values outside it, so every A x B intersection has len_c elements, with
+    len_b = length / skew_ratio and len_c = length * selectivity. Values come from numpy's
+    PCG64 seeded with `seed` rather than from rand()."""
+    len_a, len_b, len_c = int(length), int(length / skew_ratio), int(length * selectivity)
+    if not (0 <= len_c <= min(len_a, len_b)):
+        raise RTGError(1, "gen_si_dataset: selectivity leaves len_c outside [0, min(len_a, len_b)]")
+    value_range = min(int((len_a + len_b - len_c) / density), (1 << 31) - 1)
+    if value_range < len_a + len_b - len_c:
+        raise RTGError(1, "gen_si_dataset: value range too small for the set lengths")
+    rng = np.random.default_rng(seed)
+    tmp = np.sort(rng.choice(value_range, size=len_a, replace=False)).astype(np.uint32)
+    b_sets = []
+    for _ in range(num_b):
+        need, fresh = len_b - len_c, np.empty(0, np.uint32)
+        while fresh.size < need:
+            cand = rng.integers(0, value_range, size=2 * (need - fresh.size) + 16, dtype=np.uint32)
+            cand = cand[~np.isin(cand, tmp)]
+            fresh = np.unique(np.concatenate([fresh, cand]))
+        fresh = rng.permutation(fresh)[:need]
+        b_sets.append(np.sort(np.concatenate([tmp[:len_c], fresh])).astype(np.uint32))
+    lengths = np.array([len_a] * num_a + [len_b] * num_b, dtype=np.uint64)
+    offsets = np.zeros(lengths.size + 1, dtype=np.uint64)
+    np.cumsum(lengths, out=offsets[1:])
+    if offsets[-1] >= 1 << 32:
+        raise RTGError(1, "gen_si_dataset: more than 2^32 elements")
+    elements = np.concatenate([np.tile(tmp, num_a)] + b_sets).astype(np.uint32) if lengths.size else \
+        np.zeros(0, np.uint32)
+    return Dataset(num_a, num_b, value_range, len_c, offsets.astype(np.uint32), elements)
 
 
 def gen_sorted_array(n: int, seed: int = 1) -> np.ndarray:

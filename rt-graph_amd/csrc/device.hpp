@@ -38,6 +38,15 @@ struct rtg_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     hipEvent_t ev[8] = {};
+    // Long-pair strategy the set-intersection autotuner chose, per argument set (most recent
+    // last). Every strategy returns the same counts; a stale entry only costs speed.
+    struct SiTuned {
+        const void *off, *el, *pairs;
+        int64_t npairs;
+        uint32_t num_a, num_b;
+        int kind;
+    };
+    std::vector<SiTuned> si_tuned;
 };
 
 namespace rtg {

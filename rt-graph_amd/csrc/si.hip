@@ -19,10 +19,17 @@
 //                 round whatever the pairs' lengths. Every search in the launch is one binary
 //                 search deep: neither a long pair nor a run of length-1 pairs serialises a
 //                 chain of searches or leaves lanes idle. Work is assigned statically; no
-//                 counter is shared between waves.
+//                 counter is shared between waves. Unit offsets are 64-bit, so any tile fits.
+//   SI_MERGE, SI_HASH, SI_BITMAP
+//                 the same launch, with the long pairs where merging is cheaper than searching
+//                 run as merge-path segments, LDS hash chunks or LDS bitmap windows (see
+//                 kLongMerge and its siblings below).
+//   SI_AUTO       times the four batched strategies on the input once and keeps the fastest.
 #include "device.hpp"
 #include "si.hpp"
 
+#include <algorithm>
+#include <chrono>
 #include <cstring>
 
 namespace {
@@ -168,23 +175,70 @@ __global__ __launch_bounds__(kThreads) void si_reference(Lists L, PairSrc P, int
 // ---- batched ----
 constexpr uint32_t kFlatMax = 64;  // longest shorter-list handled in flat groups
 
-struct Batch {
-    uint32_t *item_off;     // per pair: item count (s > kFlatMax), scanned in place
-    uint32_t *total_items;  // written by the scan
-};
+// Strategies for the pairs whose shorter list is longer than kFlatMax. kLongSearch splits the
+// shorter list into 64-element items searched in the longer list. The others serve "balanced"
+// pairs, where a merge reads fewer bytes than the searches (s + l <= s * (1 + ceil(log2(l + 1)))
+// with s, l the shorter and longer length); skewed pairs keep the search items:
+//   kLongMerge   merge path (set_intersection/baseline/ip.cu:138-287): the merge of the two
+//                lists is cut into segments of kMergeSpan steps; a wave stages the segment's
+//                windows of both lists in LDS and each lane walks kMergeV steps of it.
+//   kLongHash    the hash scheme (hi.cu:134-195): chunks of kHashChunk elements of the shorter
+//                list go into an LDS hash table; the longer list's elements in the chunk's value
+//                range probe it.
+//   kLongBitmap  the bitmap scheme (bi_dynamic.cu:134-169) over windows of kBitmapSpan values
+//                kept in LDS, for pairs whose shorter list is dense enough (at least kBitmapFill
+//                elements per window on average); sparser balanced pairs take merge segments.
+enum : int { kLongSearch = 0, kLongMerge = 1, kLongHash = 2, kLongBitmap = 3, kLongKinds = 4 };
+constexpr uint32_t kMergeV = 16;
+constexpr uint32_t kMergeSpan = 64 * kMergeV;
+constexpr uint32_t kHashChunk = 1024;
+constexpr uint32_t kHashSlots = 2 * kHashChunk;  // load factor at most 1/2
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kBitmapWords = 2048;
+constexpr uint32_t kBitmapSpan = 32 * kBitmapWords;
+constexpr uint32_t kBitmapFill = 512;
+constexpr uint32_t kLongLds = 2048;  // uint32 per wave: merge windows, hash table or bitmap
 
-// Per pair: the number of 64-element items of its shorter list when it is longer than
-// kFlatMax (those pairs' results are cleared here and accumulated by the items), else 0.
-__global__ __launch_bounds__(kThreads) void si_classify(Lists L, PairSrc P, int64_t npairs, Batch B,
+// The unit kind of a pair whose shorter list s (length ns > kFlatMax) meets a longer list of
+// length nl, and its number of units.
+template <int kLong>
+__device__ __forceinline__ int long_kind(const uint32_t *__restrict__ s, uint32_t ns, uint32_t nl, uint64_t &units) {
+    units = (ns + 63u) / 64u;
+    if (kLong == kLongSearch) return kLongSearch;
+    const uint32_t bits = 32 - __builtin_clz(nl);  // nl >= ns > 0
+    if ((uint64_t)ns + nl > (uint64_t)ns * (1 + bits)) return kLongSearch;
+    if (kLong == kLongHash) {
+        units = (ns + kHashChunk - 1) / kHashChunk;
+        return kLongHash;
+    }
+    if (kLong == kLongBitmap) {
+        const uint64_t windows = (uint64_t)((s[ns - 1] - s[0]) / kBitmapSpan) + 1;
+        if (windows * kBitmapFill <= ns) {
+            units = windows;
+            return kLongBitmap;
+        }
+    }
+    units = ((uint64_t)ns + nl + kMergeSpan - 1) / kMergeSpan;
+    return kLongMerge;
+}
+
+// Per pair: its unit count when its shorter list is longer than kFlatMax (those pairs' results
+// are cleared here and accumulated by the units), else 0.
+template <int kLong>
+__global__ __launch_bounds__(kThreads) void si_classify(Lists L, PairSrc P, int64_t npairs, uint64_t *__restrict__ units,
                                                         uint32_t *__restrict__ out) {
     for (int64_t k = (int64_t)blockIdx.x * kThreads + threadIdx.x; k < npairs; k += (int64_t)gridDim.x * kThreads) {
         uint32_t a, b;
         P.get(k, a, b);
-        const uint32_t la = L.off[a + 1] - L.off[a], lb = L.off[b + 1] - L.off[b];
-        const uint32_t sl = la < lb ? la : lb;
-        const uint32_t items = sl > kFlatMax ? (sl + 63u) / 64u : 0u;
-        if (items) out[k] = 0u;
-        B.item_off[k] = items;
+        const uint32_t *s, *l;
+        uint32_t ns, nl;
+        short_long(L, a, b, s, ns, l, nl);
+        uint64_t u = 0;
+        if (ns > kFlatMax) {
+            (void)long_kind<kLong>(s, ns, nl, u);
+            out[k] = 0u;
+        }
+        units[k] = u;
     }
 }
 
@@ -204,49 +258,308 @@ __device__ __forceinline__ uint32_t member(const uint32_t *__restrict__ l, uint3
     return (b < nl && l[b] == v) ? 1u : 0u;
 }
 
+// Wave-cooperative search for the first i in [lo, hi) where pred(i) is false (hi if none),
+// pred being true on a prefix: each round tests 64 evenly spaced candidates, one per lane, and
+// keeps the gap between the last true and the first false one. The range shrinks 64-fold per
+// round, so a search over n positions takes ceil(log64 n) dependent loads instead of log2 n.
+template <class Pred>
+__device__ __forceinline__ uint64_t wave_partition_point(uint64_t lo, uint64_t hi, Pred pred) {
+    const uint64_t lane = (uint64_t)__lane_id();
+    while (lo < hi) {
+        const uint64_t step = (hi - lo + 63) / 64;
+        const uint64_t c = lo + lane * step;
+        const bool t = c < hi && pred(c);
+        const uint64_t m = (uint64_t)__popcll(__ballot(t));
+        const uint64_t nlo = m == 0 ? lo : lo + (m - 1) * step + 1;
+        const uint64_t nhi = min(hi, lo + m * step);
+        lo = rtgd::uniform_u64(nlo);
+        hi = rtgd::uniform_u64(nhi);
+    }
+    return lo;
+}
+
+// First index in x[from..n) holding a value >= v (v may be 2^32: then n).
+__device__ __forceinline__ uint32_t wave_lower_bound(const uint32_t *__restrict__ x, uint32_t from, uint32_t n, uint64_t v) {
+    return (uint32_t)wave_partition_point(from, n, [&](uint64_t i) { return (uint64_t)x[i] < v; });
+}
+
+// The same for one value in two arrays at once: both searches' loads are issued in the same
+// rounds, so the two chains of dependent loads overlap.
+__device__ __forceinline__ void wave_lower_bound2(const uint32_t *__restrict__ x, uint32_t &xlo, uint32_t xn,
+                                                  const uint32_t *__restrict__ y, uint32_t &ylo, uint32_t yn, uint64_t v) {
+    const uint32_t lane = __lane_id();
+    uint32_t xhi = xn, yhi = yn;
+    while (xlo < xhi || ylo < yhi) {
+        const uint32_t xs = (uint32_t)(((uint64_t)(xhi - xlo) + 63) / 64), ys = (uint32_t)(((uint64_t)(yhi - ylo) + 63) / 64);
+        const uint64_t cx = (uint64_t)xlo + (uint64_t)lane * xs, cy = (uint64_t)ylo + (uint64_t)lane * ys;
+        const bool inx = cx < xhi, iny = cy < yhi;
+        const uint32_t vx = inx ? x[cx] : 0u, vy = iny ? y[cy] : 0u;
+        const uint32_t mx = (uint32_t)__popcll(__ballot(inx && (uint64_t)vx < v));
+        const uint32_t my = (uint32_t)__popcll(__ballot(iny && (uint64_t)vy < v));
+        const uint32_t nxlo = mx == 0 ? xlo : xlo + (mx - 1) * xs + 1, nylo = my == 0 ? ylo : ylo + (my - 1) * ys + 1;
+        const uint32_t nxhi = (uint32_t)min<uint64_t>(xhi, (uint64_t)xlo + (uint64_t)mx * xs);
+        const uint32_t nyhi = (uint32_t)min<uint64_t>(yhi, (uint64_t)ylo + (uint64_t)my * ys);
+        xlo = rtgd::uniform_u32(nxlo);
+        xhi = rtgd::uniform_u32(nxhi);
+        ylo = rtgd::uniform_u32(nylo);
+        yhi = rtgd::uniform_u32(nyhi);
+    }
+}
+
+// Number of elements of A among the first d elements of the merge of A and B that takes A
+// first on ties (the merge-path co-rank).
+__device__ __forceinline__ uint32_t wave_corank(const uint32_t *__restrict__ A, uint32_t na, const uint32_t *__restrict__ B,
+                                                uint32_t nb, uint64_t d) {
+    const uint64_t lo = d > nb ? d - nb : 0, hi = min<uint64_t>(d, na);
+    return (uint32_t)wave_partition_point(lo, hi, [&](uint64_t i) { return A[i] <= B[d - i - 1]; });
+}
+
+// Calls f(x[t]) for t in [t0, t1) over the wave's lanes, kU loads per lane in flight per round.
+template <int kU, class F>
+__device__ __forceinline__ void wave_stream(const uint32_t *__restrict__ x, uint32_t t0, uint32_t t1, F f) {
+    const uint32_t lane = __lane_id();
+    for (uint32_t base = t0; base < t1; base += 64 * kU) {
+        uint32_t v[kU];
+#pragma unroll
+        for (int q = 0; q < kU; ++q) {
+            const uint32_t t = base + lane + 64 * q;
+            v[q] = t < t1 ? x[t] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < kU; ++q)
+            if (base + lane + 64 * q < t1) f(v[q]);
+    }
+}
+
+// Carried from one unit of a wave's run to the next unit of the same pair: where the previous
+// unit stopped in each list, so that a continuing unit needs no search to find its start.
+struct Carry {
+    uint32_t i = 0, j = 0;
+};
+
+// Merge windows in LDS: one padding slot per 8 elements, so that lanes reading about 8 elements
+// apart (each lane walks kMergeV steps, about half in each list) hit distinct banks.
+__device__ __forceinline__ uint32_t mpad(uint32_t t) { return t + (t >> 3); }
+constexpr uint32_t kMergeWin = kMergeSpan + kMergeSpan / 8;
+
+// Merge segment r: path steps [r * kMergeSpan, +kMergeSpan). Each element a of A taken by the
+// merge is counted when it equals the first unconsumed element of B, which is then the first
+// element of B not below a; lists are duplicate-free, so every common value counts once.
+__device__ uint32_t merge_unit(const uint32_t *__restrict__ A, uint32_t na, const uint32_t *__restrict__ B, uint32_t nb,
+                               uint64_t r, uint32_t *win, Carry &cr, bool carried) {
+    const uint32_t lane = __lane_id();
+    const uint64_t d0 = r * kMergeSpan;
+    const uint32_t len = (uint32_t)min<uint64_t>(kMergeSpan, (uint64_t)na + nb - d0);
+    uint32_t i0 = cr.i, j0 = cr.j;
+    if (!carried) {
+        i0 = wave_corank(A, na, B, nb, d0);
+        j0 = (uint32_t)(d0 - i0);
+    }
+    const uint32_t naw = min(kMergeSpan, na - i0), nbw = min(kMergeSpan, nb - j0);
+    uint32_t *sA = win, *sB = win + kMergeWin;
+    {
+        uint32_t va[kMergeV], vb[kMergeV];
+#pragma unroll
+        for (uint32_t q = 0; q < kMergeV; ++q) {
+            const uint32_t t = lane + 64 * q;
+            va[q] = t < naw ? A[i0 + t] : 0u;
+            vb[q] = t < nbw ? B[j0 + t] : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kMergeV; ++q) {
+            const uint32_t t = lane + 64 * q;
+            if (t < naw) sA[mpad(t)] = va[q];
+            if (t < nbw) sB[mpad(t)] = vb[q];
+        }
+    }
+    rtgd::wave_sync_lds();
+    // The path of this segment stays in the windows: a step that takes A happens at position
+    // i + j < len <= kMergeSpan, so j < kMergeSpan, and j == nbw there means B has ended.
+    const uint32_t t = lane * kMergeV;
+    uint32_t c = 0, i = 0, j = 0;
+    if (t < len) {
+        uint32_t lo = t > nbw ? t - nbw : 0, hi = min(t, naw);
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sA[mpad(mid)] <= sB[mpad(t - mid - 1)]) lo = mid + 1;
+            else hi = mid;
+        }
+        i = lo;
+        j = t - lo;
+        const uint32_t steps = min(kMergeV, len - t);
+        uint32_t a = i < naw ? sA[mpad(i)] : 0u, b = j < nbw ? sB[mpad(j)] : 0u;
+        for (uint32_t q = 0; q < steps; ++q) {
+            if (i < naw && (j >= nbw || a <= b)) {
+                c += (j < nbw && a == b) ? 1u : 0u;
+                ++i;
+                a = i < naw ? sA[mpad(i)] : 0u;
+            } else {
+                ++j;
+                b = j < nbw ? sB[mpad(j)] : 0u;
+            }
+        }
+    }
+    // lane 63 ends where a full segment ends: the next segment's start
+    cr.i = i0 + (uint32_t)__builtin_amdgcn_readlane((int)i, 63);
+    cr.j = j0 + (uint32_t)__builtin_amdgcn_readlane((int)j, 63);
+    rtgd::wave_sync_lds();  // the windows are rewritten by the next unit
+    return c;
+}
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t v) { return (v * 0x9E3779B1u) >> (32 - 11); }
+static_assert(kHashSlots == 2048, "hash_slot yields 11 bits");
+
+// Hash chunk r: A[r * kHashChunk, +kHashChunk) in an LDS table (linear probing, empty slots
+// hold kEmpty; a chunk holding the value kEmpty itself, necessarily A's last element, flags
+// it instead). The chunk's B range runs from the first element of B not below the chunk's
+// first element to the next chunk's, or past A's last element for the last chunk; consecutive
+// chunks' ranges meet, so a continuing chunk starts where the previous one stopped.
+__device__ uint32_t hash_unit(const uint32_t *__restrict__ A, uint32_t na, const uint32_t *__restrict__ B, uint32_t nb,
+                              uint64_t r, uint32_t *tab, Carry &cr, bool carried) {
+    const uint32_t lane = __lane_id();
+    const uint32_t a0 = (uint32_t)(r * kHashChunk), a1 = min(na, a0 + kHashChunk);
+    const uint32_t b0 = carried ? cr.j : wave_lower_bound(B, 0, nb, A[a0]);
+    const uint64_t stop = a1 < na ? (uint64_t)A[a1] : (uint64_t)A[na - 1] + 1;
+    const uint32_t b1 = wave_lower_bound(B, b0, nb, stop);
+    cr.j = b1;
+#pragma unroll
+    for (uint32_t q = 0; q < kHashSlots / 64; ++q) tab[lane + 64 * q] = kEmpty;
+    rtgd::wave_sync_lds();
+    bool has_max = false;
+    wave_stream<8>(A, a0, a1, [&](uint32_t v) {
+        if (v == kEmpty) {
+            has_max = true;
+            return;
+        }
+        uint32_t h = hash_slot(v);
+        while (atomicCAS(&tab[h], kEmpty, v) != kEmpty) h = (h + 1) & (kHashSlots - 1);
+    });
+    has_max = __ballot(has_max) != 0;
+    rtgd::wave_sync_lds();
+    uint32_t c = 0;
+    wave_stream<8>(B, b0, b1, [&](uint32_t v) {
+        if (v == kEmpty) {
+            c += has_max ? 1u : 0u;
+            return;
+        }
+        uint32_t h = hash_slot(v);
+        for (;;) {
+            const uint32_t x = tab[h];
+            if (x == v) {
+                ++c;
+                break;
+            }
+            if (x == kEmpty) break;
+            h = (h + 1) & (kHashSlots - 1);
+        }
+    });
+    rtgd::wave_sync_lds();  // the table is rewritten by the next unit
+    return c;
+}
+
+// Bitmap window r: values [A[0] + r * kBitmapSpan, +kBitmapSpan), one bit each in LDS. A's
+// elements in the window set their bits, B's elements in the window test them; both ranges
+// continue from the previous window's ends.
+__device__ uint32_t bitmap_unit(const uint32_t *__restrict__ A, uint32_t na, const uint32_t *__restrict__ B, uint32_t nb,
+                                uint64_t r, uint32_t *bits, Carry &cr, bool carried) {
+    const uint32_t lane = __lane_id();
+    const uint64_t lo = (uint64_t)rtgd::uniform_u32(A[0]) + r * kBitmapSpan, hi = lo + kBitmapSpan;
+    uint32_t a0 = cr.i, b0 = cr.j;
+    if (!carried) {
+        a0 = 0;
+        b0 = 0;
+        wave_lower_bound2(A, a0, na, B, b0, nb, lo);
+    }
+    uint32_t a1 = a0, b1 = b0;
+    wave_lower_bound2(A, a1, na, B, b1, nb, hi);
+    cr.i = a1;
+    cr.j = b1;
+    if (a0 == a1 || b0 == b1) return 0u;
+#pragma unroll
+    for (uint32_t q = 0; q < kBitmapWords / 64; ++q) bits[lane + 64 * q] = 0u;
+    rtgd::wave_sync_lds();
+    wave_stream<8>(A, a0, a1, [&](uint32_t v) {
+        const uint32_t o = (uint32_t)(v - lo);
+        atomicOr(&bits[o >> 5], 1u << (o & 31));
+    });
+    rtgd::wave_sync_lds();
+    uint32_t c = 0;
+    wave_stream<8>(B, b0, b1, [&](uint32_t v) {
+        const uint32_t o = (uint32_t)(v - lo);
+        c += (bits[o >> 5] >> (o & 31)) & 1u;
+    });
+    rtgd::wave_sync_lds();  // the bitmap is rewritten by the next unit
+    return c;
+}
+
 // Work is assigned statically (no shared queue: a single-address atomic per grab costs more
-// than the grab's work here). Items: wave w takes a contiguous run of items, so it locates
-// its first pair by one binary search over item_off and walks forward. Flat groups: wave w
+// than the grab's work here). Units: wave w takes a contiguous run of units, so it locates
+// its first pair by one search over the unit offsets and walks forward, keeping the pair's
+// lists and the unit's end position while the run stays on one pair. Flat groups: wave w
 // takes pairs 64g .. 64g+63 for g = w, w + waves, ...; pairs whose shorter list is 1..64 long
 // concatenate their elements over the lanes (a lane per element per round, one binary search
-// each), pairs with an empty list get 0, longer pairs are left to the items.
-__global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, Batch B, int64_t npairs,
+// each), pairs with an empty list get 0, longer pairs are left to the units.
+template <int kLong>
+__global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, const uint64_t *__restrict__ unit_off,
+                                                     const uint64_t *__restrict__ total_units, int64_t npairs,
                                                      uint32_t *__restrict__ out) {
     constexpr int kWaves = kThreads / 64;
-    __shared__ uint32_t ex_all[kWaves][65];   // exclusive element offsets of the group's pairs
-    __shared__ uint32_t sh_all[kWaves][64];   // shorter list: start in elements
-    __shared__ uint32_t lo_all[kWaves][64];   // longer list: start in elements
-    __shared__ uint32_t ln_all[kWaves][64];   // longer list: length
-    __shared__ uint32_t cnt_all[kWaves][64];  // per-pair counts
+    // One LDS region per wave, used by the units (merge windows, hash table or bitmap) and
+    // afterwards by the flat groups: exclusive element offsets of the group's pairs (65), the
+    // shorter lists' starts, the longer lists' starts and lengths, and per-pair counts (64 each).
+    constexpr uint32_t kFlatLds = 65 + 4 * 64;
+    constexpr uint32_t kUnitLds = kLong == kLongSearch ? 0
+                                  : kLong == kLongHash ? kHashSlots
+                                                       : 2 * kMergeWin;  // bitmap pairs may take merge units
+    static_assert(kLong != kLongBitmap || kUnitLds >= kBitmapWords, "bitmap window fits the wave's region");
+    constexpr uint32_t kRegion = kUnitLds > kFlatLds ? kUnitLds : kFlatLds;
+    __shared__ uint32_t region_all[kWaves][kRegion];
     const int lane = __lane_id();
     const int wave = (int)rtgd::uniform_u32(threadIdx.x >> 6);
-    uint32_t *ex = ex_all[wave], *sh = sh_all[wave], *lo = lo_all[wave], *ln = ln_all[wave], *cnt = cnt_all[wave];
+    uint32_t *ex = region_all[wave], *sh = ex + 65, *lo = sh + 64, *ln = lo + 64, *cnt = ln + 64;
     const int64_t nwaves = (int64_t)gridDim.x * kWaves;
     const int64_t gw = (int64_t)blockIdx.x * kWaves + wave;
 
-    // Items of the long pairs: item i belongs to the last pair k with item_off[k] <= i and
-    // covers elements 64*(i - item_off[k]) .. +63 of its shorter list.
-    const int64_t nitems = rtgd::uniform_u32(*B.total_items);
-    const int64_t per = (nitems + nwaves - 1) / nwaves;
-    const int64_t i0 = gw * per, i1 = min(nitems, i0 + per);
+    // Units of the long pairs: unit i belongs to the last pair k with unit_off[k] <= i and is
+    // that pair's unit i - unit_off[k].
+    const uint64_t nunits = rtgd::uniform_u64(*total_units);
+    const uint64_t per = (nunits + nwaves - 1) / nwaves;
+    const uint64_t i0 = (uint64_t)gw * per, i1 = min(nunits, i0 + per);
     int64_t k = -1;
-    for (int64_t i = i0; i < i1; ++i) {
-        if (k < 0 || (k + 1 < npairs && rtgd::uniform_u32(B.item_off[k + 1]) <= (uint64_t)i)) {
-            int64_t a0 = k + 1, a1 = npairs;  // upper_bound(item_off, i) - 1; item_off[0] = 0
+    uint64_t k_off = 0, k_end = 0;  // the pair's units are [k_off, k_end)
+    const uint32_t *s = nullptr, *l = nullptr;
+    uint32_t ns = 0, nl = 0;
+    int kind = kLongSearch;
+    Carry cr;
+    for (uint64_t i = i0; i < i1; ++i) {
+        const bool carried = k >= 0 && i < k_end;
+        if (!carried) {
+            int64_t a0 = k + 1, a1 = npairs;  // upper_bound(unit_off, i) - 1; unit_off[0] = 0
             while (a0 < a1) {
                 const int64_t mid = (a0 + a1) >> 1;
-                if (rtgd::uniform_u32(B.item_off[mid]) <= (uint64_t)i) a0 = mid + 1;
+                if (rtgd::uniform_u64(unit_off[mid]) <= i) a0 = mid + 1;
                 else a1 = mid;
             }
             k = a0 - 1;
+            k_off = rtgd::uniform_u64(unit_off[k]);
+            k_end = k + 1 < npairs ? rtgd::uniform_u64(unit_off[k + 1]) : nunits;
+            uint32_t a, b;
+            P.get(k, a, b);
+            short_long(L, a, b, s, ns, l, nl);
+            uint64_t nu;
+            if (kLong != kLongSearch) kind = long_kind<kLong>(s, ns, nl, nu);
         }
-        uint32_t a, b;
-        P.get(k, a, b);
-        const uint32_t *s, *l;
-        uint32_t ns, nl;
-        short_long(L, a, b, s, ns, l, nl);
-        const uint32_t e = 64u * (uint32_t)(i - B.item_off[k]) + (uint32_t)lane;
-        uint32_t c = e < ns ? member(l, nl, s[e]) : 0u;
+        const uint64_t r = i - k_off;
+        uint32_t c = 0;
+        if (kind == kLongSearch) {
+            const uint32_t e = 64u * (uint32_t)r + (uint32_t)lane;
+            c = e < ns ? member(l, nl, s[e]) : 0u;
+        } else if (kLong != kLongSearch) {
+            uint32_t *win = region_all[wave];
+            if (kind == kLongMerge) c = merge_unit(s, ns, l, nl, r, win, cr, carried);
+            else if (kind == kLongHash) c = hash_unit(s, ns, l, nl, r, win, cr, carried);
+            else c = bitmap_unit(s, ns, l, nl, r, win, cr, carried);
+        }
         c = rtgd::wave_sum_u32(c);
         if (lane == 0 && c) atomicAdd(&out[k], c);
     }
@@ -321,26 +634,39 @@ __global__ __launch_bounds__(kThreads) void si_items(Lists L, const uint2 *__res
     }
 }
 
-__global__ __launch_bounds__(kThreads) void si_alg_bytes(Lists L, PairSrc P, int64_t npairs, unsigned long long *acc) {
-    uint64_t s = 0;
+// Statistics pass: algorithmic bytes (sum over pairs of 20 + list_bytes(s, l)) and the units
+// of each long-pair kind the batched launch runs.
+template <int kLong>
+__global__ __launch_bounds__(kThreads) void si_stats_pass(Lists L, PairSrc P, int64_t npairs,
+                                                          unsigned long long *acc) {
+    uint64_t v[1 + kLongKinds] = {};
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < npairs; k += (int64_t)gridDim.x * blockDim.x) {
         uint32_t a, b;
         P.get(k, a, b);
-        const uint32_t la = L.off[a + 1] - L.off[a], lb = L.off[b + 1] - L.off[b];
-        const uint64_t sm = la < lb ? la : lb, lg = la < lb ? lb : la;
-        const uint64_t bits = lg ? 64 - __clzll(lg) : 0;
-        const uint64_t merge = sm + lg, bs = sm * (1 + bits);
-        s += 20 + 4 * (merge < bs ? merge : bs);
+        const uint32_t *s, *l;
+        uint32_t ns, nl;
+        short_long(L, a, b, s, ns, l, nl);
+        const uint64_t bits = nl ? 32 - __builtin_clz(nl) : 0;
+        const uint64_t merge = (uint64_t)ns + nl, bs = (uint64_t)ns * (1 + bits);
+        v[0] += 20 + 4 * (merge < bs ? merge : bs);
+        if (kLong >= 0 && ns > kFlatMax) {
+            uint64_t u;
+            const int kind = long_kind<kLong < 0 ? 0 : kLong>(s, ns, nl, u);
+            v[1 + kind] += u;
+        }
     }
-    // one atomic per block: same-address atomics from every wave would serialise
-    __shared__ uint64_t part[kThreads / 64];
-    s = rtgd::wave_sum_u64(s);
-    if (__lane_id() == 0) part[threadIdx.x >> 6] = s;
+    // one atomic per block and counter: same-address atomics from every wave would serialise
+    __shared__ uint64_t part[kThreads / 64][1 + kLongKinds];
+#pragma unroll
+    for (int q = 0; q <= kLongKinds; ++q) {
+        const uint64_t w = rtgd::wave_sum_u64(v[q]);
+        if (__lane_id() == 0) part[threadIdx.x >> 6][q] = w;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x <= kLongKinds) {
         uint64_t t = 0;
-        for (int w = 0; w < kThreads / 64; ++w) t += part[w];
-        if (t) atomicAdd(acc, (unsigned long long)t);
+        for (int w = 0; w < kThreads / 64; ++w) t += part[w][threadIdx.x];
+        if (t) atomicAdd(&acc[threadIdx.x], (unsigned long long)t);
     }
 }
 
@@ -356,6 +682,72 @@ struct Free {
     }
 };
 
+// What identifies a call for the autotuner's memory.
+struct SiKey {
+    const void *pairs;
+    uint32_t num_a, num_b;
+};
+
+int long_of(int32_t method) {
+    switch (method) {
+        case RTG_SI_BATCHED: return kLongSearch;
+        case RTG_SI_MERGE: return kLongMerge;
+        case RTG_SI_HASH: return kLongHash;
+        case RTG_SI_BITMAP: return kLongBitmap;
+        default: return -1;
+    }
+}
+
+struct BatchBufs {
+    uint64_t *unit_off;
+    uint64_t *total;
+    void *scan_work;
+};
+
+template <int kLong>
+int launch_batched_t(rtg_ctx *ctx, Lists L, PairSrc P, int64_t npairs, const BatchBufs &B, uint32_t *d_out) {
+    hipStream_t st = ctx->stream;
+    si_classify<kLong><<<grid_for(ctx, npairs, 16), kThreads, 0, st>>>(L, P, npairs, B.unit_off, d_out);
+    HIP_TRY(hipGetLastError());
+    RTG_TRY(rtg::scan_exclusive_u64_w(ctx, B.unit_off, B.unit_off, npairs, B.total, B.scan_work));
+    // The search drain runs two generations of blocks; the LDS-staging drains one (four blocks
+    // per CU fit), so each wave's run of units is as long as it can be.
+    const int per_cu = kLong == kLongSearch ? 8 : 4;
+    si_drain<kLong><<<(unsigned)((int64_t)ctx->cu_count * per_cu), kThreads, 0, st>>>(L, P, B.unit_off, B.total, npairs,
+                                                                                      d_out);
+    HIP_TRY(hipGetLastError());
+    return RTG_OK;
+}
+
+int launch_batched(rtg_ctx *ctx, int kind, Lists L, PairSrc P, int64_t npairs, const BatchBufs &B, uint32_t *d_out) {
+    switch (kind) {
+        case kLongSearch: return launch_batched_t<kLongSearch>(ctx, L, P, npairs, B, d_out);
+        case kLongMerge: return launch_batched_t<kLongMerge>(ctx, L, P, npairs, B, d_out);
+        case kLongHash: return launch_batched_t<kLongHash>(ctx, L, P, npairs, B, d_out);
+        default: return launch_batched_t<kLongBitmap>(ctx, L, P, npairs, B, d_out);
+    }
+}
+
+// Times every long-pair strategy once on this input (each run writes the full, exact result)
+// and returns the fastest; the launch after it is the one the caller's timing sees.
+int autotune(rtg_ctx *ctx, Lists L, PairSrc P, int64_t npairs, const BatchBufs &B, uint32_t *d_out, int *best) {
+    hipStream_t st = ctx->stream;
+    float best_ms = 0.f;
+    *best = kLongSearch;
+    for (int kind = 0; kind < kLongKinds; ++kind) {
+        HIP_TRY(hipEventRecord(ctx->ev[2], st));
+        RTG_TRY(launch_batched(ctx, kind, L, P, npairs, B, d_out));
+        HIP_TRY(hipEventRecord(ctx->ev[3], st));
+        HIP_TRY(hipEventSynchronize(ctx->ev[3]));
+        const float ms = rtg::elapsed_ms(ctx->ev[2], ctx->ev[3]);
+        if (kind == 0 || ms < best_ms) {
+            best_ms = ms;
+            *best = kind;
+        }
+    }
+    return RTG_OK;
+}
+
 int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P, int64_t npairs, int32_t method,
            uint32_t *d_out, rtg_si_stats *stats) {
     using namespace rtg;
@@ -364,53 +756,74 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (npairs <= 0) return RTG_OK;
     if (npairs >= ((int64_t)1 << 32)) return set_error(RTG_EINVAL, "rtg_si: more than 2^32 pairs");
+    const bool autotuned = method == RTG_SI_AUTO;
+    int kind = autotuned ? kLongSearch : long_of(method);
+    if (kind < 0 && method != RTG_SI_PLAIN && method != RTG_SI_REFERENCE)
+        return set_error(RTG_EINVAL, "rtg_si: unknown method");
     Free fb;
-    Batch B{};
-    void *scan_work = nullptr;
-    if (method == RTG_SI_BATCHED) {
+    BatchBufs B{};
+    if (kind >= 0) {
         const size_t sw = scan_work_bytes(npairs);
-        const size_t bytes = Arena::round(sizeof(uint32_t) * (size_t)npairs) + Arena::round(sw) + 1024;
+        const size_t bytes = Arena::round(sizeof(uint64_t) * (size_t)npairs) + Arena::round(sw) + 1024;
         HIP_TRY(hipMalloc(&fb.p, bytes));
         Arena ar{(char *)fb.p, 0, bytes};
-        B.item_off = ar.take<uint32_t>((size_t)npairs);
-        B.total_items = ar.take<uint32_t>(4);
-        scan_work = ar.take<char>(sw);
+        B.unit_off = ar.take<uint64_t>((size_t)npairs);
+        B.total = ar.take<uint64_t>(4);
+        B.scan_work = ar.take<char>(sw);
+    }
+    double t_tune = 0;
+    if (autotuned) {
+        auto &memo = ctx->si_tuned;
+        const SiKey key{P.pairs, P.num_a, P.num_b};
+        auto hit = std::find_if(memo.begin(), memo.end(), [&](const rtg_ctx::SiTuned &e) {
+            return e.off == d_off && e.el == d_el && e.pairs == key.pairs && e.npairs == npairs && e.num_a == key.num_a &&
+                   e.num_b == key.num_b;
+        });
+        if (hit != memo.end()) {
+            kind = hit->kind;
+        } else {
+            const auto t0 = std::chrono::steady_clock::now();
+            RTG_TRY(autotune(ctx, L, P, npairs, B, d_out, &kind));
+            t_tune = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (memo.size() >= 64) memo.erase(memo.begin());
+            memo.push_back({d_off, d_el, key.pairs, npairs, key.num_a, key.num_b, kind});
+        }
     }
     HIP_TRY(hipEventRecord(ctx->ev[0], st));
-    switch (method) {
-        case RTG_SI_PLAIN:
-            si_plain<<<grid_for(ctx, npairs * 64, 32), kThreads, 0, st>>>(L, P, npairs, d_out);
-            break;
-        case RTG_SI_REFERENCE:
-            si_reference<<<(unsigned)std::min<int64_t>(npairs, (int64_t)ctx->cu_count * 8), kThreads, 0, st>>>(L, P, npairs, d_out);
-            break;
-        case RTG_SI_BATCHED: {
-            si_classify<<<grid_for(ctx, npairs, 16), kThreads, 0, st>>>(L, P, npairs, B, d_out);
-            RTG_TRY(scan_exclusive_u32_w(ctx, B.item_off, B.item_off, npairs, B.total_items, scan_work));
-            si_drain<<<(unsigned)((int64_t)ctx->cu_count * 8), kThreads, 0, st>>>(L, P, B, npairs, d_out);
-            break;
-        }
-        default:
-            return set_error(RTG_EINVAL, "rtg_si: unknown method");
+    if (method == RTG_SI_PLAIN) {
+        si_plain<<<grid_for(ctx, npairs * 64, 32), kThreads, 0, st>>>(L, P, npairs, d_out);
+    } else if (method == RTG_SI_REFERENCE) {
+        si_reference<<<(unsigned)std::min<int64_t>(npairs, (int64_t)ctx->cu_count * 8), kThreads, 0, st>>>(L, P, npairs, d_out);
+    } else {
+        RTG_TRY(launch_batched(ctx, kind, L, P, npairs, B, d_out));
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[1], st));
     if (stats) {
         void *s = nullptr;
+        constexpr size_t kAcc = sizeof(unsigned long long) * (1 + kLongKinds);
         RTG_TRY(ctx_scratch(ctx, 256, &s));
-        HIP_TRY(hipMemsetAsync(s, 0, sizeof(unsigned long long), st));
-        si_alg_bytes<<<grid_for(ctx, npairs, 2), kThreads, 0, st>>>(L, P, npairs, (unsigned long long *)s);
+        HIP_TRY(hipMemsetAsync(s, 0, kAcc, st));
+        auto *acc = (unsigned long long *)s;
+        const unsigned g = grid_for(ctx, npairs, 2);
+        switch (kind) {
+            case kLongSearch: si_stats_pass<kLongSearch><<<g, kThreads, 0, st>>>(L, P, npairs, acc); break;
+            case kLongMerge: si_stats_pass<kLongMerge><<<g, kThreads, 0, st>>>(L, P, npairs, acc); break;
+            case kLongHash: si_stats_pass<kLongHash><<<g, kThreads, 0, st>>>(L, P, npairs, acc); break;
+            case kLongBitmap: si_stats_pass<kLongBitmap><<<g, kThreads, 0, st>>>(L, P, npairs, acc); break;
+            default: si_stats_pass<-1><<<g, kThreads, 0, st>>>(L, P, npairs, acc); break;
+        }
         HIP_TRY(hipGetLastError());
-        unsigned long long ab = 0;
-        uint32_t nitems = 0;
-        HIP_TRY(hipMemcpyAsync(&ab, s, sizeof(ab), hipMemcpyDeviceToHost, st));
-        if (method == RTG_SI_BATCHED) HIP_TRY(hipMemcpyAsync(&nitems, B.total_items, sizeof(nitems), hipMemcpyDeviceToHost, st));
+        unsigned long long v[1 + kLongKinds] = {};
+        HIP_TRY(hipMemcpyAsync(v, s, kAcc, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         stats->t_kernel_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
-        stats->alg_bytes = ab;
-        if (method == RTG_SI_BATCHED) {  // 64-element items of the pairs answered by items
-            stats->nbuckets = 1;
-            stats->bucket_items[0] = (int64_t)nitems;
+        stats->t_plan_ms = t_tune;
+        stats->alg_bytes = v[0];
+        stats->strategy = kind;
+        if (kind >= 0) {  // units of each long-pair kind: search items, merge segments, hash chunks, bitmap windows
+            stats->nbuckets = kLongKinds;
+            for (int q = 0; q < kLongKinds; ++q) stats->bucket_items[q] = (int64_t)v[1 + q];
         }
     } else {
         HIP_TRY(hipStreamSynchronize(st));

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 rtg = load_rtg()
 orc = load_oracle()
-SI_METHODS = [rtg.SI_PLAIN, rtg.SI_BATCHED, rtg.SI_REFERENCE]
+SI_METHODS = list(rtg.SI_METHODS)
 
 
 def zipf_workload(nlists, seed):
@@ -68,6 +68,87 @@ def test_si_tile_reference_invariant(ctx, method, len_c):
     offs, el = reference_style_dataset(4, 6, 2000, 400, len_c, rng)
     out, _ = rtg.si_tile(ctx, ctx.to_device(offs), ctx.to_device(el), 4, 6, method)
     assert np.all(out.numpy() == len_c)
+
+
+def balanced_lists(rng):
+    """Long lists for which a merge reads fewer bytes than binary searches: lengths 65 to
+    70,000 over dense and sparse value ranges, lists at the top of the uint32 range (one holding
+    0xFFFFFFFF, the hash table's empty marker), and nested, shifted and identical lists."""
+    specs = [(65, 200, 0), (1000, 1500, 0), (5000, 6000, 100), (40000, 50000, 0), (40000, 1 << 24, 0),
+             (3000, 1 << 30, 12345), (20000, 25000, (1 << 32) - 25000), (20000, 1 << 20, (1 << 32) - (1 << 20)),
+             (70000, 80000, 7)]
+    lists = []
+    for n, span, base in specs:
+        v = np.sort(rng.choice(span, size=n, replace=False)).astype(np.uint64) + base
+        lists.append(v.astype(np.uint32))
+    lists.append(np.unique(np.concatenate([lists[6][::2], np.array([0xFFFFFFFF], np.uint32)])))
+    lists.append(np.unique(np.concatenate([lists[7][1::3], np.array([0xFFFFFFFF], np.uint32)])))
+    lists.append(lists[3][::7].copy())
+    lists.append((lists[1].astype(np.uint64) + 1200).astype(np.uint32))
+    lists.append(lists[8][5:60005].copy())
+    offs = np.zeros(len(lists) + 1, np.uint32)
+    np.cumsum([x.size for x in lists], out=offs[1:])
+    return offs, np.concatenate(lists).astype(np.uint32)
+
+
+@pytest.mark.parametrize("method", SI_METHODS)
+def test_si_pairs_balanced_long(ctx, method):
+    """Every ordered pair of balanced_lists, repeated so that each wave runs several
+    consecutive units of one pair (the merge, hash and bitmap units then continue from the
+    previous unit's end instead of searching for their start)."""
+    offs, el = balanced_lists(np.random.default_rng(21))
+    n = offs.size - 1
+    base = np.array([(a, b) for a in range(n) for b in range(n)], np.uint32)
+    pairs = np.tile(base, (60, 1)).reshape(-1)
+    exp = orc.si_pairs(offs, el, pairs)
+    out, st = rtg.si_pairs(ctx, ctx.to_device(offs), ctx.to_device(el), n, ctx.to_device(pairs), pairs.size // 2,
+                           method)
+    assert np.array_equal(out.numpy(), exp)
+    kind = {rtg.SI_MERGE: "merge", rtg.SI_HASH: "hash", rtg.SI_BITMAP: "bitmap"}.get(method)
+    if kind:
+        assert st["strategy"] == kind and st["units"][kind] > 0 and st["units"]["search"] > 0
+    if method == rtg.SI_BITMAP:
+        assert st["units"]["merge"] > 0  # the sparse balanced lists
+
+
+@pytest.mark.parametrize("method", SI_METHODS)
+def test_si_tile_balanced_against_oracle(ctx, method):
+    offs, el = balanced_lists(np.random.default_rng(5))
+    n = offs.size - 1
+    num_a = 6
+    exp = orc.si_tile(offs, el, num_a, n - num_a)
+    out, _ = rtg.si_tile(ctx, ctx.to_device(offs), ctx.to_device(el), num_a, n - num_a, method)
+    assert np.array_equal(out.numpy(), exp)
+
+
+@pytest.mark.parametrize("length,skew,sel,density", [(100000, 1.0, 0.1, 0.1), (60000, 4.0, 0.2, 0.9),
+                                                     (30000, 0.5, 0.0, 0.02)])
+def test_si_tile_reference_generator(ctx, length, skew, sel, density):
+    """Datasets shaped by the reference's generator (data.cpp:21-74; the first row is the shape
+    its scripts generate, s_generator -l 100000 -r 1 -s 0.1 -d 0.1 with 1000 A sets and one B
+    set): every A x B count is len_c, for every method."""
+    num_a = 1000 if length == 100000 else 200
+    num_b = 1 if length == 100000 else 3
+    print(f"[si dataset] len {length} skew {skew} sel {sel} density {density}", flush=True)
+    ds = rtg.gen_si_dataset(num_a, num_b, length, skew, sel, density, seed=3)
+    d_off, d_el = ctx.to_device(ds.set_offsets), ctx.to_device(ds.elements)
+    for method in SI_METHODS:
+        print(f"[si dataset] method {method}", flush=True)
+        out, st = rtg.si_tile(ctx, d_off, d_el, num_a, num_b, method)
+        assert np.all(out.numpy() == ds.intersection), (method, st)
+
+
+def test_si_auto_remembers_choice(ctx):
+    offs, el = balanced_lists(np.random.default_rng(8))
+    n = offs.size - 1
+    pairs = np.tile(np.array([(a, b) for a in range(n) for b in range(n)], np.uint32), (20, 1)).reshape(-1)
+    exp = orc.si_pairs(offs, el, pairs)
+    d_off, d_el, d_p = ctx.to_device(offs), ctx.to_device(el), ctx.to_device(pairs)
+    out1, st1 = rtg.si_pairs(ctx, d_off, d_el, n, d_p, pairs.size // 2, rtg.SI_AUTO)
+    out2, st2 = rtg.si_pairs(ctx, d_off, d_el, n, d_p, pairs.size // 2, rtg.SI_AUTO)
+    assert np.array_equal(out1.numpy(), exp) and np.array_equal(out2.numpy(), exp)
+    assert st1["t_plan_ms"] > 0 and st2["t_plan_ms"] == 0
+    assert st1["strategy"] == st2["strategy"] != "none"
 
 
 def test_binary_search_based_intersection_dataset(tmp_path):
