@@ -240,6 +240,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-si", action="store_true", help="skip the set-intersection microbench")
+    ap.add_argument("--si-only", action="store_true",
+                    help="run only the set-intersection legs and print their JSON (not the bench line)")
     ap.add_argument("--serve-queries", type=int, default=1 << 24)
     ap.add_argument("--serve-rate", type=float, default=2.0e6)
     ap.add_argument("--tune-cache", default=str(ROOT / "rt-graph_amd" / "tune_cache.json"),
@@ -252,6 +254,10 @@ def main() -> int:
         return 2
 
     rtg = load_pkg()
+    if args.si_only:
+        ctx = rtg.Context(0)
+        print(json.dumps({"set_intersection": si_leg(rtg, ctx)}), flush=True)
+        return 0
     scale, ef = WORKLOADS[args.workload]
     nsamples = ef << scale
     ctx = rtg.Context(0)

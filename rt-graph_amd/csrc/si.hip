@@ -426,7 +426,7 @@ __device__ uint32_t hash_unit(const uint32_t *__restrict__ A, uint32_t na, const
     for (uint32_t q = 0; q < kHashSlots / 64; ++q) tab[lane + 64 * q] = kEmpty;
     rtgd::wave_sync_lds();
     bool has_max = false;
-    wave_stream<8>(A, a0, a1, [&](uint32_t v) {
+    wave_stream<16>(A, a0, a1, [&](uint32_t v) {
         if (v == kEmpty) {
             has_max = true;
             return;
@@ -437,7 +437,7 @@ __device__ uint32_t hash_unit(const uint32_t *__restrict__ A, uint32_t na, const
     has_max = __ballot(has_max) != 0;
     rtgd::wave_sync_lds();
     uint32_t c = 0;
-    wave_stream<8>(B, b0, b1, [&](uint32_t v) {
+    wave_stream<16>(B, b0, b1, [&](uint32_t v) {
         if (v == kEmpty) {
             c += has_max ? 1u : 0u;
             return;
@@ -478,13 +478,13 @@ __device__ uint32_t bitmap_unit(const uint32_t *__restrict__ A, uint32_t na, con
 #pragma unroll
     for (uint32_t q = 0; q < kBitmapWords / 64; ++q) bits[lane + 64 * q] = 0u;
     rtgd::wave_sync_lds();
-    wave_stream<8>(A, a0, a1, [&](uint32_t v) {
+    wave_stream<16>(A, a0, a1, [&](uint32_t v) {
         const uint32_t o = (uint32_t)(v - lo);
         atomicOr(&bits[o >> 5], 1u << (o & 31));
     });
     rtgd::wave_sync_lds();
     uint32_t c = 0;
-    wave_stream<8>(B, b0, b1, [&](uint32_t v) {
+    wave_stream<16>(B, b0, b1, [&](uint32_t v) {
         const uint32_t o = (uint32_t)(v - lo);
         c += (bits[o >> 5] >> (o & 31)) & 1u;
     });
@@ -509,8 +509,10 @@ __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, const u
     // shorter lists' starts, the longer lists' starts and lengths, and per-pair counts (64 each).
     constexpr uint32_t kFlatLds = 65 + 4 * 64;
     constexpr uint32_t kUnitLds = kLong == kLongSearch ? 0
-                                  : kLong == kLongHash ? kHashSlots
-                                                       : 2 * kMergeWin;  // bitmap pairs may take merge units
+                                  : kLong == kLongHash  ? kHashSlots
+                                  : kLong == kLongMerge ? 2 * kMergeWin
+                                                        // bitmap pairs may take merge units
+                                                        : (2 * kMergeWin > kBitmapWords ? 2 * kMergeWin : kBitmapWords);
     static_assert(kLong != kLongBitmap || kUnitLds >= kBitmapWords, "bitmap window fits the wave's region");
     constexpr uint32_t kRegion = kUnitLds > kFlatLds ? kUnitLds : kFlatLds;
     __shared__ uint32_t region_all[kWaves][kRegion];
