@@ -242,19 +242,40 @@ __global__ __launch_bounds__(kThreads) void si_classify(Lists L, PairSrc P, int6
     }
 }
 
-// Number of elements of l[0..nl) below v, nl >= 1, by binary lifting: the trip count depends
-// only on nl, and every probe address is clamped into the list.
-__device__ __forceinline__ uint32_t count_below_lift(const uint32_t *__restrict__ l, uint32_t nl, uint32_t v) {
+// Number of elements of l[0..nl) below v, nl >= 1, by K-way lifting (K = 4 or 8): from the
+// largest power of K not above nl down to 1, each level issues K - 1 independent probes and
+// advances by the number of them below v. The trip count depends only on nl, every probe
+// address is clamped into the list, and the chain of dependent loads is log_K(nl) long
+// instead of binary lifting's log_2(nl), for (K - 1) / log2(K) times the loads. Four-way suits
+// the throughput launches (more loads compete for the memory system); eight-way the latency
+// path, whose launches leave the memory system idle.
+template <uint32_t K>
+__device__ __forceinline__ uint32_t count_below_kway(const uint32_t *__restrict__ l, uint32_t nl, uint32_t v) {
+    static_assert(K == 4 || K == 8, "K-way lifting with K = 4 or 8");
+    constexpr uint32_t kBits = K == 4 ? 2 : 3;
+    const uint32_t lg = 31 - __builtin_clz(nl);
     uint32_t base = 0;
-    for (uint32_t step = 1u << (31 - __builtin_clz(nl)); step > 0; step >>= 1) {
-        const uint32_t x = l[min(base + step, nl) - 1];
-        if (base + step <= nl && x < v) base += step;
+    for (uint32_t step = 1u << (lg - lg % kBits); step > 0; step >>= kBits) {  // from the largest power of K <= nl
+        const uint32_t rem = nl - base;
+        uint32_t x[K - 1];
+        bool in[K - 1];
+#pragma unroll
+        for (uint32_t j = 0; j < K - 1; ++j) {
+            // (j + 1) * step fits 32 bits for K = 4 (3 * 2^30); for K = 8 only when it is <= rem
+            in[j] = K == 4 ? (j + 1) * step <= rem : (uint64_t)(j + 1) * step <= rem;
+            x[j] = l[base + (in[j] ? (j + 1) * step : rem) - 1];
+        }
+        uint32_t add = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < K - 1; ++j) add += (in[j] && x[j] < v) ? step : 0u;
+        base += add;
     }
     return base;
 }
 
+template <uint32_t K>
 __device__ __forceinline__ uint32_t member(const uint32_t *__restrict__ l, uint32_t nl, uint32_t v) {
-    const uint32_t b = count_below_lift(l, nl, v);
+    const uint32_t b = count_below_kway<K>(l, nl, v);
     return (b < nl && l[b] == v) ? 1u : 0u;
 }
 
@@ -555,7 +576,7 @@ __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, const u
         uint32_t c = 0;
         if (kind == kLongSearch) {
             const uint32_t e = 64u * (uint32_t)r + (uint32_t)lane;
-            c = e < ns ? member(l, nl, s[e]) : 0u;
+            c = e < ns ? member<4>(l, nl, s[e]) : 0u;
         } else if (kLong != kLongSearch) {
             uint32_t *win = region_all[wave];
             if (kind == kLongMerge) c = merge_unit(s, ns, l, nl, r, win, cr, carried);
@@ -599,7 +620,7 @@ __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, const u
                 for (uint32_t step = 32; step > 0; step >>= 1)
                     if (ex[p + step] <= e) p += step;
                 const uint32_t v = L.el[sh[p] + (e - ex[p])];
-                if (member(L.el + lo[p], ln[p], v)) atomicAdd(&cnt[p], 1u);
+                if (member<4>(L.el + lo[p], ln[p], v)) atomicAdd(&cnt[p], 1u);
             }
         }
         rtgd::wave_sync_lds();
@@ -629,7 +650,7 @@ __global__ __launch_bounds__(kThreads) void si_items(Lists L, const uint2 *__res
         uint32_t c = 0;
         if (i < ns) {
             const uint32_t v = s[i];
-            c = member(l, nl, v);
+            c = member<8>(l, nl, v);
         }
         c = rtgd::wave_sum_u32(c);
         if (__lane_id() == 0 && c) atomicAdd(&out[item.x], c);
