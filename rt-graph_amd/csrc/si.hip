@@ -13,12 +13,12 @@
 //   SI_BATCHED    a classify pass gives every pair whose shorter list is longer than 64
 //                 ceil(s/64) items of 64 of its elements (item counts scanned into offsets).
 //                 One launch then runs the items (a wave per item, a lane per element, one
-//                 binary search each, partial counts added into the result) and flat groups
-//                 of 64 consecutive pairs, whose shorter lists of 0..64 elements are
-//                 concatenated over the lanes so that every lane does one binary search per
-//                 round whatever the pairs' lengths. Every search in the launch is one binary
-//                 search deep: neither a long pair nor a run of length-1 pairs serialises a
-//                 chain of searches or leaves lanes idle. Work is assigned statically; no
+//                 search each, partial counts added into the result) and flat groups of 64
+//                 consecutive pairs, whose shorter lists of 0..64 elements are concatenated
+//                 over the lanes so that every lane does one search per round whatever the
+//                 pairs' lengths. Every search in the launch is one search deep (a four-way
+//                 lifting search of the longer list): neither a long pair nor a run of
+//                 length-1 pairs serialises a chain of searches or leaves lanes idle. Work is assigned statically; no
 //                 counter is shared between waves. Unit offsets are 64-bit, so any tile fits.
 //   SI_MERGE, SI_HASH, SI_BITMAP
 //                 the same launch, with the long pairs where merging is cheaper than searching
@@ -197,7 +197,6 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kBitmapWords = 2048;
 constexpr uint32_t kBitmapSpan = 32 * kBitmapWords;
 constexpr uint32_t kBitmapFill = 512;
-constexpr uint32_t kLongLds = 2048;  // uint32 per wave: merge windows, hash table or bitmap
 
 // The unit kind of a pair whose shorter list s (length ns > kFlatMax) meets a longer list of
 // length nl, and its number of units.
@@ -518,7 +517,7 @@ __device__ uint32_t bitmap_unit(const uint32_t *__restrict__ A, uint32_t na, con
 // its first pair by one search over the unit offsets and walks forward, keeping the pair's
 // lists and the unit's end position while the run stays on one pair. Flat groups: wave w
 // takes pairs 64g .. 64g+63 for g = w, w + waves, ...; pairs whose shorter list is 1..64 long
-// concatenate their elements over the lanes (a lane per element per round, one binary search
+// concatenate their elements over the lanes (a lane per element per round, one search
 // each), pairs with an empty list get 0, longer pairs are left to the units.
 template <int kLong>
 __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, const uint64_t *__restrict__ unit_off,
@@ -631,10 +630,10 @@ __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, const u
 
 // ---- probe items (latency path) ----
 // items[i] = {k, first}: one wave takes elements first .. first+63 of pair k's shorter list,
-// one lane per element and one binary search of the longer list each, and adds its count to
-// out[k] (cleared beforehand). Splitting every pair into 64-element items puts all of a
-// batch's probes in flight at once: the launch lasts about one binary search, however
-// skewed the pairs are.
+// one lane per element and one eight-way lifting search of the longer list each, and adds its
+// count to out[k] (cleared beforehand). Splitting every pair into 64-element items puts all of
+// a batch's probes in flight at once: the launch lasts about one search, however skewed the
+// pairs are.
 __global__ __launch_bounds__(kThreads) void si_items(Lists L, const uint2 *__restrict__ pairs,
                                                      const uint2 *__restrict__ items, int64_t nitems,
                                                      uint32_t *__restrict__ out) {
