@@ -149,24 +149,29 @@ def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
 def si_leg(rtg, ctx) -> dict:
     """Set-intersection microbench: 2^21 Zipf(1.8) lists (lengths clipped at 65,536) of uniform
     values below 2^24, seed 1, paired (2k, 2k+1) into 2^20 pairs; plain (wave per pair) against
-    batched, median of 5 kernel times each."""
+    batched and autotuned batched, median of 5 kernel times each (the autotuner's first call,
+    which times every strategy, is one of the 5)."""
     lens = rtg.gen_zipf_lengths(1 << 21, 1.8, 65536, seed=1)
     offs, el = rtg.gen_sorted_lists(lens, 1 << 24, seed=1)
     pairs = np.arange(1 << 21, dtype=np.uint32)
     d_off, d_el, d_p = ctx.to_device(offs), ctx.to_device(el), ctx.to_device(pairs)
-    res = {}
-    for name, m in (("plain", rtg.SI_PLAIN), ("batched", rtg.SI_BATCHED)):
+    res, auto = {}, None
+    for name, m in (("plain", rtg.SI_PLAIN), ("batched", rtg.SI_BATCHED), ("auto", rtg.SI_AUTO)):
         ts = []
         for _ in range(5):
             out, st = rtg.si_pairs(ctx, d_off, d_el, offs.size - 1, d_p, pairs.size // 2, m)
             ts.append(st["t_kernel_ms"])
+            if m == rtg.SI_AUTO:
+                auto = st["strategy"]
         res[name] = sorted(ts)[2]
         out.free()
     for a in (d_off, d_el, d_p):
         a.free()
-    log(f"set intersection (2^20 Zipf pairs): plain {res['plain']:.3f} ms, batched {res['batched']:.3f} ms")
+    log(f"set intersection (2^20 Zipf pairs): plain {res['plain']:.3f} ms, batched {res['batched']:.3f} ms, "
+        f"autotuned {res['auto']:.3f} ms ({auto})")
     return {"pairs": 1 << 20, "elements": int(el.size), "plain_ms": res["plain"], "batched_ms": res["batched"],
-            "batched_over_plain": res["plain"] / res["batched"], "alg_bytes": st["alg_bytes"],
+            "batched_over_plain": res["plain"] / res["batched"], "auto_ms": res["auto"], "auto_strategy": auto,
+            "alg_bytes": st["alg_bytes"],
             "tile": si_tile_leg(rtg, ctx)}
 
 

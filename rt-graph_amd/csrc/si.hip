@@ -335,19 +335,23 @@ __device__ __forceinline__ uint32_t wave_corank(const uint32_t *__restrict__ A, 
 }
 
 // Calls f(x[t]) for t in [t0, t1) over the wave's lanes, kU loads per lane in flight per round.
+// Offsets are taken relative to the round's base, so no index passes t1 (or wraps).
 template <int kU, class F>
 __device__ __forceinline__ void wave_stream(const uint32_t *__restrict__ x, uint32_t t0, uint32_t t1, F f) {
     const uint32_t lane = __lane_id();
-    for (uint32_t base = t0; base < t1; base += 64 * kU) {
+    for (uint32_t base = t0; base < t1;) {
+        const uint32_t lim = t1 - base;
         uint32_t v[kU];
 #pragma unroll
-        for (int q = 0; q < kU; ++q) {
-            const uint32_t t = base + lane + 64 * q;
-            v[q] = t < t1 ? x[t] : 0u;
+        for (uint32_t q = 0; q < kU; ++q) {
+            const uint32_t o = lane + 64 * q;
+            v[q] = o < lim ? x[base + o] : 0u;
         }
 #pragma unroll
-        for (int q = 0; q < kU; ++q)
-            if (base + lane + 64 * q < t1) f(v[q]);
+        for (uint32_t q = 0; q < kU; ++q)
+            if (lane + 64 * q < lim) f(v[q]);
+        if (lim <= 64 * kU) break;
+        base += 64 * kU;
     }
 }
 
@@ -462,16 +466,30 @@ __device__ uint32_t hash_unit(const uint32_t *__restrict__ A, uint32_t na, const
             c += has_max ? 1u : 0u;
             return;
         }
+        // Slots h .. h+3 are read at once: with linear probing and no deletions, every slot
+        // from a value's home to the value itself is occupied, so v is present iff it shows up
+        // before the first empty slot. Only when all four hold other values (rare at a load
+        // factor of 1/2) does the walk go on slot by slot.
         uint32_t h = hash_slot(v);
-        for (;;) {
-            const uint32_t x = tab[h];
-            if (x == v) {
-                ++c;
-                break;
-            }
-            if (x == kEmpty) break;
-            h = (h + 1) & (kHashSlots - 1);
+        uint32_t x[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) x[i] = tab[(h + i) & (kHashSlots - 1)];
+        bool hit = false, open = false;
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            hit |= x[i] == v;
+            open |= x[i] == kEmpty;
         }
+        if (!hit && !open) {
+            for (h = (h + 4) & (kHashSlots - 1);; h = (h + 1) & (kHashSlots - 1)) {
+                const uint32_t y = tab[h];
+                if (y == v || y == kEmpty) {
+                    hit = y == v;
+                    break;
+                }
+            }
+        }
+        c += hit ? 1u : 0u;
     });
     rtgd::wave_sync_lds();  // the table is rewritten by the next unit
     return c;
