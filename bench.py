@@ -68,13 +68,24 @@ def cpu_model() -> str:
     return "unknown"
 
 
+ORACLE_THREADS = [0]  # the oracle's OpenMP thread count before any run narrows it
+
+
+def oracle_module():
+    """The CPU oracle (the checker and the CPU baseline only)."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    if not ORACLE_THREADS[0]:
+        ORACLE_THREADS[0] = orc.max_threads()
+    return orc
+
+
 def cpu_baseline(off: np.ndarray, adj: np.ndarray, target_s: float) -> dict:
     """Times the oracle's 1A2 count on a sample of edge ranges spread over the CSR: all host
     threads on a sample of about target_s seconds, then one thread on about target_s / 3."""
-    sys.path.insert(0, str(ROOT))
-    from oracle import oracle as orc  # checker / CPU baseline only
+    orc = oracle_module()
     m = adj.size
-    threads = orc.max_threads()
+    threads = ORACLE_THREADS[0]
     nr = 64
 
     def run(frac: float, nthreads: int):
@@ -205,6 +216,91 @@ def si_tile_leg(rtg, ctx) -> dict:
             "elements": int(ds.elements.size)}
 
 
+def per_edge_leg(rtg, ctx, g, count: int, alg_bytes: int, tune_cache) -> dict:
+    """Per-edge counts |N+(u) & N+(v)| on the benchmark graph: the batched per-edge plan (same
+    tuned configuration as the count; median of 5 kernel times) against one wave per edge (wall
+    time of rtg_tc_per_edge form 2 into a preallocated buffer). Both outputs are compared and
+    summed against the count."""
+    m = g.m
+    t0 = time.perf_counter()
+    plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=False, tune_cache=tune_cache, per_edge=True)
+    t_plan = (time.perf_counter() - t0) * 1e3
+    out, ts = None, []
+    for _ in range(5):
+        out, cnt, st = plan.run_per_edge(out)
+        ts.append(st["t_count_ms"])
+    t_ms = sorted(ts)[2]
+    a = out.numpy()[:m]
+    plan.close()
+    b = rtg.DeviceArray(ctx, (max(m, 1),), np.uint32)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rtg._check(rtg._lib.rtg_tc_per_edge(ctx.handle, g._h, rtg.C.c_int32(2), rtg.C.c_void_p(b.ptr)))
+    t_wave = (time.perf_counter() - t0) * 1e3
+    same = bool(np.array_equal(a, b.numpy()[:m]))
+    b.free()
+    out.free()
+    total = int(a.sum(dtype=np.uint64))
+    if cnt != count or total != count or not same:
+        raise RuntimeError(f"per-edge counts disagree: kernel total {cnt}, sum {total}, count {count}, equal {same}")
+    byts = alg_bytes + 4 * m  # the count's algorithmic bytes plus one 4-byte result per edge
+    log(f"per-edge: batched plan {t_ms:.2f} ms ({byts / t_ms / 1e6:.0f} GB/s algorithmic; plan {t_plan:.0f} ms), "
+        f"wave per edge {t_wave:.1f} ms wall")
+    return {"edges": m, "batched_ms": t_ms, "plan_ms": t_plan, "wave_per_edge_wall_ms": t_wave,
+            "batched_over_wave": t_wave / t_ms, "alg_bytes": byts, "achieved_gbs": byts / t_ms / 1e6,
+            "sum_equals_count": True}
+
+
+def ba_rows_leg(rtg, ctx) -> dict:
+    """The reference's CPU-runnable graph shape, BA(n=100,000, m=5, seed 1): file ingest in the
+    binary pair, text and Matrix Market formats (host readers, MB/s), the device CSR build, and
+    the two-hop (2A1) count on the device against the batched count and against the oracle's
+    2A1 on the host."""
+    import tempfile
+    orc = oracle_module()
+    pairs = rtg.gen_ba(100000, 5, seed=1)
+    res = {"graph": "BA n=100000 m=5 seed 1", "pairs": int(pairs.size // 2)}
+    with tempfile.TemporaryDirectory() as td:
+        pb, pt, pm = Path(td) / "g.bin", Path(td) / "g.txt", Path(td) / "g.mtx"
+        rtg._check(rtg._lib.rtg_write_binary_pairs(str(pb).encode(), rtg._ptr(pairs), rtg.C.c_int64(pairs.size // 2)))
+        p2 = pairs.reshape(-1, 2)
+        np.savetxt(pt, p2, fmt="%d")
+        n = int(pairs.max()) + 1
+        with open(pm, "w") as f:
+            f.write("%%MatrixMarket matrix coordinate pattern general\n")
+            f.write(f"{n} {n} {p2.shape[0]}\n")
+            np.savetxt(f, p2 + 1, fmt="%d")
+        ingest = {}
+        for name, path, fn in (("binary", pb, rtg.read_binary_pairs), ("text", pt, rtg.read_text_pairs),
+                               ("mtx", pm, rtg.read_mtx_pairs)):
+            best = float("inf")
+            for _ in range(3):
+                t0 = time.perf_counter()
+                got = fn(path)
+                best = min(best, time.perf_counter() - t0)
+            if not np.array_equal(got.reshape(-1), pairs.reshape(-1)):
+                raise RuntimeError(f"ingest: the {name} reader returned other pairs")
+            ingest[name] = {"bytes": path.stat().st_size, "ms": best * 1e3, "mb_per_s": path.stat().st_size / best / 1e6}
+    res["ingest"] = ingest
+    g = rtg.DeviceGraph.from_pairs(ctx, pairs)
+    res["csr_build_ms"] = g.build_stats.t_total_ms
+    cnt, st = g.tc_count(method=rtg.TC_BATCHED)
+    c2, st2 = g.tc_count(method=rtg.TC_2A1)
+    off, adj = g.download()
+    t0 = time.perf_counter()
+    c_orc = orc.tc_count_2a1(off, adj, ORACLE_THREADS[0])
+    t_orc = (time.perf_counter() - t0) * 1e3
+    if not (cnt == c2 == c_orc):
+        raise RuntimeError(f"2A1: device {c2}, batched {cnt}, oracle {c_orc}")
+    g.close()
+    res.update({"triangles": cnt, "tc_2a1_ms": st2["t_count_ms"], "tc_batched_ms": st["t_count_ms"],
+                "oracle_2a1_host_ms": t_orc, "oracle_threads": ORACLE_THREADS[0]})
+    log(f"BA 100k: ingest " + ", ".join(f"{k} {v['mb_per_s']:.0f} MB/s" for k, v in ingest.items()) +
+        f"; CSR build {res['csr_build_ms']:.2f} ms; 2A1 {st2['t_count_ms']:.2f} ms (oracle 2A1 {t_orc:.1f} ms "
+        f"host), batched {st['t_count_ms']:.3f} ms, T={cnt}")
+    return res
+
+
 def copy_peak_gbs(rtg, ctx, nbytes: int = 1 << 30) -> float:
     """Measured device-to-device copy rate (bytes read + bytes written per second, GB/s), best of 5."""
     a, b = rtg.DeviceArray(ctx, (nbytes,), np.uint8), rtg.DeviceArray(ctx, (nbytes,), np.uint8)
@@ -320,12 +416,14 @@ def main() -> int:
         cpu = cpu_baseline(off, adj, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']:.4g} edges/s on {cpu['cores']} threads")
 
+    per_edge = per_edge_leg(rtg, ctx, g, cnt, alg, args.tune_cache or None)
     si = None if args.no_si else si_leg(rtg, ctx)
     serving = None
     if not args.no_serving:
         plan.close()
         g.close()
         serving = serving_leg(rtg, ctx, args.serve_rate, args.serve_queries)
+    ba_rows = ba_rows_leg(rtg, ctx)
 
     traffic = read_traffic("tc_batched", args.workload)
     copy_gbs = copy_peak_gbs(rtg, ctx)
@@ -347,6 +445,7 @@ def main() -> int:
         "t_build_ms": bst.t_total_ms, "build_model_bytes": build_bytes,
         "build_gbs": build_bytes / (bst.t_total_ms * 1e-3) / 1e9, "t_plan_ms": plan.plan_stats["t_plan_ms"],
         "t_end_to_end_ms": bst.t_total_ms + plan.plan_stats["t_plan_ms"] + t_ms,
+        "per_edge": per_edge, "ba_100k": ba_rows,
         "set_intersection": si,
         "wall_s_timed": wall, "alg_bytes": alg, "merge_bytes": st["merge_bytes"],
         "serving": serving,

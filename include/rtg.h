@@ -132,7 +132,8 @@ typedef struct rtg_tc_opts {
                                 read if it has the key, written after autotuning             */
     int32_t autotune;        /* batched: time the candidate configurations when the cache has
                                 no entry (RTG_TC_TUNE in the environment overrides both)     */
-    int32_t reserved;
+    int32_t per_edge;        /* batched: also record each span's edge, so that
+                                rtg_tc_plan_run_per_edge can run (4 more bytes per edge)      */
 } rtg_tc_opts;
 
 #define RTG_TC_MAX_BUCKETS 8
@@ -158,13 +159,22 @@ typedef struct rtg_tc_plan rtg_tc_plan;
 int rtg_tc_plan_create(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *opts,
                        rtg_tc_plan **out, rtg_tc_stats *stats);
 int rtg_tc_plan_run(rtg_ctx *ctx, rtg_tc_plan *plan, uint64_t *count, rtg_tc_stats *stats);
+/* Per-edge counts through a batched plan created with per_edge = 1: for every oriented edge
+   e = (u -> v) of the plan's range, d_out[e] = |N+(u) & N+(v)| (triangle a<b<c attributed to
+   edge a->b; per-edge outputs as the REDUCE==1 per-ray counts of rt_tc.cpp:64-72); entries
+   outside the range are left alone. count (optional) receives the range's total; stats the
+   kernel time.                                                                              */
+int rtg_tc_plan_run_per_edge(rtg_ctx *ctx, rtg_tc_plan *plan, uint32_t *d_out, uint64_t *count,
+                             rtg_tc_stats *stats);
 int rtg_tc_plan_destroy(rtg_tc_plan *plan);
 /* The plan's configuration as "key=value,..." text (method, tune, grid, table size, and
    whether the tune came from the autotuner, the cache, RTG_TC_TUNE or the defaults). */
 int rtg_tc_plan_describe(const rtg_tc_plan *plan, char *buf, size_t len);
 
-/* Per-edge counts |N+(u) & N+(v)| for every oriented edge (triangle a<b<c attributed to edge
-   a->b), written to d_out[m]. The 1A2 attribution (edge a->c) is available with form = 1.   */
+/* Per-edge counts for every oriented edge, written to d_out[m]. form 0: |N+(u) & N+(v)|
+   (triangle a<b<c attributed to edge a->b) through a one-off batched per-edge plan; form 2:
+   the same counts by one wave per edge (no planning: for small graphs); form 1: the 1A2
+   attribution (edge a->c, ListIntersectionKernel's order, bs_tc.cu:120-148).               */
 int rtg_tc_per_edge(rtg_ctx *ctx, const rtg_graph *g, int32_t form, uint32_t *d_out);
 
 /* ---- sorted-set intersection ---------------------------------------------------------- */

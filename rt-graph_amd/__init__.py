@@ -98,7 +98,7 @@ class BuildStats(C.Structure):
 class TcOpts(C.Structure):
     _fields_ = [("method", C.c_int32), ("use_hip_graph", C.c_int32), ("e_begin", C.c_int64),
                 ("e_end", C.c_int64), ("tune_cache", C.c_char_p), ("autotune", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("per_edge", C.c_int32)]
 
 
 class TcStats(C.Structure):
@@ -273,10 +273,13 @@ class DeviceGraph:
         return cnt.value, st.as_dict()
 
     def tc_plan(self, method: int = TC_BATCHED, e_begin: int = 0, e_end: int = 0,
-                use_hip_graph: bool = True, autotune: bool = False, tune_cache: str | None = None) -> "TcPlan":
-        return TcPlan(self, method, e_begin, e_end, use_hip_graph, autotune, tune_cache)
+                use_hip_graph: bool = True, autotune: bool = False, tune_cache: str | None = None,
+                per_edge: bool = False) -> "TcPlan":
+        return TcPlan(self, method, e_begin, e_end, use_hip_graph, autotune, tune_cache, per_edge)
 
     def tc_per_edge(self, form: int = 0) -> np.ndarray:
+        """rtg_tc_per_edge: form 0 |N+(u) & N+(v)| through a batched per-edge plan, form 2 the
+        same by one wave per edge, form 1 the 1A2 attribution (edge a->c)."""
         out = DeviceArray(self.ctx, (self.m,), np.uint32)
         _check(_lib.rtg_tc_per_edge(self.ctx.handle, self._h, C.c_int32(form), C.c_void_p(out.ptr)))
         return out.numpy()
@@ -297,12 +300,12 @@ class TcPlan:
     """rtg_tc_plan: buckets, work queue and captured launch sequence, reusable across runs."""
 
     def __init__(self, g: DeviceGraph, method: int, e_begin: int, e_end: int, use_hip_graph: bool,
-                 autotune: bool = False, tune_cache: str | None = None):
+                 autotune: bool = False, tune_cache: str | None = None, per_edge: bool = False):
         self.g = g
         self._h = C.c_void_p()
         self._cache = None if tune_cache is None else os.fsencode(tune_cache)
         opts = TcOpts(method=method, use_hip_graph=int(use_hip_graph), e_begin=e_begin, e_end=e_end,
-                      tune_cache=self._cache, autotune=int(autotune))
+                      tune_cache=self._cache, autotune=int(autotune), per_edge=int(per_edge))
         st = TcStats()
         _check(_lib.rtg_tc_plan_create(g.ctx.handle, g._h, C.byref(opts), C.byref(self._h), C.byref(st)))
         self.plan_stats = st.as_dict()
@@ -321,6 +324,17 @@ class TcPlan:
         st = TcStats()
         _check(_lib.rtg_tc_plan_run(self.g.ctx.handle, self._h, C.byref(cnt), C.byref(st)))
         return cnt.value, st.as_dict()
+
+    def run_per_edge(self, d_out: DeviceArray | None = None) -> tuple[DeviceArray, int, dict]:
+        """rtg_tc_plan_run_per_edge (plan created with per_edge=True): d_out[e] = |N+(u) & N+(v)|
+        for the plan's edge range; returns the device array, the range's total and the stats."""
+        if d_out is None:
+            d_out = DeviceArray(self.g.ctx, (max(self.g.m, 1),), np.uint32)
+        cnt = C.c_uint64()
+        st = TcStats()
+        _check(_lib.rtg_tc_plan_run_per_edge(self.g.ctx.handle, self._h, C.c_void_p(d_out.ptr), C.byref(cnt),
+                                             C.byref(st)))
+        return d_out, cnt.value, st.as_dict()
 
     def close(self) -> None:
         if self._h:

@@ -3,52 +3,123 @@
 //   binary  little-endian int32 (src, dst) pairs, npairs = file_size / 8
 //           (Graph::ReadBinaryEdges, tc/graph/graph.cpp:23-39)
 //   text    one "src dst" pair per line, unparsable lines skipped with a message
-//           (Graph::Txt2Bin, tc/graph/graph.cpp:179-214)
+//           (Graph::Txt2Bin, tc/graph/graph.cpp:179-214); parsed in place from one read of
+//           the file, as the reference's `int` extraction reads each line
 //   .mtx    Matrix Market coordinate pattern/real/integer, 1-based, converted to 0-based;
 //           "symmetric" files get every off-diagonal entry twice, (r,c) then (c,r)
 //           (Graph::ReadMtx, bfs/graph/graph.cpp:17-122)
 //
-// All readers use the two-call convention: pairs == NULL returns the pair count only.
+// All readers use the two-call convention: pairs == NULL returns the pair count only (the text
+// reader reports unparsable lines in the second, filling call).
 #include "rtg_internal.hpp"
 
 #include <cctype>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <cstdlib>
 #include <fstream>
-#include <sstream>
 #include <string>
 #include <vector>
 
 namespace {
 
-int read_text_pairs(const char *path, std::vector<int32_t> &out) {
-    std::ifstream in(path);
-    if (!in) return rtg_fail(RTG_EIO, std::string("cannot open ") + path);
-    std::string line;
-    while (std::getline(in, line)) {
-        std::stringstream ss(line);
-        int32_t a, b;  // as the reference's `int` extraction: an out-of-range value fails the line
-        if (ss >> a >> b) {
+// The whole file, NUL-terminated (the parsers below scan it in place).
+int slurp(const char *path, std::vector<char> &buf, const char *open_error) {
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return rtg_fail(RTG_EIO, std::string(open_error) + path);
+    struct Closer {
+        FILE *f;
+        ~Closer() { std::fclose(f); }
+    } closer{f};
+    if (std::fseek(f, 0, SEEK_END) != 0) return rtg_fail(RTG_EIO, std::string("cannot seek ") + path);
+    const long size = std::ftell(f);
+    if (size < 0) return rtg_fail(RTG_EIO, std::string("cannot size ") + path);
+    std::rewind(f);
+    buf.resize((size_t)size + 1);
+    if (size > 0 && std::fread(buf.data(), 1, (size_t)size, f) != (size_t)size)
+        return rtg_fail(RTG_EIO, std::string("short read of ") + path);
+    buf[(size_t)size] = '\0';
+    buf.resize((size_t)size + 1);
+    return RTG_OK;
+}
+
+inline bool space_in_line(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+inline bool space_any(char c) { return space_in_line(c) || c == '\n'; }
+
+// One int32 as `std::istream >> int32_t` reads it: leading white space skipped, an optional sign,
+// decimal digits up to the first other character; no digit, or a value outside int32, fails.
+inline bool parse_i32(const char *&p, const char *end, int32_t &v) {
+    while (p < end && space_in_line(*p)) ++p;
+    bool neg = false;
+    if (p < end && (*p == '+' || *p == '-')) neg = *p++ == '-';
+    if (p >= end || *p < '0' || *p > '9') return false;
+    int64_t x = 0;
+    bool over = false;
+    for (; p < end && *p >= '0' && *p <= '9'; ++p) {
+        x = x * 10 + (*p - '0');
+        if (x > (int64_t)INT32_MAX + 1) {
+            over = true;
+            x = (int64_t)INT32_MAX + 1;
+        }
+    }
+    if (neg) x = -x;
+    if (over || x < INT32_MIN || x > INT32_MAX) return false;
+    v = (int32_t)x;
+    return true;
+}
+
+// One integer as scanf's %lld reads it (white space including newlines skipped, optional sign,
+// decimal digits); values beyond +-2^62 saturate (such entries are out of any matrix anyway).
+inline bool parse_i64(const char *&p, long long &v) {
+    while (space_any(*p)) ++p;
+    bool neg = false;
+    if (*p == '+' || *p == '-') neg = *p++ == '-';
+    if (*p < '0' || *p > '9') return false;
+    long long x = 0;
+    for (; *p >= '0' && *p <= '9'; ++p) x = x < (1ll << 62) / 10 ? x * 10 + (*p - '0') : (1ll << 62);
+    v = neg ? -x : x;
+    return true;
+}
+
+// Text edge list: every line that begins with two int32 values gives a pair (anything after them
+// is ignored); other lines are skipped, with a message unless empty or a '#' / '%' comment.
+int read_text_pairs(const char *path, std::vector<int32_t> &out, bool report) {
+    std::vector<char> buf;
+    if (const int rc = slurp(path, buf, "cannot open ")) return rc;
+    const char *p = buf.data(), *end = buf.data() + buf.size() - 1;
+    out.reserve((size_t)(end - p) / 6);
+    while (p < end) {
+        const char *eol = static_cast<const char *>(std::memchr(p, '\n', (size_t)(end - p)));
+        if (!eol) eol = end;
+        const char *q = p;
+        int32_t a, b;
+        if (parse_i32(q, eol, a) && parse_i32(q, eol, b)) {
             out.push_back(a);
             out.push_back(b);
-        } else if (!line.empty() && line[0] != '#' && line[0] != '%') {
-            std::fprintf(stderr, "Error parsing line: %s\n", line.c_str());
+        } else if (report && eol > p && *p != '#' && *p != '%') {
+            std::fprintf(stderr, "Error parsing line: %.*s\n", (int)(eol - p), p);
         }
+        p = eol + 1;
     }
     return RTG_OK;
 }
 
 int read_mtx_pairs(const char *path, std::vector<int32_t> &out, int32_t *rows_out) {
-    FILE *f = std::fopen(path, "r");
-    if (!f) return rtg_fail(RTG_EIO, std::string("File could not be opened: ") + path);
-    struct Closer {
-        FILE *f;
-        ~Closer() { std::fclose(f); }
-    } closer{f};
-    char buf[1024];
-    if (!std::fgets(buf, sizeof(buf), f)) return rtg_fail(RTG_EIO, "Could not process Matrix Market banner");
-    std::string banner(buf);
+    std::vector<char> buf;
+    if (const int rc = slurp(path, buf, "File could not be opened: ")) return rc;
+    const char *p = buf.data(), *end = buf.data() + buf.size() - 1;
+    auto next_line = [&](std::string &line) {  // false at the end of the file
+        if (p >= end) return false;
+        const char *eol = static_cast<const char *>(std::memchr(p, '\n', (size_t)(end - p)));
+        if (!eol) eol = end;
+        line.assign(p, eol);
+        p = eol < end ? eol + 1 : end;
+        return true;
+    };
+    std::string banner, line;
+    if (!next_line(banner)) return rtg_fail(RTG_EIO, "Could not process Matrix Market banner");
     for (auto &c : banner) c = (char)std::tolower((unsigned char)c);
     if (banner.rfind("%%matrixmarket", 0) != 0) return rtg_fail(RTG_EIO, "Could not process Matrix Market banner");
     if (banner.find("matrix") == std::string::npos || banner.find("array") != std::string::npos)
@@ -59,9 +130,9 @@ int read_mtx_pairs(const char *path, std::vector<int32_t> &out, int32_t *rows_ou
     if (!pattern && !real) return rtg_fail(RTG_EIO, "Unrecognized matrix market format type");
     long long rows = 0, cols = 0, nnz = 0;
     bool have_size = false;
-    while (std::fgets(buf, sizeof(buf), f)) {
-        if (buf[0] == '%') continue;
-        if (std::sscanf(buf, "%lld %lld %lld", &rows, &cols, &nnz) == 3) {
+    while (next_line(line)) {
+        if (!line.empty() && line[0] == '%') continue;
+        if (std::sscanf(line.c_str(), "%lld %lld %lld", &rows, &cols, &nnz) == 3) {
             have_size = true;
             break;
         }
@@ -69,12 +140,17 @@ int read_mtx_pairs(const char *path, std::vector<int32_t> &out, int32_t *rows_ou
     }
     if (!have_size || rows < 0 || cols < 0 || nnz < 0 || rows > INT32_MAX || cols > INT32_MAX)
         return rtg_fail(RTG_EIO, "Could not read file info (M, N, NNZ)");
-    out.reserve((size_t)nnz * (symmetric ? 4 : 2));
+    out.reserve((size_t)std::min<long long>(nnz, (long long)(end - p) / 4 + 1) * (symmetric ? 4 : 2));
     for (long long i = 0; i < nnz; ++i) {
         long long r = 0, c = 0;
-        double w = 0.0;
-        int got = pattern ? std::fscanf(f, " %lld %lld", &r, &c) : std::fscanf(f, " %lld %lld %lf", &r, &c, &w);
-        if (got != (pattern ? 2 : 3)) return rtg_fail(RTG_EIO, "Could not read edge from market file");
+        bool ok = parse_i64(p, r) && parse_i64(p, c);
+        if (ok && !pattern) {  // the value column is read and ignored
+            char *after = nullptr;
+            (void)std::strtod(p, &after);
+            ok = after != p;
+            p = after;
+        }
+        if (!ok) return rtg_fail(RTG_EIO, "Could not read edge from market file");
         if (r == 0 || c == 0) return rtg_fail(RTG_EIO, "Market file is zero-indexed");
         // The reference stores such entries unchecked (bfs/graph/graph.cpp:53-54, 74-75) and
         // then indexes its CSR with them; they are rejected here.
@@ -126,7 +202,7 @@ RTG_API int rtg_read_binary_pairs(const char *path, int32_t *pairs, int64_t *npa
 RTG_API int rtg_read_text_pairs(const char *path, int32_t *pairs, int64_t *npairs) {
     if (!path) return rtg_fail(RTG_EINVAL, "rtg_read_text_pairs: null path");
     std::vector<int32_t> v;
-    int rc = read_text_pairs(path, v);
+    int rc = read_text_pairs(path, v, pairs != nullptr);  // unparsable lines reported once, by the filling call
     if (rc != RTG_OK) return rc;
     return deliver(v, pairs, npairs);
 }

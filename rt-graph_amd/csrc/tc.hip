@@ -270,12 +270,16 @@ __device__ __forceinline__ uint32_t probe(const Table &c, uint32_t x) {
 // Owners too long for any LDS table: every element of the partner lists wspan[p0 .. p1) is
 // binary-searched in the owner's sorted list own[0 .. d) in HBM. Kept out of line and compact
 // (no unrolling): it is rare, and a second copy of the streaming loop would only crowd the
-// instruction cache of the common path.
+// instruction cache of the common path. With eout set, each span's count is also stored at
+// eout[wedge[pi]] (per-edge counts).
+template <bool kPerEdge>
 __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
-                                               int64_t p0, int64_t p1, const int32_t *__restrict__ own, int32_t d) {
+                                               int64_t p0, int64_t p1, const int32_t *__restrict__ own, int32_t d,
+                                               const uint32_t *__restrict__ wedge, uint32_t *__restrict__ eout) {
     uint32_t hits = 0;
     for (int64_t pi = p0; pi < p1; ++pi) {
         const int2 sp = wspan[pi];
+        uint32_t sh = 0;
         for (int32_t j = __lane_id(); j < sp.y; j += 64) {
             const int32_t x = adj[sp.x + j];
             int32_t lo = 0, hi = d;
@@ -284,7 +288,12 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
                 if (own[mid] < x) lo = mid + 1;
                 else hi = mid;
             }
-            hits += (lo < d && own[lo] == x) ? 1u : 0u;
+            sh += (lo < d && own[lo] == x) ? 1u : 0u;
+        }
+        hits += sh;
+        if constexpr (kPerEdge) {
+            sh = rtgd::wave_sum_u32(sh);
+            if (__lane_id() == 0) eout[wedge[pi]] = sh;
         }
     }
     return hits;
@@ -296,10 +305,13 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
 // reading element j (coalesced, no index arithmetic beyond base + j); shorter ones are
 // concatenated: their lengths are scanned into the wave's LDS slots and each lane walks a
 // cursor, so that lane j of a round handles element j of the concatenation.
-template <int kMode, int U>
+// kPerEdge: every span's count is also stored at eout[wedge[pi]] — a long span's by a wave
+// sum, a concatenated short span's through its LDS counter pe[ci].
+template <int kMode, int U, bool kPerEdge>
 __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                  int64_t p0, int64_t p1, const Table &c, int32_t long_min,
-                                                 int32_t *ex, int32_t *st) {
+                                                 int32_t *ex, int32_t *st, const uint32_t *__restrict__ wedge,
+                                                 uint32_t *__restrict__ eout, uint32_t *pe) {
     const int lane = __lane_id();
     const uint64_t lt = rtgd::lanemask_lt();
     uint32_t hits = 0;
@@ -314,6 +326,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             lm &= lm - 1;
             const int32_t s0 = __builtin_amdgcn_readlane(sp.x, p);
             const int32_t l0 = __builtin_amdgcn_readlane(sp.y, p);
+            uint32_t sh = 0;  // this span's hits (per-edge counts only)
             for (int32_t jb = 0; jb < l0; jb += 64 * U) {
                 uint32_t x[U];
 #pragma unroll
@@ -322,8 +335,17 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
                     x[u] = j < l0 ? (uint32_t)adj[s0 + j] : kEmpty;
                 }
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (x[u] != kEmpty) hits += probe<kMode>(c, x[u]);
+                for (int u = 0; u < U; ++u) {
+                    if (x[u] != kEmpty) {
+                        if constexpr (kPerEdge) sh += probe<kMode>(c, x[u]);
+                        else hits += probe<kMode>(c, x[u]);
+                    }
+                }
+            }
+            if constexpr (kPerEdge) {
+                hits += sh;
+                sh = rtgd::wave_sum_u32(sh);
+                if (lane == 0) eout[wedge[bs + p]] = sh;
             }
         }
         const bool is_short = !is_long && sp.y > 0;
@@ -337,12 +359,14 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
                 const int ci = __popcll(sm & lt);
                 ex[ci] = incl - sl;
                 st[ci] = sp.x;
+                if constexpr (kPerEdge) pe[ci] = 0u;
             }
             if (lane == 0) ex[ns] = total;
             rtgd::wave_sync_lds();
             int k = 0;
             for (int32_t jb = 0; jb < total; jb += 64 * U) {
                 uint32_t x[U];
+                int ku[kPerEdge ? U : 1];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int32_t j = jb + u * 64 + lane;
@@ -351,12 +375,23 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
                         while (ex[k + 1] <= j) ++k;
                         x[u] = (uint32_t)adj[st[k] + (j - ex[k])];
                     }
+                    if constexpr (kPerEdge) ku[u] = k;
                 }
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (x[u] != kEmpty) hits += probe<kMode>(c, x[u]);
+                for (int u = 0; u < U; ++u) {
+                    if (x[u] != kEmpty) {
+                        const uint32_t h = probe<kMode>(c, x[u]);
+                        hits += h;
+                        if constexpr (kPerEdge)
+                            if (h) atomicAdd(&pe[ku[u]], 1u);
+                    }
+                }
             }
             rtgd::wave_sync_lds();
+            if constexpr (kPerEdge) {
+                if (is_short) eout[wedge[pi]] = pe[__popcll(sm & lt)];
+                rtgd::wave_sync_lds();  // pe is rewritten by the next group of partners
+            }
         }
     }
     return hits;
@@ -375,14 +410,17 @@ struct BatchArgs {
     int32_t wave_table;         // wave-private table slots
     int32_t long_min;           // direct-stream threshold
     int32_t grab;               // partners a wave takes per grab in a block item (<= 64)
+    const uint32_t *wedge;      // per-edge counts: oriented edge of each span (else null)
+    uint32_t *eout;             // per-edge counts: output by oriented edge (else null)
 };
 
-template <int kThreads, int U>
+template <int kThreads, int U, bool kPerEdge>
 __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
     constexpr int kWaves = kThreads / 64;
     extern __shared__ uint32_t table[];
     __shared__ int32_t ex_all[kWaves][65];
     __shared__ int32_t st_all[kWaves][64];
+    __shared__ uint32_t pe_all[kPerEdge ? kWaves : 1][kPerEdge ? 64 : 1];
     __shared__ uint64_t item_slot;
     __shared__ int32_t next_batch;
     __shared__ uint32_t fail_flag;
@@ -391,6 +429,7 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
     const int wave = (int)rtgd::uniform_u32(threadIdx.x >> 6);
     int32_t *ex = ex_all[wave];
     int32_t *st = st_all[wave];
+    uint32_t *pe = pe_all[kPerEdge ? wave : 0];
     uint64_t hits = 0;
 
     // Queue grabs are published through LDS and read back as wave-uniform (scalar) values so
@@ -415,8 +454,12 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
-                if (c.linear) hits += stream_spans<kLinearProbe, U>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
-                else hits += stream_spans<kCuckoo, U>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st);
+                if (c.linear)
+                    hits += stream_spans<kLinearProbe, U, kPerEdge>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
+                                                                    a.eout, pe);
+                else
+                    hits += stream_spans<kCuckoo, U, kPerEdge>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
+                                                               a.eout, pe);
             }
         } else if (it.kind == 2) {
             // Search item: an owner whose list is too long for an LDS table; partners
@@ -430,7 +473,7 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
-                hits += stream_search(a.adj, a.wspan, p0, p1, a.adj + xb, d);
+                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d, a.wedge, a.eout);
             }
         } else {
             // Wave items: small owners small_list[it.b, it.c), one per wave at a time.
@@ -439,8 +482,12 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
                 const int32_t x = a.small_list[s];
                 const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
                 const Table c = build_wave_table(tab, a.adj, xb, d);
-                if (c.linear) hits += stream_spans<kLinearProbe, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
-                else hits += stream_spans<kCuckoo, U>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex, st);
+                if (c.linear)
+                    hits += stream_spans<kLinearProbe, U, kPerEdge>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min,
+                                                                    ex, st, a.wedge, a.eout, pe);
+                else
+                    hits += stream_spans<kCuckoo, U, kPerEdge>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex,
+                                                               st, a.wedge, a.eout, pe);
                 rtgd::wave_sync_lds();
             }
         }
@@ -458,15 +505,20 @@ __global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
 
 using BatchKernel = void (*)(BatchArgs);
 
-// Instantiated (block size, unroll) pairs of the batched kernel.
-BatchKernel batched_kernel(int threads, int unroll) {
-    if (threads == 256 && unroll == 2) return tc_batched<256, 2>;
-    if (threads == 256 && unroll == 4) return tc_batched<256, 4>;
-    if (threads == 256 && unroll == 8) return tc_batched<256, 8>;
-    if (threads == 512 && unroll == 2) return tc_batched<512, 2>;
-    if (threads == 512 && unroll == 4) return tc_batched<512, 4>;
-    if (threads == 512 && unroll == 8) return tc_batched<512, 8>;
+// Instantiated (block size, unroll) pairs of the batched kernel, counting or per-edge.
+template <bool kPerEdge>
+BatchKernel batched_kernel_t(int threads, int unroll) {
+    if (threads == 256 && unroll == 2) return tc_batched<256, 2, kPerEdge>;
+    if (threads == 256 && unroll == 4) return tc_batched<256, 4, kPerEdge>;
+    if (threads == 256 && unroll == 8) return tc_batched<256, 8, kPerEdge>;
+    if (threads == 512 && unroll == 2) return tc_batched<512, 2, kPerEdge>;
+    if (threads == 512 && unroll == 4) return tc_batched<512, 4, kPerEdge>;
+    if (threads == 512 && unroll == 8) return tc_batched<512, 8, kPerEdge>;
     return nullptr;
+}
+
+BatchKernel batched_kernel(int threads, int unroll, bool per_edge = false) {
+    return per_edge ? batched_kernel_t<true>(threads, unroll) : batched_kernel_t<false>(threads, unroll);
 }
 
 void launch_batched(BatchKernel k, int grid, int threads, size_t lds, hipStream_t st, const BatchArgs &a) {
@@ -509,7 +561,7 @@ __global__ void plan_owner_counts(const int32_t *__restrict__ off, const int32_t
 __global__ void plan_scatter(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
                              const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
                              const uint32_t *__restrict__ wofs, uint32_t *__restrict__ cursor,
-                             int2 *__restrict__ wspan) {
+                             int2 *__restrict__ wspan, uint32_t *__restrict__ wedge) {
     for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int32_t u = row[e], v = adj[e];
@@ -519,6 +571,7 @@ __global__ void plan_scatter(const int32_t *__restrict__ off, const int32_t *__r
         const int32_t owner = uo ? u : v, partner = uo ? v : u;
         const uint32_t slot = atomicAdd(&cursor[owner], 1u);
         wspan[wofs[owner] + slot] = make_int2(off[partner], uo ? dv : du);
+        if (wedge) wedge[wofs[owner] + slot] = (uint32_t)e;
     }
 }
 
@@ -626,10 +679,11 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     p->merge_bytes = hb[1];
     p->n_work_edges = (int64_t)hb[2];
     RTG_TRY(dev_alloc(&p->d_wspan, (size_t)std::max<int64_t>(p->n_work_edges, 1)));
+    if (p->opts.per_edge) RTG_TRY(dev_alloc(&p->d_wedge, (size_t)std::max<int64_t>(p->n_work_edges, 1)));
     HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (size_t)(n + 1), st));
     if (ne > 0) {
         plan_scatter<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end,
-                                                                (const uint32_t *)p->d_wofs, d_cnt, p->d_wspan);
+                                                                (const uint32_t *)p->d_wofs, d_cnt, p->d_wspan, p->d_wedge);
         HIP_TRY(hipGetLastError());
     }
     p->hoff.resize((size_t)n + 1);
@@ -778,6 +832,14 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
     p->block_threads = threads;
     p->wave_owner_max = wave_owner_max;
     p->kernel = (void *)k;
+    if (p->opts.per_edge) {
+        BatchKernel ke = batched_kernel(threads, tune.unroll, true);
+        int per_cu_e = 0;
+        HIP_TRY(hipFuncSetAttribute((const void *)ke, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_e, ke, threads, lds));
+        p->kernel_pe = (void *)ke;
+        p->grid_pe = (int32_t)std::min<int64_t>((int64_t)std::max(per_cu_e, 1) * ctx->cu_count, std::max<int64_t>(p->n_items, 1));
+    }
     p->unroll = tune.unroll;
     p->long_min = tune.long_min;
     p->grab = tune.grab;
@@ -948,7 +1010,7 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
             if (p->n_items == 0) break;
             BatchArgs a{g->d_offsets, g->d_adjs, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
                         p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
-                        p->long_min, p->grab};
+                        p->long_min, p->grab, nullptr, nullptr};
             const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
             launch_batched((BatchKernel)p->kernel, p->grid, p->block_threads, lds, st, a);
             break;
@@ -956,6 +1018,26 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
         default:
             return rtg::set_error(RTG_EINVAL, "unknown TC method");
     }
+    HIP_TRY(hipGetLastError());
+    return RTG_OK;
+}
+
+// Per-edge counts of the plan's edge range: the range of eout is cleared, then the per-edge
+// instantiation of the batched kernel writes every span's count at its edge (edges whose
+// shorter list is empty keep 0). The total is accumulated as in a count.
+int launch_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *eout) {
+    const rtg_graph *g = p->g;
+    hipStream_t st = ctx->stream;
+    const int64_t ne = p->e_end - p->e_begin;
+    if (ne > 0) HIP_TRY(hipMemsetAsync(eout + p->e_begin, 0, sizeof(uint32_t) * (size_t)ne, st));
+    reset_counters<<<1, 64, 0, st>>>(p->d_count);
+    HIP_TRY(hipGetLastError());
+    if (p->n_items == 0) return RTG_OK;
+    BatchArgs a{g->d_offsets, g->d_adjs, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
+                p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
+                p->long_min, p->grab, p->d_wedge, eout};
+    const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
+    launch_batched((BatchKernel)p->kernel_pe, p->grid_pe, p->block_threads, lds, st, a);
     HIP_TRY(hipGetLastError());
     return RTG_OK;
 }
@@ -1063,6 +1145,29 @@ RTG_API int rtg_tc_plan_run(rtg_ctx *ctx, rtg_tc_plan *p, uint64_t *count, rtg_t
     return RTG_OK;
 }
 
+RTG_API int rtg_tc_plan_run_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *d_out, uint64_t *count,
+                                     rtg_tc_stats *stats) {
+    RTG_TRY(rtg::ctx_enter(ctx));
+    if (!p || !d_out) return rtg_fail(RTG_EINVAL, "rtg_tc_plan_run_per_edge: null argument");
+    if (p->opts.method != RTG_TC_BATCHED || !p->opts.per_edge)
+        return rtg_fail(RTG_EINVAL, "rtg_tc_plan_run_per_edge: the plan was not created as a batched per-edge plan");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipEventRecord(ctx->ev[0], st));
+    RTG_TRY(launch_per_edge(ctx, p, d_out));
+    HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpyAsync(&c, p->d_count, sizeof(c), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (count) *count = c;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        fill_stats(p, stats);
+        stats->t_count_ms = rtg::elapsed_ms(ctx->ev[0], ctx->ev[1]);
+    }
+    return RTG_OK;
+}
+
 RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
     if (!p || !buf || len == 0) return rtg_fail(RTG_EINVAL, "rtg_tc_plan_describe: bad arguments");
     std::string d = "method=" + std::to_string(p->opts.method);
@@ -1082,7 +1187,7 @@ RTG_API int rtg_tc_plan_destroy(rtg_tc_plan *p) {
     if (p->ctx) (void)hipSetDevice(p->ctx->device);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    void *bufs[] = {p->d_row, p->d_wofs, p->d_wspan, p->d_small, p->d_items, p->d_count};
+    void *bufs[] = {p->d_row, p->d_wofs, p->d_wspan, p->d_wedge, p->d_small, p->d_items, p->d_count};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;
@@ -1093,16 +1198,26 @@ RTG_API int rtg_tc_per_edge(rtg_ctx *ctx, const rtg_graph *g, int32_t form, uint
     RTG_TRY(rtg::ctx_enter(ctx));
     if (!g || (!d_out && g->m > 0)) return rtg_fail(RTG_EINVAL, "rtg_tc_per_edge: null argument");
     if (!g->oriented) return rtg_fail(RTG_EINVAL, "rtg_tc_per_edge: graph is not degree-oriented");
-    if (form != 0 && form != 1) return rtg_fail(RTG_EINVAL, "rtg_tc_per_edge: form must be 0 or 1");
-    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (form < 0 || form > 2) return rtg_fail(RTG_EINVAL, "rtg_tc_per_edge: form must be 0, 1 or 2");
     if (g->m == 0) return RTG_OK;
+    if (form == 0) {  // a one-off batched per-edge plan (default configuration, no capture)
+        rtg_tc_opts o{};
+        o.method = RTG_TC_BATCHED;
+        o.per_edge = 1;
+        rtg_tc_plan *p = nullptr;
+        RTG_TRY(rtg_tc_plan_create(ctx, g, &o, &p, nullptr));
+        const int rc = rtg_tc_plan_run_per_edge(ctx, p, d_out, nullptr, nullptr);
+        rtg_tc_plan_destroy(p);
+        return rc;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
     hipStream_t st = ctx->stream;
     int32_t *row = nullptr;
     RTG_TRY(dev_alloc(&row, (size_t)g->m));
     DevFree fr;
     fr.p = row;
     edge_rows_kernel<<<grid_cap(ctx, g->n, 256, 16), 256, 0, st>>>(g->d_offsets, g->n, row);
-    if (form == 0)
+    if (form == 2)
         tc_per_edge_intersect<<<grid_cap(ctx, g->m * 64, kPlainThreads, 32), kPlainThreads, 0, st>>>(
             g->d_offsets, g->d_adjs, row, g->m, d_out);
     else

@@ -25,6 +25,7 @@ struct rtg_tc_plan {
     int32_t *d_row = nullptr;        // edge -> source row (all methods)
     int32_t *d_wofs = nullptr;       // batched: partner-list offsets per owner (n+1)
     int2 *d_wspan = nullptr;         // batched: partner out-list spans {start, length} by owner
+    uint32_t *d_wedge = nullptr;     // batched, per-edge plans: oriented edge of each span
     int32_t *d_small = nullptr;      // batched: small owners, heaviest first
     TcItem *d_items = nullptr;       // batched: work queue, heaviest first
     unsigned long long *d_count = nullptr;  // [0] = triangle count, [1] = queue head
@@ -40,6 +41,8 @@ struct rtg_tc_plan {
     int32_t grid = 0;
     int32_t unroll = 4;
     void *kernel = nullptr;          // batched: selected kernel instantiation
+    void *kernel_pe = nullptr;       // batched, per-edge plans: the per-edge instantiation
+    int32_t grid_pe = 0;
     // byte model
     uint64_t alg_bytes = 0, merge_bytes = 0;
     // captured launch sequence

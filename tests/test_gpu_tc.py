@@ -107,8 +107,57 @@ def test_two_hop_2a1(ctx):
 def test_per_edge_counts(ctx, name):
     off, adj, _ = oracle_csr(name)
     g = rtg.DeviceGraph.from_csr(ctx, off, adj)
-    for form in (0, 1):
-        assert np.array_equal(g.tc_per_edge(form), orc.tc_per_edge(off, adj, form))
+    for form, oform in ((0, 0), (1, 1), (2, 0)):  # 0 batched plan and 2 wave per edge: a->b; 1: a->c
+        assert np.array_equal(g.tc_per_edge(form), orc.tc_per_edge(off, adj, oform)), form
+
+
+@pytest.mark.parametrize("name", list(GRAPHS))
+def test_per_edge_plan(ctx, name):
+    """Batched per-edge plans (every span's count stored at its edge) on every test graph,
+    with and without an edge range; run twice to check that the output range is cleared."""
+    off, adj, t = oracle_csr(name)
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    exp = orc.tc_per_edge(off, adj, 0)
+    m = adj.size
+    plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=False, per_edge=True)
+    for _ in range(2):
+        out, cnt, st = plan.run_per_edge()
+        assert cnt == t and np.array_equal(out.numpy()[:m], exp)
+    assert plan.run()[0] == t  # the plan still counts
+    plan.close()
+    if m > 10:
+        a, b = m // 3, m - m // 4
+        d = rtg.DeviceArray(ctx, (m,), np.uint32)
+        d.copy_from(np.full(m, 7, np.uint32))
+        p2 = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=False, e_begin=a, e_end=b, per_edge=True)
+        _, cnt, _ = p2.run_per_edge(d)
+        got = d.numpy()
+        assert cnt == int(exp[a:b].sum())
+        assert np.array_equal(got[a:b], exp[a:b]) and np.all(got[:a] == 7) and np.all(got[b:] == 7)
+        p2.close()
+
+
+def test_per_edge_plan_search_items(ctx, monkeypatch):
+    """Per-edge counts from owners too long for the LDS table (search items): a cap of 64 slots
+    sends every owner there."""
+    for name in ("hub", "plcm"):
+        off, adj, t = oracle_csr(name)
+        g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+        monkeypatch.setenv("RTG_TC_TUNE", "threads=256,unroll=8,item=65536,wave_item=4096,long=128,grab=16,table_cap=64")
+        plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=False, per_edge=True)
+        assert int(plan.describe()["search_items"]) > 0
+        out, cnt, _ = plan.run_per_edge()
+        assert cnt == t and np.array_equal(out.numpy()[:adj.size], orc.tc_per_edge(off, adj, 0)), name
+        plan.close()
+
+
+def test_per_edge_plan_rejects_count_plan(ctx):
+    off, adj, _ = oracle_csr("rmat_s12")
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    plan = g.tc_plan(method=rtg.TC_BATCHED)
+    with pytest.raises(rtg.RTGError):
+        plan.run_per_edge()
+    plan.close()
 
 
 def test_edge_ranges_partition(ctx):
