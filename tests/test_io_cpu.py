@@ -38,6 +38,30 @@ def test_text_pairs_like_txt2bin(tmp_path):
     assert rtg.read_binary_pairs(out).tolist() == [1, 2, 4, 5, -5, 7, 8, 9]
 
 
+def test_text_pairs_int_extraction_edge_cases(tmp_path):
+    """Each line is read as `stream >> int32 >> int32` reads it: leading blanks and CR skipped,
+    an optional sign, digits up to the first other character, int32 bounds, and anything after
+    the second value ignored; the last line needs no newline."""
+    text = ("1 2\r\n+5 -6\n12abc 5\n5 8.9\n0x10 5\n  \t \n2147483647 -2147483648\n"
+            "2147483648 0\n-2147483649 0\n7\t\t8 junk\n-0 +0\n- 1\n9 10")
+    got = rtg.read_text_pairs(write(tmp_path, "edge.txt", text)).tolist()
+    assert got == [1, 2, 5, -6, 5, 8, 2147483647, -2147483648, 7, 8, 0, 0, 9, 10]
+
+
+def test_text_and_mtx_readers_at_scale(tmp_path):
+    rng = np.random.default_rng(4)
+    pairs = rng.integers(0, 1 << 20, size=(200000, 2)).astype(np.int32)
+    pt = tmp_path / "g.txt"
+    np.savetxt(pt, pairs, fmt="%d")
+    assert np.array_equal(rtg.read_text_pairs(pt), pairs.reshape(-1))
+    pm = tmp_path / "g.mtx"
+    with open(pm, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate integer general\n% generated\n")
+        f.write(f"{1 << 20} {1 << 20} {pairs.shape[0]}\n")
+        np.savetxt(f, np.column_stack([pairs + 1, rng.integers(-5, 5, size=pairs.shape[0])]), fmt="%d")
+    assert np.array_equal(rtg.read_mtx_pairs(pm), pairs.reshape(-1))
+
+
 def test_mtx_pattern_symmetric_expands_off_diagonal(tmp_path):
     p = write(tmp_path, "s.mtx", "%%MatrixMarket matrix coordinate pattern symmetric\n% c\n3 3 3\n1 2\n2 3\n3 3\n")
     # (r, c) then (c, r) for off-diagonal entries, diagonal once, 1-based -> 0-based
