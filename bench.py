@@ -175,14 +175,23 @@ def si_leg(rtg, ctx) -> dict:
             if m == rtg.SI_AUTO:
                 auto = st["strategy"]
         res[name] = sorted(ts)[2]
+        got = out.numpy()
         out.free()
     for a in (d_off, d_el, d_p):
         a.free()
+    orc = oracle_module()
+    t0 = time.perf_counter()
+    exp = orc.si_pairs(offs, el, pairs, ORACLE_THREADS[0])
+    t_cpu = (time.perf_counter() - t0) * 1e3
+    if not np.array_equal(got, exp):
+        raise RuntimeError("set intersection (Zipf pairs): device counts differ from the oracle")
     log(f"set intersection (2^20 Zipf pairs): plain {res['plain']:.3f} ms, batched {res['batched']:.3f} ms, "
-        f"autotuned {res['auto']:.3f} ms ({auto})")
+        f"autotuned {res['auto']:.3f} ms ({auto}); oracle {t_cpu:.1f} ms on {ORACLE_THREADS[0]} threads")
     return {"pairs": 1 << 20, "elements": int(el.size), "plain_ms": res["plain"], "batched_ms": res["batched"],
             "batched_over_plain": res["plain"] / res["batched"], "auto_ms": res["auto"], "auto_strategy": auto,
             "alg_bytes": st["alg_bytes"],
+            "cpu_baseline": {"ms": t_cpu, "threads": ORACLE_THREADS[0], "kind": "port",
+                             "sample": "the whole workload, one call of the oracle's pairwise intersection"},
             "tile": si_tile_leg(rtg, ctx)}
 
 
@@ -209,11 +218,20 @@ def si_tile_leg(rtg, ctx) -> dict:
         alg = st["alg_bytes"]
     for a in (out, d_el, d_off):
         a.free()
+    orc = oracle_module()
+    t0 = time.perf_counter()
+    exp = orc.si_tile(ds.set_offsets, ds.elements, 1000, 1, ORACLE_THREADS[0])
+    t_cpu = (time.perf_counter() - t0) * 1e3
+    if not np.all(exp == ds.intersection):
+        raise RuntimeError("set-intersection tile: the oracle disagrees with the generator's invariant")
     best = min(v for k, v in ms.items() if k != "auto")
-    log("set intersection tile (1000 x 1 sets of 100,000): " + ", ".join(f"{k} {v:.3f} ms" for k, v in ms.items()))
+    log("set intersection tile (1000 x 1 sets of 100,000): " + ", ".join(f"{k} {v:.3f} ms" for k, v in ms.items()) +
+        f"; oracle {t_cpu:.1f} ms on {ORACLE_THREADS[0]} threads")
     return {"workload": "1000 A sets x 1 B set, length 100000, selectivity 0.1, density 0.1", "ms": ms,
             "auto_strategy": auto, "alg_bytes": alg, "alg_gbs_best": alg / best / 1e6,
-            "elements": int(ds.elements.size)}
+            "elements": int(ds.elements.size),
+            "cpu_baseline": {"ms": t_cpu, "threads": ORACLE_THREADS[0], "kind": "port",
+                             "sample": "the whole tile, one call of the oracle's tile intersection"}}
 
 
 def per_edge_leg(rtg, ctx, g, count: int, alg_bytes: int, tune_cache) -> dict:
