@@ -104,6 +104,21 @@ int rtg_graph_download(const rtg_graph *g, int32_t *offsets /* n+1 */, int32_t *
 int rtg_graph_device_ptrs(const rtg_graph *g, const int32_t **d_offsets, const int32_t **d_adjs);
 int rtg_graph_destroy(rtg_graph *g);
 
+/* Preprocessed-CSR cache (checkpoint / resume of ingest + preprocessing; the reference keeps a
+   text -> binary edge cache, tc/graph/graph.cpp:179-214). A file holds a header (magic,
+   version, oriented flag, n, m, a caller-defined key, a checksum) and the CSR arrays; a read
+   with another key, a bad checksum or a wrong size fails with RTG_EIO. `key` identifies the
+   input, e.g. rtg_pairs_fingerprint(pairs, npairs, flags). Writes go to "<path>.part" and are
+   renamed into place. rtg_csr_file_read uses the two-call convention (offsets == NULL: sizes
+   only); both file calls are host-only.                                                      */
+int rtg_pairs_fingerprint(const int32_t *pairs, int64_t npairs, int32_t flags, uint64_t *out);
+int rtg_csr_file_write(const char *path, uint64_t key, int32_t oriented, const int32_t *offsets,
+                       const int32_t *adjs, int32_t n, int64_t m);
+int rtg_csr_file_read(const char *path, uint64_t key, int32_t *offsets, int32_t *adjs, int32_t *n,
+                      int64_t *m, int32_t *oriented);
+int rtg_graph_save(const rtg_graph *g, const char *path, uint64_t key);
+int rtg_graph_load(rtg_ctx *ctx, const char *path, uint64_t key, rtg_graph **out);
+
 /* Device-side synthetic inputs (same values as the host generators below). */
 int rtg_gen_rmat_device(rtg_ctx *ctx, uint32_t scale, int64_t nsamples, double a, double b,
                         double c, uint64_t seed, int32_t *d_pairs /* 2*nsamples */);

@@ -33,6 +33,7 @@ __all__ = [
     "SI_METHODS", "gen_si_dataset", "gen_ba", "gen_rmat", "gen_powerlaw_degrees",
     "gen_config_model", "gen_zipf_lengths", "gen_sorted_lists", "gen_sorted_array",
     "gen_uniform_u32", "read_binary_pairs", "read_text_pairs", "read_mtx_pairs", "lib_path",
+    "pairs_fingerprint", "csr_file_write", "csr_file_read",
 ]
 
 _HERE = Path(__file__).resolve().parent
@@ -253,6 +254,18 @@ class DeviceGraph:
                                        C.c_int64(adjs.size), C.byref(h)))
         return cls(ctx, h)
 
+    @classmethod
+    def load(cls, ctx: Context, path, key: int = 0) -> "DeviceGraph":
+        """rtg_graph_load: a CSR cached by save(); RTGError(RTG_EIO) if the key, checksum or size
+        does not match."""
+        h = C.c_void_p()
+        _check(_lib.rtg_graph_load(ctx.handle, os.fsencode(path), C.c_uint64(key), C.byref(h)))
+        return cls(ctx, h)
+
+    def save(self, path, key: int = 0) -> None:
+        """rtg_graph_save: the CSR to a cache file tagged with `key` (e.g. pairs_fingerprint)."""
+        _check(_lib.rtg_graph_save(self._h, os.fsencode(path), C.c_uint64(key)))
+
     def download(self) -> tuple[np.ndarray, np.ndarray]:
         off = np.empty(self.n + 1, dtype=np.int32)
         adj = np.empty(self.m, dtype=np.int32)
@@ -372,6 +385,35 @@ def _read_pairs(fn, path) -> np.ndarray:
     out = np.empty(2 * n.value, dtype=np.int32)
     _check(fn(p, _ptr(out) if out.size else None, C.byref(n)))
     return out[: 2 * n.value]
+
+
+def pairs_fingerprint(pairs: np.ndarray, flags: int = PAIRS_RELABEL | PAIRS_ORIENT) -> int:
+    """rtg_pairs_fingerprint: a 64-bit key for a pair list and its preprocessing flags."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+    out = C.c_uint64()
+    _check(_lib.rtg_pairs_fingerprint(_ptr(pairs) if pairs.size else None, C.c_int64(pairs.size // 2),
+                                      C.c_int32(flags), C.byref(out)))
+    return out.value
+
+
+def csr_file_write(path, offsets: np.ndarray, adjs: np.ndarray, key: int = 0, oriented: bool = True) -> None:
+    """rtg_csr_file_write (host only): a CSR to a cache file."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+    adjs = np.ascontiguousarray(adjs, dtype=np.int32)
+    _check(_lib.rtg_csr_file_write(os.fsencode(path), C.c_uint64(key), C.c_int32(int(oriented)), _ptr(offsets),
+                                   _ptr(adjs) if adjs.size else None, C.c_int32(offsets.size - 1),
+                                   C.c_int64(adjs.size)))
+
+
+def csr_file_read(path, key: int = 0) -> tuple[np.ndarray, np.ndarray, bool]:
+    """rtg_csr_file_read (host only): (offsets, adjs, oriented) of a cache file."""
+    p = os.fsencode(path)
+    n, m, o = C.c_int32(), C.c_int64(), C.c_int32()
+    _check(_lib.rtg_csr_file_read(p, C.c_uint64(key), None, None, C.byref(n), C.byref(m), C.byref(o)))
+    off = np.empty(n.value + 1, dtype=np.int32)
+    adj = np.empty(max(m.value, 1), dtype=np.int32)
+    _check(_lib.rtg_csr_file_read(p, C.c_uint64(key), _ptr(off), _ptr(adj), C.byref(n), C.byref(m), C.byref(o)))
+    return off, adj[: m.value], bool(o.value)
 
 
 def read_text_pairs(path: str | os.PathLike) -> np.ndarray:

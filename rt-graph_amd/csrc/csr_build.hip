@@ -285,6 +285,13 @@ int csr_max_row(rtg_ctx *ctx, const int32_t *d_offsets, int32_t n, int32_t *out)
 
 }  // namespace rtg
 
+namespace rtg {
+bool graph_oriented(const rtg_graph *g) { return g && g->oriented; }
+void graph_set_oriented(rtg_graph *g, bool oriented) {
+    if (g) g->oriented = oriented;
+}
+}  // namespace rtg
+
 extern "C" {
 
 RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npairs, uint32_t flags,

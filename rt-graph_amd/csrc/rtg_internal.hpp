@@ -40,6 +40,11 @@ void parallel_for(int64_t n, F f, int64_t min_chunk = 4096) {
     for (auto &t : ts) t.join();
 }
 
+// Whether a device graph is degree-oriented, and setting it (csr_build.hip; for graphs stored
+// to and loaded from a file).
+bool graph_oriented(const rtg_graph *g);
+void graph_set_oriented(rtg_graph *g, bool oriented);
+
 }  // namespace rtg
 
 inline int rtg_fail(int code, const std::string &msg) { return rtg::set_error(code, msg); }

@@ -338,3 +338,26 @@ def test_bs_tc_command_line(tmp_path):
         r = subprocess.run([str(cli), str(tmp_path / name), "0"], capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, (name, r.stdout, r.stderr)
         assert f"counting result = {t & 0xFFFFFFFF}" in r.stdout, (name, r.stdout)
+
+
+def test_csr_cache_save_load(ctx, tmp_path):
+    """A preprocessed graph saved to the CSR cache and loaded back counts the same; the
+    symmetric (unoriented) form keeps its flag; a stale key is refused."""
+    pairs = GRAPHS["rmat_s12"]
+    key = rtg.pairs_fingerprint(pairs)
+    g = rtg.DeviceGraph.from_pairs(ctx, pairs)
+    t = g.tc_count()[0]
+    g.save(tmp_path / "g.csr", key)
+    g2 = rtg.DeviceGraph.load(ctx, tmp_path / "g.csr", key)
+    o1, a1 = g.download()
+    o2, a2 = g2.download()
+    assert np.array_equal(o1, o2) and np.array_equal(a1, a2) and g2.tc_count()[0] == t
+    with pytest.raises(rtg.RTGError):
+        rtg.DeviceGraph.load(ctx, tmp_path / "g.csr", key ^ 1)
+    s = rtg.DeviceGraph.from_pairs(ctx, pairs, orient=False)
+    s.save(tmp_path / "s.csr", 5)
+    off, adj, oriented = rtg.csr_file_read(tmp_path / "s.csr", 5)
+    assert not oriented
+    s2 = rtg.DeviceGraph.load(ctx, tmp_path / "s.csr", 5)
+    with pytest.raises(rtg.RTGError):  # counting needs a degree-oriented graph
+        s2.tc_count()

@@ -6,7 +6,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import load_rtg
+from conftest import load_oracle, load_rtg
 
 rtg = load_rtg()
 
@@ -93,3 +93,35 @@ def test_missing_files(tmp_path):
     for fn in (rtg.read_text_pairs, rtg.read_mtx_pairs):
         with pytest.raises(rtg.RTGError):
             fn(tmp_path / "missing")
+
+
+def test_csr_cache_file_round_trip_and_rejections(tmp_path):
+    """The preprocessed-CSR cache: exact round trip, and refusal of another key, a flipped
+    byte, a truncated file and a file of another kind."""
+    orc = load_oracle()
+    pairs = rtg.gen_ba(2000, 4, seed=3)
+    off, adj = orc.csr_from_pairs(pairs)
+    key = rtg.pairs_fingerprint(pairs)
+    assert key == rtg.pairs_fingerprint(pairs.copy()) and key != rtg.pairs_fingerprint(pairs, flags=1)
+    p = tmp_path / "g.csr"
+    rtg.csr_file_write(p, off, adj, key=key)
+    o2, a2, oriented = rtg.csr_file_read(p, key=key)
+    assert oriented and np.array_equal(o2, off) and np.array_equal(a2, adj)
+    assert not (tmp_path / "g.csr.part").exists()
+    with pytest.raises(rtg.RTGError):
+        rtg.csr_file_read(p, key=key + 1)
+    raw = bytearray(p.read_bytes())
+    raw[100] ^= 1
+    (tmp_path / "flip.csr").write_bytes(bytes(raw))
+    with pytest.raises(rtg.RTGError):
+        rtg.csr_file_read(tmp_path / "flip.csr", key=key)
+    (tmp_path / "short.csr").write_bytes(p.read_bytes()[:-4])
+    with pytest.raises(rtg.RTGError):
+        rtg.csr_file_read(tmp_path / "short.csr", key=key)
+    with pytest.raises(rtg.RTGError):
+        rtg.csr_file_read(write(tmp_path, "x.csr", "%%MatrixMarket matrix coordinate pattern general\n"), key=key)
+    # an empty graph and a symmetric (unoriented) one
+    rtg.csr_file_write(tmp_path / "e.csr", np.zeros(1, np.int32), np.zeros(0, np.int32), key=7, oriented=False)
+    o3, a3, or3 = rtg.csr_file_read(tmp_path / "e.csr", key=7)
+    assert o3.tolist() == [0] and a3.size == 0 and not or3
+
