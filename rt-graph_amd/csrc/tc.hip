@@ -707,16 +707,31 @@ std::vector<int64_t> order_heaviest_first(const std::vector<uint64_t> &key) {
         const int frac = e >= 4 ? (int)((k >> (e - 4)) & 15u) : (int)((k << (4 - e)) & 15u);
         return 1 + e * kSub + frac;
     };
-    std::vector<int64_t> start(kBuckets, 0);
-    for (uint64_t k : key) ++start[bucket(k)];
+    // Per-range histograms, then bucket starts (heaviest bucket first; within a bucket by range,
+    // so by index: the order does not depend on the thread count), then a parallel scatter.
+    const int64_t n = (int64_t)key.size();
+    const int64_t nr = std::max<int64_t>(1, std::min<int64_t>(rtg::host_threads(), n / 65536));
+    std::vector<std::vector<int64_t>> start((size_t)nr, std::vector<int64_t>(kBuckets, 0));
+    std::vector<uint16_t> bk((size_t)n);
+    rtg::parallel_for(nr, [&](int64_t r0, int64_t r1) {
+        for (int64_t r = r0; r < r1; ++r)
+            for (int64_t i = n * r / nr; i < n * (r + 1) / nr; ++i) {
+                bk[(size_t)i] = (uint16_t)bucket(key[(size_t)i]);
+                ++start[(size_t)r][bk[(size_t)i]];
+            }
+    }, 1);
     int64_t acc = 0;
-    for (int b = kBuckets - 1; b >= 0; --b) {
-        const int64_t c = start[b];
-        start[b] = acc;
-        acc += c;
-    }
-    std::vector<int64_t> out(key.size());
-    for (size_t i = 0; i < key.size(); ++i) out[start[bucket(key[i])]++] = (int64_t)i;
+    for (int b = kBuckets - 1; b >= 0; --b)
+        for (int64_t r = 0; r < nr; ++r) {
+            const int64_t c = start[(size_t)r][b];
+            start[(size_t)r][b] = acc;
+            acc += c;
+        }
+    std::vector<int64_t> out((size_t)n);
+    rtg::parallel_for(nr, [&](int64_t r0, int64_t r1) {
+        for (int64_t r = r0; r < r1; ++r)
+            for (int64_t i = n * r / nr; i < n * (r + 1) / nr; ++i) out[(size_t)start[(size_t)r][bk[(size_t)i]]++] = i;
+    }, 1);
     return out;
 }
 
@@ -754,28 +769,60 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
     while (2 * (int32_t)cuckoo_half(fit + 1) <= wave_table) ++fit;
     const int32_t wave_owner_max = tune.wave_owner_max >= 0 ? std::min(tune.wave_owner_max, fit) : fit;
 
+    // Owners are classified over contiguous ranges in parallel; the ranges' lists are joined in
+    // order, so the lists are the same as a serial pass would give.
+    struct Part {
+        std::vector<TcItem> items;
+        std::vector<uint64_t> cost;
+        std::vector<std::pair<uint64_t, int32_t>> small;  // (work, owner)
+        int64_t search = 0;
+    };
+    const int64_t nparts = std::max<int64_t>(1, std::min<int64_t>(host_threads(), ((int64_t)n + 65535) / 65536));
+    std::vector<Part> parts((size_t)nparts);
+    parallel_for(nparts, [&](int64_t c0, int64_t c1) {
+        for (int64_t cpart = c0; cpart < c1; ++cpart) {
+            Part &pt = parts[(size_t)cpart];
+            const int32_t x0 = (int32_t)((int64_t)n * cpart / nparts), x1 = (int32_t)((int64_t)n * (cpart + 1) / nparts);
+            for (int32_t x = x0; x < x1; ++x) {
+                const int32_t c = hcnt[x + 1] - hcnt[x];
+                if (c == 0) continue;
+                const int32_t d = hoff[x + 1] - hoff[x];
+                const uint64_t w = hwork[x];
+                if (d <= wave_owner_max) {
+                    pt.small.push_back({w + (uint64_t)d, x});
+                    continue;
+                }
+                int64_t chunks = std::max<int64_t>(1, (int64_t)((w + tune.item_work / 2) / tune.item_work));
+                chunks = std::min<int64_t>(chunks, c);
+                const bool search = d > fit_block;  // no LDS table: binary searches, ~log2(d) reads each
+                const uint64_t per = search ? (uint64_t)(32 - __builtin_clz((uint32_t)d)) / 2 : 1;
+                for (int64_t k = 0; k < chunks; ++k) {
+                    const int32_t b = hcnt[x] + (int32_t)(c * k / chunks), e = hcnt[x] + (int32_t)(c * (k + 1) / chunks);
+                    pt.items.push_back(TcItem{search ? 2 : 0, x, b, e});
+                    pt.cost.push_back(w / chunks * per + (uint64_t)d);
+                    pt.search += search ? 1 : 0;
+                }
+            }
+        }
+    }, 1);
     std::vector<TcItem> items;
     std::vector<std::pair<uint64_t, int32_t>> small;  // (work, owner)
-    p->n_search_items = 0;
     std::vector<uint64_t> item_cost;
-    for (int32_t x = 0; x < n; ++x) {
-        const int32_t c = hcnt[x + 1] - hcnt[x];
-        if (c == 0) continue;
-        const int32_t d = hoff[x + 1] - hoff[x];
-        const uint64_t w = hwork[x];
-        if (d <= wave_owner_max) {
-            small.push_back({w + (uint64_t)d, x});
-            continue;
+    {
+        size_t ni = 0, ns = 0;
+        for (const Part &pt : parts) {
+            ni += pt.items.size();
+            ns += pt.small.size();
         }
-        int64_t chunks = std::max<int64_t>(1, (int64_t)((w + tune.item_work / 2) / tune.item_work));
-        chunks = std::min<int64_t>(chunks, c);
-        const bool search = d > fit_block;  // no LDS table: binary searches, ~log2(d) reads each
-        const uint64_t per = search ? (uint64_t)(32 - __builtin_clz((uint32_t)d)) / 2 : 1;
-        for (int64_t k = 0; k < chunks; ++k) {
-            const int32_t b = hcnt[x] + (int32_t)(c * k / chunks), e = hcnt[x] + (int32_t)(c * (k + 1) / chunks);
-            items.push_back(TcItem{search ? 2 : 0, x, b, e});
-            item_cost.push_back(w / chunks * per + (uint64_t)d);
-            p->n_search_items += search ? 1 : 0;
+        items.reserve(ni);
+        item_cost.reserve(ni);
+        small.reserve(ns);
+        p->n_search_items = 0;
+        for (const Part &pt : parts) {
+            items.insert(items.end(), pt.items.begin(), pt.items.end());
+            item_cost.insert(item_cost.end(), pt.cost.begin(), pt.cost.end());
+            small.insert(small.end(), pt.small.begin(), pt.small.end());
+            p->n_search_items += pt.search;
         }
     }
     // Heaviest first.
