@@ -527,12 +527,16 @@ void launch_batched(BatchKernel k, int grid, int threads, size_t lds, hipStream_
 
 // ---- planning kernels ----
 
-// Per edge: owner/partner assignment, per-owner partner count and streamed work, and the
-// algorithmic / merge-only byte models.
-__global__ void plan_owner_counts(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                  const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
-                                  uint32_t *__restrict__ cnt, unsigned long long *__restrict__ work,
-                                  unsigned long long *bytes /* [0] alg, [1] merge, [2] edges */) {
+// Per edge of the range [e_begin, e_end): the byte models, and a grouping key — the owner
+// (the endpoint with the longer out-list) in the low `bits`, the edge's index in the range
+// above. `bits` is a whole number of bytes: the radix sort orders by 8-bit digits, so the
+// owner field must fill its last digit (edge bits inside it would split an owner's edges).
+// Edges with an empty list get owner n, so that a radix sort on the low bits (stable) lists
+// every owner's edges together, in edge order, and those edges last.
+__global__ __launch_bounds__(256) void plan_keys(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                 const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
+                                                 int bits, int32_t n, uint64_t *__restrict__ keys,
+                                                 unsigned long long *bytes /* [0] alg, [1] merge, [2] edges */) {
     uint64_t alg = 0, mrg = 0, ne = 0;
     for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
          e += (int64_t)gridDim.x * blockDim.x) {
@@ -542,36 +546,65 @@ __global__ void plan_owner_counts(const int32_t *__restrict__ off, const int32_t
         const uint32_t lg = l ? 32u - __clz(l) : 0u;  // ceil(log2(l+1))
         alg += 24ull + 4ull * min((uint64_t)s + l, (uint64_t)s * (1u + lg));
         mrg += 24ull + 4ull * ((uint64_t)du + dv);
-        if (s == 0) continue;
-        const int32_t owner = (du > dv || (du == dv && u < v)) ? u : v;
-        atomicAdd(&cnt[owner], 1u);
-        atomicAdd(&work[owner], (unsigned long long)s);
-        ne += 1;
+        int32_t owner = n;
+        if (s > 0) {
+            owner = (du > dv || (du == dv && u < v)) ? u : v;
+            ne += 1;
+        }
+        keys[e - e_begin] = ((uint64_t)(e - e_begin) << bits) | (uint64_t)(uint32_t)owner;
     }
+    // one atomic per block and counter
+    __shared__ uint64_t part[4][3];
     alg = rtgd::wave_sum_u64(alg);
     mrg = rtgd::wave_sum_u64(mrg);
     ne = rtgd::wave_sum_u64(ne);
     if (__lane_id() == 0) {
-        atomicAdd(&bytes[0], (unsigned long long)alg);
-        atomicAdd(&bytes[1], (unsigned long long)mrg);
-        atomicAdd(&bytes[2], (unsigned long long)ne);
+        part[threadIdx.x >> 6][0] = alg;
+        part[threadIdx.x >> 6][1] = mrg;
+        part[threadIdx.x >> 6][2] = ne;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint64_t t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w][threadIdx.x];
+        if (t) atomicAdd(&bytes[threadIdx.x], (unsigned long long)t);
     }
 }
 
-__global__ void plan_scatter(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                             const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
-                             const uint32_t *__restrict__ wofs, uint32_t *__restrict__ cursor,
-                             int2 *__restrict__ wspan, uint32_t *__restrict__ wedge) {
-    for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
-         e += (int64_t)gridDim.x * blockDim.x) {
+// Position i of the owner-sorted keys (the first nw are work edges): the partner's out-list
+// span {start, length} (and the edge, for per-edge plans), its length for the work sums, and
+// each owner's first position.
+__global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                           const int32_t *__restrict__ row, int64_t e_begin, const uint64_t *__restrict__ sorted,
+                           int64_t nw, int bits, int2 *__restrict__ wspan, uint32_t *__restrict__ wedge,
+                           uint64_t *__restrict__ lens, uint32_t *__restrict__ first) {
+    const uint64_t mask = (1ull << bits) - 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = sorted[i];
+        const int32_t owner = (int32_t)(key & mask);
+        const int64_t e = e_begin + (int64_t)(key >> bits);
         const int32_t u = row[e], v = adj[e];
-        const int32_t du = off[u + 1] - off[u], dv = off[v + 1] - off[v];
-        if (du == 0 || dv == 0) continue;
-        const bool uo = (du > dv || (du == dv && u < v));
-        const int32_t owner = uo ? u : v, partner = uo ? v : u;
-        const uint32_t slot = atomicAdd(&cursor[owner], 1u);
-        wspan[wofs[owner] + slot] = make_int2(off[partner], uo ? dv : du);
-        if (wedge) wedge[wofs[owner] + slot] = (uint32_t)e;
+        const int32_t partner = owner == u ? v : u;
+        const int32_t ps = off[partner], len = off[partner + 1] - ps;
+        wspan[i] = make_int2(ps, len);
+        if (wedge) wedge[i] = (uint32_t)e;
+        lens[i] = (uint64_t)len;
+        if (i == 0 || (int32_t)(sorted[i - 1] & mask) != owner) first[owner] = (uint32_t)i;
+    }
+}
+
+// At each owner's last position: its partner count and its streamed elements (from the
+// exclusive prefix sums of the span lengths).
+__global__ void plan_runs(const uint64_t *__restrict__ sorted, int64_t nw, int bits, const uint32_t *__restrict__ first,
+                          const uint64_t *__restrict__ lens, const uint64_t *__restrict__ lens_ex, uint32_t *__restrict__ cnt,
+                          unsigned long long *__restrict__ work) {
+    const uint64_t mask = (1ull << bits) - 1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t owner = (int32_t)(sorted[i] & mask);
+        if (i + 1 < nw && (int32_t)(sorted[i + 1] & mask) == owner) continue;
+        const uint32_t f = first[owner];
+        cnt[owner] = (uint32_t)(i + 1 - f);
+        work[owner] = (unsigned long long)(lens_ex[i] + lens[i] - lens_ex[f]);
     }
 }
 
@@ -644,48 +677,68 @@ void tune_from_string(BatchTune &t, const std::string &s) {
 
 // Graph-level part of the batched plan: owner assignment, partner spans grouped by owner, the
 // byte models, and host copies of the per-owner counts that item building needs.
+// Groups the range's work edges by owner without atomics: owner keys, a radix sort on the
+// owner bits (stable, so each owner's partners stay in edge order), spans and run boundaries,
+// then per-owner counts, work and partner offsets; the host gets row offsets, partner offsets
+// and work for the queue.
 int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     using namespace rtg;
     const rtg_graph *g = p->g;
     const int32_t n = g->n;
     hipStream_t st = ctx->stream;
-    uint32_t *d_cnt = nullptr;
-    unsigned long long *d_work = nullptr, *d_bytes = nullptr;
-    DevFree f1, f2, f3, f4;
-    RTG_TRY(dev_alloc(&d_cnt, (size_t)n + 1));
-    f1.p = d_cnt;
-    RTG_TRY(dev_alloc(&d_work, (size_t)n + 1));
-    f2.p = d_work;
-    RTG_TRY(dev_alloc(&d_bytes, 4));
-    f3.p = d_bytes;
+    const int64_t ne = p->e_end - p->e_begin;
+    // owner ids 0 .. n, in whole bytes (the radix sort's digits)
+    const int bits = (std::max(1, 32 - __builtin_clz((uint32_t)n + 1u)) + 7) / 8 * 8;
+    if (ne >= ((int64_t)1 << (63 - bits))) return set_error(RTG_EINVAL, "batched TC: edge range too large for the plan keys");
+    const size_t kn = (size_t)std::max<int64_t>(ne, 1);
+    const size_t bytes = Arena::round(sizeof(uint32_t) * ((size_t)n + 1)) * 2 +
+                         Arena::round(sizeof(unsigned long long) * ((size_t)n + 1)) + Arena::round(64) +
+                         Arena::round(sizeof(uint64_t) * kn) * 4 + Arena::round(radix_sort_work_bytes((int64_t)kn)) +
+                         Arena::round(std::max(scan_work_bytes((int64_t)kn), scan_work_bytes(n + 1))) + 4096;
+    char *buf = nullptr;
+    RTG_TRY(dev_alloc(&buf, bytes));
+    DevFree fbuf;
+    fbuf.p = buf;
+    Arena ar{buf, 0, bytes};
+    uint32_t *d_cnt = ar.take<uint32_t>((size_t)n + 1);
+    uint32_t *d_first = ar.take<uint32_t>((size_t)n + 1);
+    unsigned long long *d_work = ar.take<unsigned long long>((size_t)n + 1);
+    unsigned long long *d_bytes = ar.take<unsigned long long>(4);
+    uint64_t *keys[2] = {ar.take<uint64_t>(kn), ar.take<uint64_t>(kn)};
+    uint64_t *lens = ar.take<uint64_t>(kn);
+    uint64_t *lens_ex = ar.take<uint64_t>(kn);
+    void *sort_work = ar.take<char>(radix_sort_work_bytes((int64_t)kn));
+    void *scan_work = ar.take<char>(std::max(scan_work_bytes((int64_t)kn), scan_work_bytes(n + 1)));
     HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (size_t)(n + 1), st));
     HIP_TRY(hipMemsetAsync(d_work, 0, sizeof(unsigned long long) * (size_t)(n + 1), st));
     HIP_TRY(hipMemsetAsync(d_bytes, 0, sizeof(unsigned long long) * 4, st));
-    const int64_t ne = p->e_end - p->e_begin;
+    int which = 0;
     if (ne > 0) {
-        plan_owner_counts<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin,
-                                                                     p->e_end, d_cnt, d_work, d_bytes);
+        plan_keys<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end, bits,
+                                                             n, keys[0], d_bytes);
         HIP_TRY(hipGetLastError());
+        int passes = 0;
+        RTG_TRY(radix_sort_u64(ctx, keys, ne, bits, &which, &passes, sort_work));
     }
-    RTG_TRY(dev_alloc(&p->d_wofs, (size_t)n + 1));
-    void *scan_work = nullptr;
-    RTG_TRY(dev_alloc((char **)&scan_work, scan_work_bytes(n + 1)));
-    f4.p = scan_work;
-    RTG_TRY(scan_exclusive_u32_w(ctx, d_cnt, (uint32_t *)p->d_wofs, n + 1, nullptr, scan_work));
     unsigned long long hb[4] = {0, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(hb, d_bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     p->alg_bytes = hb[0];
     p->merge_bytes = hb[1];
     p->n_work_edges = (int64_t)hb[2];
-    RTG_TRY(dev_alloc(&p->d_wspan, (size_t)std::max<int64_t>(p->n_work_edges, 1)));
-    if (p->opts.per_edge) RTG_TRY(dev_alloc(&p->d_wedge, (size_t)std::max<int64_t>(p->n_work_edges, 1)));
-    HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (size_t)(n + 1), st));
-    if (ne > 0) {
-        plan_scatter<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end,
-                                                                (const uint32_t *)p->d_wofs, d_cnt, p->d_wspan, p->d_wedge);
+    const int64_t nw = p->n_work_edges;
+    RTG_TRY(dev_alloc(&p->d_wspan, (size_t)std::max<int64_t>(nw, 1)));
+    if (p->opts.per_edge) RTG_TRY(dev_alloc(&p->d_wedge, (size_t)std::max<int64_t>(nw, 1)));
+    if (nw > 0) {
+        plan_spans<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, keys[which],
+                                                              nw, bits, p->d_wspan, p->d_wedge, lens, d_first);
+        HIP_TRY(hipGetLastError());
+        RTG_TRY(scan_exclusive_u64_w(ctx, lens, lens_ex, nw, nullptr, scan_work));
+        plan_runs<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(keys[which], nw, bits, d_first, lens, lens_ex, d_cnt, d_work);
         HIP_TRY(hipGetLastError());
     }
+    RTG_TRY(dev_alloc(&p->d_wofs, (size_t)n + 1));
+    RTG_TRY(scan_exclusive_u32_w(ctx, d_cnt, (uint32_t *)p->d_wofs, n + 1, nullptr, scan_work));
     p->hoff.resize((size_t)n + 1);
     p->hcnt.resize((size_t)n + 1);
     p->hwork.resize((size_t)n + 1);
@@ -693,6 +746,11 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     HIP_TRY(hipMemcpyAsync(p->hwork.data(), d_work, sizeof(unsigned long long) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(p->hcnt.data(), p->d_wofs, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    // The counting kernel reads partner spans through these offsets: refuse a plan whose
+    // offsets are not a partition of the nw spans.
+    bool ok = p->hcnt[0] == 0 && (int64_t)p->hcnt[(size_t)n] == nw;
+    for (int32_t x = 0; ok && x < n; ++x) ok = p->hcnt[(size_t)x] <= p->hcnt[(size_t)x + 1];
+    if (!ok) return set_error(RTG_EHIP, "batched TC: inconsistent partner offsets in the plan");
     return RTG_OK;
 }
 
