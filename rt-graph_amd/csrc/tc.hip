@@ -10,13 +10,15 @@
 //                    shorter of N+(u), N+(v) in the longer.
 //
 // Batched kernel (tc_batched): every edge (u,v) with both lists non-empty is assigned to its
-// owner — the endpoint with the longer out-list — and grouped by owner at plan time. A
+// owner — the endpoint with the longer out-list — and grouped by owner at plan time (a stable
+// radix sort on the owner, so each owner's partners stay in edge order). A
 // persistent grid pulls items from a queue ordered heaviest first. For an item the owner's
 // out-list is inserted into an LDS open-addressing hash table (block-wide for large owners,
 // wave-private for small ones), then the partners' out-lists — always the shorter side — are
 // streamed with coalesced loads, flattened across the 64 lanes of a wave, and probed. Work per
 // edge is min(d+(u), d+(v)) LDS probes and 4*min bytes streamed; the longer list is read once
-// per item instead of once per edge.
+// per item instead of once per edge. The per-edge instantiation (kPerEdge) also stores every
+// partner span's count at its edge.
 #include "tc_kernels.hpp"
 
 #include <algorithm>
