@@ -1,6 +1,6 @@
 /*
- * rtg_compat.h — drop-in replacements, with the reference's own signatures, for the three
- * SIMT entry points of the path. A reference driver links librtg instead of compiling the
+ * rtg_compat.h — drop-in replacements, with the reference's own signatures, for the SIMT
+ * entry points of the path (the count, the lookup and the set-intersection baselines). A reference driver links librtg instead of compiling the
  * CUDA files and includes this header in place of the reference header named on each line.
  *
  * Error behaviour matches the reference: a device or runtime failure prints a message and
@@ -44,12 +44,20 @@ int rtg_binary_search_based_intersection(const uint32_t *set_offsets, const uint
                                          uint32_t num_a, uint32_t num_b, uint32_t block_size,
                                          int device_id, uint32_t *results, double *ms);
 
+/* The same tile with any set-intersection method of rtg.h (RTG_SI_*): the C form behind the
+   other baselines' replacements below (hi.cu, ip.cu, bi_naive.cu, bi_dynamic.cu). */
+int rtg_set_intersection_tile_host(const uint32_t *set_offsets, const uint32_t *elements,
+                                   uint32_t num_a, uint32_t num_b, int32_t method, int device_id,
+                                   uint32_t *results, double *ms);
+
 #ifdef __cplusplus
 }
 
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
+
+#include "rtg.h"
 
 /* BinarySearchBasedIntersection(const Dataset&, uint blocks, uint block_size,
    std::vector<uint>& results) (bs.cu:21-22), for the reference's Dataset type or any type
@@ -66,6 +74,67 @@ inline void BinarySearchBasedIntersection(const DatasetT &dataset, unsigned bloc
         std::fprintf(stderr, "BinarySearchBasedIntersection failed\n");
         std::exit(2);
     }
+    std::printf("Binary Search based Set Intertsection: kernel time = %f ms\n", ms);
+}
+
+namespace rtg_compat_detail {
+template <class DatasetT>
+inline double tile(const DatasetT &dataset, int method, std::vector<unsigned> &results, const char *name) {
+    results.resize((size_t)dataset.num_of_sets_a * dataset.num_of_sets_b);
+    double ms = 0.0;
+    if (rtg_set_intersection_tile_host(dataset.set_offsets.data(), dataset.elements.data(), dataset.num_of_sets_a,
+                                       dataset.num_of_sets_b, method, 0, results.data(), &ms) != 0) {
+        std::fprintf(stderr, "%s failed\n", name);
+        std::exit(2);
+    }
+    return ms;
+}
+}  // namespace rtg_compat_detail
+
+/* The other set-intersection baselines with their signatures and report lines; each runs the
+   corresponding method of rtg.h (the launch-shape and bucket arguments are accepted and the
+   shapes chosen by the library). */
+/* HashBasedIntersection, set_intersection/baseline/hi.cu:22-23 -> RTG_SI_HASH */
+template <class DatasetT>
+inline void HashBasedIntersection(const DatasetT &dataset, unsigned blocks, unsigned block_size, unsigned buckets,
+                                  unsigned max_bucket_size, std::vector<unsigned> &results) {
+    (void)blocks, (void)block_size, (void)buckets, (void)max_bucket_size;
+    const double ms = rtg_compat_detail::tile(dataset, RTG_SI_HASH, results, "HashBasedIntersection");
+    std::printf("Hash based Set Intertsection: kernel time = %f ms\n", ms);
+}
+/* IntersectPathBasedIntersection, ip.cu:32-33 -> RTG_SI_MERGE */
+template <class DatasetT>
+inline void IntersectPathBasedIntersection(const DatasetT &dataset, unsigned blocks, unsigned block_size,
+                                           std::vector<unsigned> &results) {
+    (void)blocks, (void)block_size;
+    const double ms = rtg_compat_detail::tile(dataset, RTG_SI_MERGE, results, "IntersectPathBasedIntersection");
+    std::printf("Intersect Path based Set Intertsection: kernel time = %f ms\n", ms);
+}
+/* BitmapNaiveBasedIntersection, bi_naive.cu:26-27 -> RTG_SI_BITMAP (its bitmaps are built
+   inside the counting launch: the construction time is reported as 0) */
+template <class DatasetT>
+inline void BitmapNaiveBasedIntersection(const DatasetT &dataset, unsigned blocks, unsigned block_size,
+                                         std::vector<unsigned> &results) {
+    (void)blocks, (void)block_size;
+    const double ms = rtg_compat_detail::tile(dataset, RTG_SI_BITMAP, results, "BitmapNaiveBasedIntersection");
+    std::printf("Bitmap(naive) based Set Intertsection:\nconstruction time = %f ms, counting time = %f ms , kernel "
+                "time = %f ms\n", 0.0, ms, ms);
+}
+/* BitmapDynamicBasedIntersection, bi_dynamic.cu:24-25 -> RTG_SI_BITMAP */
+template <class DatasetT>
+inline void BitmapDynamicBasedIntersection(const DatasetT &dataset, unsigned blocks, unsigned block_size,
+                                           std::vector<unsigned> &results) {
+    (void)blocks, (void)block_size;
+    const double ms = rtg_compat_detail::tile(dataset, RTG_SI_BITMAP, results, "BitmapDynamicBasedIntersection");
+    std::printf("Bitmap(dynamic) based Set Intertsection: kernel time = %f ms\n", ms);
+}
+/* ThreadBinarySearch, bs.cu:25-26 (a thread per element of the A sets) -> RTG_SI_BATCHED */
+template <class DatasetT>
+inline void ThreadBinarySearch(const DatasetT &dataset, unsigned blocks, unsigned block_size,
+                               std::vector<unsigned> &results) {
+    (void)blocks, (void)block_size;
+    std::printf("-----> thread binary search\n");
+    const double ms = rtg_compat_detail::tile(dataset, RTG_SI_BATCHED, results, "ThreadBinarySearch");
     std::printf("Binary Search based Set Intertsection: kernel time = %f ms\n", ms);
 }
 #endif

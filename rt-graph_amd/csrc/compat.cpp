@@ -84,11 +84,10 @@ RTG_API void BSAccessCountFunction(int *data, int data_size, int *query, int que
     std::printf("Binary search time = %f ms\n", ms);
 }
 
-RTG_API int rtg_binary_search_based_intersection(const uint32_t *set_offsets, const uint32_t *elements, uint32_t num_a,
-                                                 uint32_t num_b, uint32_t block_size, int device_id, uint32_t *results,
-                                                 double *ms) {
-    (void)block_size;
-    if (!set_offsets || !results) return rtg_fail(RTG_EINVAL, "rtg_binary_search_based_intersection: null argument");
+RTG_API int rtg_set_intersection_tile_host(const uint32_t *set_offsets, const uint32_t *elements, uint32_t num_a,
+                                           uint32_t num_b, int32_t method, int device_id, uint32_t *results,
+                                           double *ms) {
+    if (!set_offsets || !results) return rtg_fail(RTG_EINVAL, "rtg_set_intersection_tile_host: null argument");
     Ctx ctx;
     int rc = rtg_create(device_id, &ctx.c);
     if (rc != RTG_OK) return rc;
@@ -102,13 +101,20 @@ RTG_API int rtg_binary_search_based_intersection(const uint32_t *set_offsets, co
         (nel && (rc = rtg_memcpy_h2d(ctx.c, del.p, elements, sizeof(uint32_t) * nel))))
         return rc;
     rtg_si_stats st{};
-    if ((rc = rtg_si_tile(ctx.c, (const uint32_t *)doff.p, (const uint32_t *)del.p, num_a, num_b, RTG_SI_BATCHED,
+    if ((rc = rtg_si_tile(ctx.c, (const uint32_t *)doff.p, (const uint32_t *)del.p, num_a, num_b, method,
                           (uint32_t *)dout.p, &st)))
         return rc;
     if ((uint64_t)num_a * num_b && (rc = rtg_memcpy_d2h(ctx.c, results, dout.p, sizeof(uint32_t) * (uint64_t)num_a * num_b)))
         return rc;
     if (ms) *ms = st.t_kernel_ms;
     return RTG_OK;
+}
+
+RTG_API int rtg_binary_search_based_intersection(const uint32_t *set_offsets, const uint32_t *elements, uint32_t num_a,
+                                                 uint32_t num_b, uint32_t block_size, int device_id, uint32_t *results,
+                                                 double *ms) {
+    (void)block_size;
+    return rtg_set_intersection_tile_host(set_offsets, elements, num_a, num_b, RTG_SI_BATCHED, device_id, results, ms);
 }
 
 }  // extern "C"

@@ -46,3 +46,46 @@ def test_struct_sizes_match_header():
     # rtg_tc_stats layout: 2 doubles, 2 u64, i64, i32, 8 i32, 8 i64, 8 doubles, 2 i32
     assert ctypes.sizeof(rtg.TcStats) == 8 * 2 + 8 * 2 + 8 + 4 + 4 * 8 + 4 + 8 * 8 + 8 * 8 + 4 * 2
     assert ctypes.sizeof(rtg.TcOpts) == 4 + 4 + 8 + 8 + 8 + 4 + 4
+
+
+def test_compat_header_templates_compile_and_link(tmp_path):
+    """A reference-style driver that calls every drop-in of include/rtg_compat.h builds and links
+    against librtg (the set-intersection baselines are header templates over the reference's
+    Dataset field names); it is compiled, not run."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++") or shutil.which("clang++")
+    if cxx is None:
+        import pytest
+        pytest.skip("no host C++ compiler")
+    src = tmp_path / "driver.cpp"
+    src.write_text(r'''
+#include <vector>
+#include "rtg_compat.h"
+struct Dataset {  // set_intersection/utils/util.h:19-28 field names
+    unsigned num_of_sets_a, num_of_sets_b, num_of_sets, max_element, intersection;
+    std::vector<unsigned> set_offsets, elements;
+};
+int main(int argc, char **) {
+    Dataset d{1, 1, 2, 10, 1, {0, 2, 4}, {1, 5, 5, 7}};
+    std::vector<unsigned> r;
+    if (argc > 99) {  // never taken: instantiate and link every drop-in without running it
+        BinarySearchBasedIntersection(d, 1, 256, r);
+        ThreadBinarySearch(d, 1, 256, r);
+        HashBasedIntersection(d, 1, 256, 1024, 16, r);
+        IntersectPathBasedIntersection(d, 1, 256, r);
+        BitmapNaiveBasedIntersection(d, 1, 256, r);
+        BitmapDynamicBasedIntersection(d, 1, 256, r);
+        int adjs[1] = {0}, offs[2] = {0, 0}, data[1] = {1}, q[1] = {1}, out[1];
+        CountOnGPU(adjs, offs, 1, 0, 0);
+        BSAccessCountFunction(data, 1, q, 1, out);
+    }
+    return 0;
+}
+''')
+    lib = rtg.lib_path()
+    exe = tmp_path / "driver"
+    proc = subprocess.run([cxx, "-std=c++17", "-O1", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+                           f"-L{lib.parent}", "-lrtg", f"-Wl,-rpath,{lib.parent}"], capture_output=True, text=True)
+    assert proc.returncode == 0, proc.stderr
+    assert subprocess.run([str(exe)]).returncode == 0

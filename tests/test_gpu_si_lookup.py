@@ -299,3 +299,17 @@ def test_lookup_stateless_both_forms(ctx, n, q):
     # the same array seen from its second element: 4-byte aligned only
     out2, _ = rtg.lookup(ctx, d_data.ptr + 4, n, d_keys, keys.size)
     assert np.array_equal(out2.numpy(), orc.lookup(data[1:], keys))
+
+
+@pytest.mark.parametrize("method", SI_METHODS)
+def test_set_intersection_tile_host_entry(method):
+    """rtg_set_intersection_tile_host, the C form behind the compat header's baseline drop-ins
+    (host arrays in, host counts out), for every method."""
+    import ctypes as C
+    rng = np.random.default_rng(11)
+    offs, el = reference_style_dataset(5, 4, 3000, 2500, 700, rng)
+    out = np.zeros(20, np.uint32)
+    ms = C.c_double()
+    rc = rtg._lib.rtg_set_intersection_tile_host(rtg._ptr(offs), rtg._ptr(el), C.c_uint32(5), C.c_uint32(4),
+                                                 C.c_int32(method), C.c_int(0), rtg._ptr(out), C.byref(ms))
+    assert rc == 0 and np.all(out == 700) and ms.value > 0
