@@ -195,43 +195,55 @@ def si_leg(rtg, ctx) -> dict:
             "tile": si_tile_leg(rtg, ctx)}
 
 
-def si_tile_leg(rtg, ctx) -> dict:
-    """The reference's set-intersection dataset shape (its generator with length 100,000,
-    skew ratio 1, selectivity 0.1, density 0.1; 1000 A sets against one B set; seed 1) in tile
-    mode: median of 5 kernel times for every method, and the strategy SI_AUTO keeps."""
-    ds = rtg.gen_si_dataset(1000, 1, 100000, 1.0, 0.1, 0.1, seed=1)
-    d_off, d_el = ctx.to_device(ds.set_offsets), ctx.to_device(ds.elements)
-    out = rtg.DeviceArray(ctx, (1000,), np.uint32)
-    names = {rtg.SI_PLAIN: "plain", rtg.SI_REFERENCE: "reference", rtg.SI_BATCHED: "batched_search",
-             rtg.SI_MERGE: "merge", rtg.SI_HASH: "hash", rtg.SI_BITMAP: "bitmap", rtg.SI_AUTO: "auto"}
-    ms, alg, auto = {}, 0, None
-    for m, name in names.items():
-        ts = []
-        for _ in range(5):
-            _, st = rtg.si_tile(ctx, d_off, d_el, 1000, 1, m, d_out=out)
-            ts.append(st["t_kernel_ms"])
-            if m == rtg.SI_AUTO:
-                auto = st["strategy"]
-        if not np.all(out.numpy() == ds.intersection):
-            raise RuntimeError(f"set-intersection tile: method {name} gave a wrong count")
-        ms[name] = sorted(ts)[2]
-        alg = st["alg_bytes"]
-    for a in (out, d_el, d_off):
-        a.free()
-    orc = oracle_module()
-    t0 = time.perf_counter()
-    exp = orc.si_tile(ds.set_offsets, ds.elements, 1000, 1, ORACLE_THREADS[0])
-    t_cpu = (time.perf_counter() - t0) * 1e3
-    if not np.all(exp == ds.intersection):
-        raise RuntimeError("set-intersection tile: the oracle disagrees with the generator's invariant")
-    best = min(v for k, v in ms.items() if k != "auto")
-    log("set intersection tile (1000 x 1 sets of 100,000): " + ", ".join(f"{k} {v:.3f} ms" for k, v in ms.items()) +
-        f"; oracle {t_cpu:.1f} ms on {ORACLE_THREADS[0]} threads")
-    return {"workload": "1000 A sets x 1 B set, length 100000, selectivity 0.1, density 0.1", "ms": ms,
-            "auto_strategy": auto, "alg_bytes": alg, "alg_gbs_best": alg / best / 1e6,
-            "elements": int(ds.elements.size),
-            "cpu_baseline": {"ms": t_cpu, "threads": ORACLE_THREADS[0], "kind": "port",
-                             "sample": "the whole tile, one call of the oracle's tile intersection"}}
+SI_TILE_SHAPES = (
+    # (label, length, skew ratio, selectivity, density): the generator's arguments of the
+    # reference's scripts — its default dataset (s_generator -l 100000 -r 1 -s 0.1 -d 0.1) and
+    # the density sweep's file 10000_50000_5000_5500000 (len_a 10,000, len_b 50,000, len_c
+    # 5,000, 5.5M values)
+    ("length 100000, selectivity 0.1, density 0.1", 100000, 1.0, 0.1, 0.1),
+    ("len_a 10000, len_b 50000, len_c 5000, density 0.01", 10000, 0.2, 0.5, 0.01),
+)
+
+
+def si_tile_leg(rtg, ctx) -> list:
+    """The reference's set-intersection dataset shapes (SI_TILE_SHAPES; 1000 A sets against one
+    B set; seed 1) in tile mode: median of 5 kernel times for every method, the strategy SI_AUTO
+    keeps, and the oracle's time on the host."""
+    res = []
+    for label, length, skew, sel, dens in SI_TILE_SHAPES:
+        ds = rtg.gen_si_dataset(1000, 1, length, skew, sel, dens, seed=1)
+        d_off, d_el = ctx.to_device(ds.set_offsets), ctx.to_device(ds.elements)
+        out = rtg.DeviceArray(ctx, (1000,), np.uint32)
+        names = {rtg.SI_PLAIN: "plain", rtg.SI_REFERENCE: "reference", rtg.SI_BATCHED: "batched_search",
+                 rtg.SI_MERGE: "merge", rtg.SI_HASH: "hash", rtg.SI_BITMAP: "bitmap", rtg.SI_AUTO: "auto"}
+        ms, alg, auto = {}, 0, None
+        for m, name in names.items():
+            ts = []
+            for _ in range(5):
+                _, st = rtg.si_tile(ctx, d_off, d_el, 1000, 1, m, d_out=out)
+                ts.append(st["t_kernel_ms"])
+                if m == rtg.SI_AUTO:
+                    auto = st["strategy"]
+            if not np.all(out.numpy() == ds.intersection):
+                raise RuntimeError(f"set-intersection tile ({label}): method {name} gave a wrong count")
+            ms[name] = sorted(ts)[2]
+            alg = st["alg_bytes"]
+        for a in (out, d_el, d_off):
+            a.free()
+        orc = oracle_module()
+        t0 = time.perf_counter()
+        exp = orc.si_tile(ds.set_offsets, ds.elements, 1000, 1, ORACLE_THREADS[0])
+        t_cpu = (time.perf_counter() - t0) * 1e3
+        if not np.all(exp == ds.intersection):
+            raise RuntimeError("set-intersection tile: the oracle disagrees with the generator's invariant")
+        best = min(v for k, v in ms.items() if k != "auto")
+        log(f"set intersection tile (1000 x 1 sets, {label}): " + ", ".join(f"{k} {v:.3f} ms" for k, v in ms.items()) +
+            f"; oracle {t_cpu:.1f} ms on {ORACLE_THREADS[0]} threads")
+        res.append({"workload": f"1000 A sets x 1 B set, {label}", "ms": ms, "auto_strategy": auto, "alg_bytes": alg,
+                    "alg_gbs_best": alg / best / 1e6, "elements": int(ds.elements.size),
+                    "cpu_baseline": {"ms": t_cpu, "threads": ORACLE_THREADS[0], "kind": "port",
+                                     "sample": "the whole tile, one call of the oracle's tile intersection"}})
+    return res
 
 
 def per_edge_leg(rtg, ctx, g, count: int, alg_bytes: int, tune_cache) -> dict:

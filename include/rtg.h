@@ -204,17 +204,17 @@ int rtg_tc_per_edge(rtg_ctx *ctx, const rtg_graph *g, int32_t form, uint32_t *d_
 #define RTG_SI_HASH 4      /* LDS hash table per 1024-element chunk (baseline/hi.cu:134-195)  */
 #define RTG_SI_BITMAP 5    /* LDS bitmap per 65,536-value window for dense lists, merge-path
                               segments for the sparser ones (baseline/bi_dynamic.cu:134-169)  */
-#define RTG_SI_AUTO 6      /* times BATCHED, MERGE, HASH and BITMAP once per argument set
-                              (offsets, elements, pairs pointers and sizes) and keeps the
-                              fastest; t_plan_ms holds the tuning time when it ran            */
+#define RTG_SI_AUTO 6      /* times BATCHED, MERGE, HASH, BITMAP and REFERENCE once per
+                              argument set (offsets, elements, pairs pointers and sizes) and
+                              keeps the fastest; t_plan_ms holds the tuning time when it ran */
 
 typedef struct rtg_si_stats {
     double t_kernel_ms;
     double t_plan_ms;
     uint64_t alg_bytes;  /* sum over pairs of 20 + list_bytes(|A|,|B|)                         */
     int32_t nbuckets;
-    int32_t strategy;    /* long-pair strategy run: 0 search, 1 merge, 2 hash, 3 bitmap;
-                            -1 for PLAIN and REFERENCE                                         */
+    int32_t strategy;    /* long-pair strategy run: 0 search, 1 merge, 2 hash, 3 bitmap,
+                            4 the REFERENCE form chosen by AUTO; -1 for PLAIN and REFERENCE   */
     int64_t bucket_items[RTG_TC_MAX_BUCKETS]; /* batched: units of each strategy (index as
                             strategy): 64-element search items, merge segments of 1024 steps,
                             hash chunks, bitmap windows                                        */

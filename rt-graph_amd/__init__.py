@@ -42,7 +42,7 @@ _LIB_PATH = _HERE / "librtg.so"
 TC_PLAIN_THREAD, TC_PLAIN_WAVE, TC_BATCHED, TC_2A1 = 0, 1, 2, 3
 SI_PLAIN, SI_BATCHED, SI_REFERENCE, SI_MERGE, SI_HASH, SI_BITMAP, SI_AUTO = 0, 1, 2, 3, 4, 5, 6
 SI_METHODS = (SI_PLAIN, SI_BATCHED, SI_REFERENCE, SI_MERGE, SI_HASH, SI_BITMAP, SI_AUTO)
-SI_STRATEGIES = {-1: "none", 0: "search", 1: "merge", 2: "hash", 3: "bitmap"}
+SI_STRATEGIES = {-1: "none", 0: "search", 1: "merge", 2: "hash", 3: "bitmap", 4: "sampled"}
 PAIRS_RELABEL, PAIRS_ORIENT, PAIRS_DEVICE = 0x1, 0x2, 0x4
 TC_MAX_BUCKETS = 8
 

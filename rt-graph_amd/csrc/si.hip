@@ -24,7 +24,8 @@
 //                 the same launch, with the long pairs where merging is cheaper than searching
 //                 run as merge-path segments, LDS hash chunks or LDS bitmap windows (see
 //                 kLongMerge and its siblings below).
-//   SI_AUTO       times the four batched strategies on the input once and keeps the fastest.
+//   SI_AUTO       times the four batched strategies and SI_REFERENCE on the input once and
+//                 keeps the fastest.
 #include "device.hpp"
 #include "si.hpp"
 
@@ -770,13 +771,25 @@ int launch_batched(rtg_ctx *ctx, int kind, Lists L, PairSrc P, int64_t npairs, c
 
 // Times every long-pair strategy once on this input (each run writes the full, exact result)
 // and returns the fastest; the launch after it is the one the caller's timing sees.
+// The block-per-pair form with LDS samples (RTG_SI_REFERENCE), also an autotuner candidate:
+// with few, long pairs whose longer list is shared (a tile with one B set) it can win.
+constexpr int kSampled = kLongKinds;
+
+int launch_sampled(rtg_ctx *ctx, Lists L, PairSrc P, int64_t npairs, uint32_t *d_out) {
+    si_reference<<<(unsigned)std::min<int64_t>(npairs, (int64_t)ctx->cu_count * 8), kThreads, 0, ctx->stream>>>(L, P, npairs,
+                                                                                                              d_out);
+    HIP_TRY(hipGetLastError());
+    return RTG_OK;
+}
+
 int autotune(rtg_ctx *ctx, Lists L, PairSrc P, int64_t npairs, const BatchBufs &B, uint32_t *d_out, int *best) {
     hipStream_t st = ctx->stream;
     float best_ms = 0.f;
     *best = kLongSearch;
-    for (int kind = 0; kind < kLongKinds; ++kind) {
+    for (int kind = 0; kind <= kSampled; ++kind) {
         HIP_TRY(hipEventRecord(ctx->ev[2], st));
-        RTG_TRY(launch_batched(ctx, kind, L, P, npairs, B, d_out));
+        if (kind == kSampled) RTG_TRY(launch_sampled(ctx, L, P, npairs, d_out));
+        else RTG_TRY(launch_batched(ctx, kind, L, P, npairs, B, d_out));
         HIP_TRY(hipEventRecord(ctx->ev[3], st));
         HIP_TRY(hipEventSynchronize(ctx->ev[3]));
         const float ms = rtg::elapsed_ms(ctx->ev[2], ctx->ev[3]);
@@ -832,8 +845,8 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
     HIP_TRY(hipEventRecord(ctx->ev[0], st));
     if (method == RTG_SI_PLAIN) {
         si_plain<<<grid_for(ctx, npairs * 64, 32), kThreads, 0, st>>>(L, P, npairs, d_out);
-    } else if (method == RTG_SI_REFERENCE) {
-        si_reference<<<(unsigned)std::min<int64_t>(npairs, (int64_t)ctx->cu_count * 8), kThreads, 0, st>>>(L, P, npairs, d_out);
+    } else if (method == RTG_SI_REFERENCE || kind == kSampled) {
+        RTG_TRY(launch_sampled(ctx, L, P, npairs, d_out));
     } else {
         RTG_TRY(launch_batched(ctx, kind, L, P, npairs, B, d_out));
     }
@@ -861,7 +874,7 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
         stats->t_plan_ms = t_tune;
         stats->alg_bytes = v[0];
         stats->strategy = kind;
-        if (kind >= 0) {  // units of each long-pair kind: search items, merge segments, hash chunks, bitmap windows
+        if (kind >= 0 && kind < kLongKinds) {  // units of each long-pair kind: search items, merge segments, hash chunks, bitmap windows
             stats->nbuckets = kLongKinds;
             for (int q = 0; q < kLongKinds; ++q) stats->bucket_items[q] = (int64_t)v[1 + q];
         }

@@ -121,7 +121,7 @@ def test_si_tile_balanced_against_oracle(ctx, method):
     assert np.array_equal(out.numpy(), exp)
 
 
-@pytest.mark.parametrize("length,skew,sel,density", [(100000, 1.0, 0.1, 0.1), (60000, 4.0, 0.2, 0.9),
+@pytest.mark.parametrize("length,skew,sel,density", [(100000, 1.0, 0.1, 0.1), (10000, 0.2, 0.5, 0.01), (60000, 4.0, 0.2, 0.9),
                                                      (30000, 0.5, 0.0, 0.02)])
 def test_si_tile_reference_generator(ctx, length, skew, sel, density):
     """Datasets shaped by the reference's generator (data.cpp:21-74; the first row is the shape
