@@ -34,6 +34,9 @@
 namespace {
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+// Fourth word of a raw buffer resource on gfx9-family GPUs (gfx950 included): a 32-bit data
+// format, which enables the range check that makes loads at offsets >= num_records return 0.
+constexpr int kBufferWord3 = 0x00020000;
 constexpr int kPlainThreads = 256;
 
 __global__ void edge_rows_kernel(const int32_t *__restrict__ off, int32_t n, int32_t *__restrict__ row) {
@@ -304,7 +307,7 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
 // Streams the out-lists of partner spans wspan[p0 .. p1) ({start, length} in adj) through the
 // 64 lanes of one wave and probes every element against the owner table. Partners come 64 at
 // a time. Lists of at least `long_min` elements are streamed one after another with lane j
-// reading element j (coalesced, no index arithmetic beyond base + j); shorter ones are
+// reading element j (coalesced buffer loads bounded by the list); shorter ones are
 // concatenated: their lengths are scanned into the wave's LDS slots and each lane walks a
 // cursor, so that lane j of a round handles element j of the concatenation.
 // kPerEdge: every span's count is also stored at eout[wedge[pi]] — a long span's by a wave
@@ -317,6 +320,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
     const int lane = __lane_id();
     const uint64_t lt = rtgd::lanemask_lt();
     uint32_t hits = 0;
+    const uint32_t has0 = probe<kMode>(c, 0u);  // what a load past a long span's end counts
     for (int64_t bs = p0; bs < p1; bs += 64) {
         const int64_t pi = bs + lane;
         int2 sp = make_int2(0, 0);
@@ -329,20 +333,31 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             const int32_t s0 = __builtin_amdgcn_readlane(sp.x, p);
             const int32_t l0 = __builtin_amdgcn_readlane(sp.y, p);
             uint32_t sh = 0;  // this span's hits (per-edge counts only)
-            for (int32_t jb = 0; jb < l0; jb += 64 * U) {
+            // Raw buffer loads whose range is the span: a lane past its end reads 0 without a
+            // memory access, so the rounds need no per-element bounds test or branch, and the
+            // 2U probes of a round issue back to back.
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<int32_t *>(adj + s0), (short)0, l0 * 4, kBufferWord3);
+            int32_t jb = 0;
+            for (; jb < l0; jb += 64 * U) {
+                const int vo = (jb + lane) * 4;
                 uint32_t x[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int32_t j = jb + u * 64 + lane;
-                    x[u] = j < l0 ? (uint32_t)adj[s0 + j] : kEmpty;
-                }
+                for (int u = 0; u < U; ++u) x[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, vo + u * 256, 0, 0);
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    if (x[u] != kEmpty) {
-                        if constexpr (kPerEdge) sh += probe<kMode>(c, x[u]);
-                        else hits += probe<kMode>(c, x[u]);
-                    }
+                    if constexpr (kPerEdge) sh += probe<kMode>(c, x[u]);
+                    else hits += probe<kMode>(c, x[u]);
                 }
+            }
+            {
+                // This lane's loads past the end in the last round each probed 0 and hit iff
+                // the owner's list holds 0 (has0): take them back.
+                const int32_t r = l0 - (jb - 64 * U) - lane;
+                const int32_t in = r <= 0 ? 0 : min(U, (r + 63) >> 6);
+                const uint32_t pad = has0 * (uint32_t)(U - in);
+                if constexpr (kPerEdge) sh -= pad;
+                else hits -= pad;
             }
             if constexpr (kPerEdge) {
                 hits += sh;
@@ -638,7 +653,7 @@ int dev_alloc(T **p, size_t count) {
 // set by the RTG_TC_TUNE environment variable.
 struct BatchTune {
     int32_t block_threads = 256;
-    int32_t unroll = 8;
+    int32_t unroll = 4;
     int64_t item_work = 1 << 21;   // streamed elements per block item
     int64_t wave_item_work = 1 << 12;
     int32_t wave_owner_max = -1;   // -1: as large as the wave's table share allows
@@ -958,16 +973,22 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
 // ---- autotuning ----------------------------------------------------------------------------
 // Candidate configurations, timed on the plan's own graph. The first is the default.
 const char *kCandidates[] = {
-    "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
-    "threads=256,unroll=8,item=524288,wave_item=4096,wave_max=-1,long=128,grab=64",
     "threads=256,unroll=4,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
-    "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=64,grab=64",
-    "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=32",
+    "threads=256,unroll=4,item=2097152,wave_item=4096,wave_max=64,long=128,grab=64",
+    "threads=256,unroll=4,item=524288,wave_item=4096,wave_max=-1,long=128,grab=64",
+    "threads=256,unroll=4,item=2097152,wave_item=4096,wave_max=-1,long=64,grab=64",
+    "threads=256,unroll=4,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=32",
+    "threads=256,unroll=4,item=2097152,wave_item=16384,wave_max=0,long=128,grab=64",
+    "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
     "threads=256,unroll=8,item=2097152,wave_item=4096,wave_max=64,long=128,grab=64",
-    "threads=256,unroll=8,item=2097152,wave_item=16384,wave_max=0,long=128,grab=64",
     "threads=256,unroll=2,item=131072,wave_item=4096,wave_max=-1,long=32,grab=16",
+    "threads=512,unroll=4,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
     "threads=512,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
 };
+
+// Revision of the batched kernel's code, part of the cache key: raised when a change to the
+// kernel can change which candidate is fastest, so that entries tuned earlier are re-tuned.
+constexpr int kTuneRevision = 2;
 
 uint64_t fnv1a(const void *data, size_t bytes, uint64_t h = 1469598103934665603ull) {
     const unsigned char *c = (const unsigned char *)data;
@@ -975,15 +996,15 @@ uint64_t fnv1a(const void *data, size_t bytes, uint64_t h = 1469598103934665603u
     return h;
 }
 
-// Cache key: device architecture and CU count, graph shape, a hash of the row offsets and the
-// edge range.
+// Cache key: device architecture and CU count, kernel revision, graph shape, a hash of the row
+// offsets and the edge range.
 std::string tune_key(rtg_ctx *ctx, const rtg_tc_plan *p) {
     uint64_t h = fnv1a(p->hoff.data(), sizeof(int32_t) * p->hoff.size());
     int64_t r[2] = {p->e_begin, p->e_end};
     h = fnv1a(r, sizeof(r), h);
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "%s/cu%d/n%d/m%lld/%016llx", ctx->prop.gcnArchName, ctx->cu_count, p->g->n,
-                  (long long)p->g->m, (unsigned long long)h);
+    std::snprintf(buf, sizeof(buf), "%s/cu%d/r%d/n%d/m%lld/%016llx", ctx->prop.gcnArchName, ctx->cu_count,
+                  kTuneRevision, p->g->n, (long long)p->g->m, (unsigned long long)h);
     return buf;
 }
 

@@ -210,6 +210,47 @@ def test_every_candidate_configuration(ctx, monkeypatch):
         p.close()
 
 
+def _zero_heavy_csr(seed=11, n=3000):
+    """A sorted CSR (not an orientation: any CSR is valid input to the count) whose out-lists
+    mostly hold vertex 0 and have ragged lengths 0..700, so that long partner spans end inside
+    a round and the owners' tables hold 0: what a load past a span's end returns."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 701, size=n)
+    short = rng.random(n) < 0.3
+    lens[short] = rng.integers(0, 70, size=int(short.sum()))
+    rows = []
+    for u in range(n):
+        row = rng.choice(np.arange(1, n), size=min(int(lens[u]), n - 1), replace=False)
+        if u and rng.random() < 0.8:
+            row = np.concatenate([[0], row[:-1]]) if row.size else np.array([0])
+        rows.append(np.unique(row[row != u]).astype(np.int32))  # no self-loops
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum([r.size for r in rows])
+    return off, np.concatenate(rows).astype(np.int32)
+
+
+@pytest.mark.parametrize("unroll", [2, 4, 8])
+def test_long_spans_with_vertex_zero(ctx, monkeypatch, unroll):
+    """Long partner spans are read by loads bounded by the span, which return 0 past its end;
+    the probes of those lanes are taken back when the owner's list holds 0. Every span long
+    (long=1) or only some (long=128), counts and per-edge counts against the oracle."""
+    off, adj = _zero_heavy_csr()
+    assert (adj == 0).sum() > 1000
+    t = orc.tc_count_intersect(off, adj)
+    exp = orc.tc_per_edge(off, adj, 0)
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    for long_min in (1, 128):
+        monkeypatch.setenv("RTG_TC_TUNE", f"threads=256,unroll={unroll},item=65536,wave_item=4096,long={long_min},grab=16")
+        p = g.tc_plan(method=rtg.TC_BATCHED)
+        assert p.run()[0] == t, long_min
+        p.close()
+        p = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=False, per_edge=True)
+        out, cnt, _ = p.run_per_edge()
+        assert cnt == t and np.array_equal(out.numpy()[: adj.size], exp), long_min
+        out.free()
+        p.close()
+
+
 @pytest.mark.parametrize("cap", [2048, 64])
 def test_owners_beyond_the_lds_table(ctx, monkeypatch, cap):
     """Owners whose cuckoo table would exceed the LDS cap are binary-searched in HBM instead
