@@ -220,8 +220,10 @@ typedef struct rtg_si_stats {
                             hash chunks, bitmap windows                                        */
 } rtg_si_stats;
 
-/* Lists are sorted, duplicate-free uint32 runs elements[offsets[i] .. offsets[i+1]).
-   For pair k: d_out[k] = |list(pairs[2k]) & list(pairs[2k+1])|.                             */
+/* Lists are sorted, duplicate-free uint32 runs elements[offsets[i] .. offsets[i+1]),
+   i < nlists (at most 2^32 - 1). For pair k: d_out[k] = |list(pairs[2k]) & list(pairs[2k+1])|.
+   A pair naming an id >= nlists is counted as 0 and the call returns RTG_EINVAL after it ran
+   (the device never reads past offsets[nlists]).                                            */
 int rtg_si_pairs(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements,
                  int64_t nlists, const uint32_t *d_pairs, int64_t npairs, int32_t method,
                  uint32_t *d_out, rtg_si_stats *stats);

@@ -72,10 +72,23 @@ __device__ __forceinline__ uint32_t contains(const uint32_t *__restrict__ data, 
 struct Lists {
     const uint32_t *off;
     const uint32_t *el;
-    __device__ __forceinline__ void span(uint32_t id, const uint32_t *&p, uint32_t &len) const {
-        const uint32_t b = off[id];
-        p = el + b;
+    uint32_t n;     // lists in off; an id at or above n reads as an empty list
+    uint32_t *bad;  // set to 1 when a pair names such an id (null: not reported)
+    // Start (in el) and length of list id.
+    __device__ __forceinline__ void range(uint32_t id, uint32_t &b, uint32_t &len) const {
+        if (id >= n) {
+            if (bad) *bad = 1u;
+            b = 0;
+            len = 0;
+            return;
+        }
+        b = off[id];
         len = off[id + 1] - b;
+    }
+    __device__ __forceinline__ void span(uint32_t id, const uint32_t *&p, uint32_t &len) const {
+        uint32_t b;
+        range(id, b, len);
+        p = el + b;
     }
 };
 
@@ -615,7 +628,9 @@ __global__ __launch_bounds__(kThreads) void si_drain(Lists L, PairSrc P, const u
         if (in) {
             uint32_t a, b;
             P.get(kk, a, b);
-            const uint32_t ba = L.off[a], la = L.off[a + 1] - ba, bb = L.off[b], lb = L.off[b + 1] - bb;
+            uint32_t ba, la, bb, lb;
+            L.range(a, ba, la);
+            L.range(b, bb, lb);
             if (la <= lb) { s0 = ba; ns = la; l0 = bb; nl = lb; }
             else { s0 = bb; ns = lb; l0 = ba; nl = la; }
             mine = ns <= kFlatMax;
@@ -801,13 +816,21 @@ int autotune(rtg_ctx *ctx, Lists L, PairSrc P, int64_t npairs, const BatchBufs &
     return RTG_OK;
 }
 
-int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P, int64_t npairs, int32_t method,
-           uint32_t *d_out, rtg_si_stats *stats) {
+// nlists: lists in d_off; pairs naming an id at or above it count 0 and make the call fail
+// with RTG_EINVAL after it ran (the kernels never read past the offsets).
+int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, uint32_t nlists, PairSrc P, int64_t npairs,
+           int32_t method, uint32_t *d_out, rtg_si_stats *stats) {
     using namespace rtg;
     hipStream_t st = ctx->stream;
-    Lists L{d_off, d_el};
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (npairs <= 0) return RTG_OK;
+    // Context scratch: bytes [0, 64) hold the statistics pass's sums, the word at byte 192 the
+    // out-of-range flag.
+    void *scr = nullptr;
+    RTG_TRY(ctx_scratch(ctx, 256, &scr));
+    uint32_t *d_bad = reinterpret_cast<uint32_t *>(static_cast<char *>(scr) + 192);
+    HIP_TRY(hipMemsetAsync(d_bad, 0, sizeof(uint32_t), st));
+    Lists L{d_off, d_el, nlists, d_bad};
     if (npairs >= ((int64_t)1 << 32)) return set_error(RTG_EINVAL, "rtg_si: more than 2^32 pairs");
     const bool autotuned = method == RTG_SI_AUTO;
     int kind = autotuned ? kLongSearch : long_of(method);
@@ -852,12 +875,12 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    uint32_t bad = 0;
     if (stats) {
-        void *s = nullptr;
         constexpr size_t kAcc = sizeof(unsigned long long) * (1 + kLongKinds);
-        RTG_TRY(ctx_scratch(ctx, 256, &s));
-        HIP_TRY(hipMemsetAsync(s, 0, kAcc, st));
-        auto *acc = (unsigned long long *)s;
+        static_assert(kAcc <= 192, "statistics sums overlap the out-of-range flag");
+        HIP_TRY(hipMemsetAsync(scr, 0, kAcc, st));
+        auto *acc = (unsigned long long *)scr;
         const unsigned g = grid_for(ctx, npairs, 2);
         switch (kind) {
             case kLongSearch: si_stats_pass<kLongSearch><<<g, kThreads, 0, st>>>(L, P, npairs, acc); break;
@@ -868,7 +891,8 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
         }
         HIP_TRY(hipGetLastError());
         unsigned long long v[1 + kLongKinds] = {};
-        HIP_TRY(hipMemcpyAsync(v, s, kAcc, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(v, scr, kAcc, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         stats->t_kernel_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
         stats->t_plan_ms = t_tune;
@@ -879,8 +903,10 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, PairSrc P,
             for (int q = 0; q < kLongKinds; ++q) stats->bucket_items[q] = (int64_t)v[1 + q];
         }
     } else {
+        HIP_TRY(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
     }
+    if (bad) return set_error(RTG_EINVAL, "rtg_si: a pair names a list id at or above the number of lists (counted as 0)");
     return RTG_OK;
 }
 
@@ -891,7 +917,7 @@ namespace rtg {
 int si_launch_items(rtg_ctx *ctx, hipStream_t st, const uint32_t *d_off, const uint32_t *d_el, const uint32_t *d_pairs,
                     const uint32_t *d_items, int64_t nitems, uint32_t *d_out) {
     if (nitems <= 0) return RTG_OK;
-    Lists L{d_off, d_el};
+    Lists L{d_off, d_el, 0xFFFFFFFFu, nullptr};  // vertex ids were checked when the requests came in
     const int64_t blocks = ceil_div(nitems, (int64_t)(kThreads / 64));
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)ctx->cu_count * 16));
     si_items<<<grid, kThreads, 0, st>>>(L, reinterpret_cast<const uint2 *>(d_pairs), reinterpret_cast<const uint2 *>(d_items),
@@ -911,9 +937,10 @@ RTG_API int rtg_si_pairs(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t
     if (nlists < 0 || npairs < 0 || !d_offsets || (npairs > 0 && (!d_pairs || !d_out)))
         return rtg_fail(RTG_EINVAL, "rtg_si_pairs: bad arguments");
     if (((uintptr_t)d_pairs & 7) != 0) return rtg_fail(RTG_EINVAL, "rtg_si_pairs: d_pairs must be 8-byte aligned");
+    if (nlists > (int64_t)0xFFFFFFFFu) return rtg_fail(RTG_EINVAL, "rtg_si_pairs: list ids are 32-bit (at most 2^32 - 1 lists)");
     std::lock_guard<std::mutex> lock(ctx->mu);
     PairSrc P{d_pairs, 0, 1};
-    return si_run(ctx, d_offsets, d_elements, P, npairs, method, d_out, stats);
+    return si_run(ctx, d_offsets, d_elements, (uint32_t)nlists, P, npairs, method, d_out, stats);
 }
 
 RTG_API int rtg_si_tile(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements, uint32_t num_a,
@@ -921,9 +948,10 @@ RTG_API int rtg_si_tile(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t 
     RTG_TRY(rtg::ctx_enter(ctx));
     const int64_t np = (int64_t)num_a * num_b;
     if (!d_offsets || (np > 0 && !d_counts)) return rtg_fail(RTG_EINVAL, "rtg_si_tile: bad arguments");
+    if ((uint64_t)num_a + num_b > 0xFFFFFFFFull) return rtg_fail(RTG_EINVAL, "rtg_si_tile: num_a + num_b must fit 32 bits");
     std::lock_guard<std::mutex> lock(ctx->mu);
     PairSrc P{nullptr, num_a, num_b ? num_b : 1};
-    return si_run(ctx, d_offsets, d_elements, P, np, method, d_counts, stats);
+    return si_run(ctx, d_offsets, d_elements, num_a + num_b, P, np, method, d_counts, stats);
 }
 
 }  // extern "C"

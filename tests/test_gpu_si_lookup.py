@@ -62,6 +62,30 @@ def test_si_pairs_edge_cases(ctx, method):
 
 
 @pytest.mark.parametrize("method", SI_METHODS)
+def test_si_pairs_ids_out_of_range(ctx, method):
+    """A pair naming a list id >= nlists reads as an empty list on the device (the offsets are
+    never read past nlists) and the call reports RTG_EINVAL; the other pairs are still counted."""
+    offs, el = balanced_lists(np.random.default_rng(12))
+    n = offs.size - 1
+    pairs = np.array([(a, b) for a in range(n) for b in range(n)], np.uint32)
+    exp = orc.si_pairs(offs, el, pairs.reshape(-1))
+    bad_rows = [1, 7]
+    pairs[bad_rows[0]] = (0, n)            # the first id past the end
+    pairs[bad_rows[1]] = (0xFFFFFFFF, 2)   # the largest id
+    exp[bad_rows] = 0
+    d_off, d_el, d_p = ctx.to_device(offs), ctx.to_device(el), ctx.to_device(pairs.reshape(-1))
+    d_out = rtg.DeviceArray(ctx, (pairs.shape[0],), np.uint32)
+    with pytest.raises(rtg.RTGError) as e:
+        rtg.si_pairs(ctx, d_off, d_el, n, d_p, pairs.shape[0], method, d_out=d_out)
+    assert e.value.code == 1  # RTG_EINVAL
+    assert np.array_equal(d_out.numpy(), exp)
+    # the same lists and pairs with every id in range are accepted again
+    good = pairs.copy()
+    good[bad_rows] = (0, 1)
+    rtg.si_pairs(ctx, d_off, d_el, n, ctx.to_device(good.reshape(-1)), good.shape[0], method, d_out=d_out)
+
+
+@pytest.mark.parametrize("method", SI_METHODS)
 @pytest.mark.parametrize("len_c", [0, 5, 300])
 def test_si_tile_reference_invariant(ctx, method, len_c):
     rng = np.random.default_rng(len_c + 1)
