@@ -431,8 +431,11 @@ struct BatchArgs {
     uint32_t *eout;             // per-edge counts: output by oriented edge (else null)
 };
 
+// The per-edge instantiation is held to 8 waves per SIMD (at most 64 VGPRs, a few spilled to
+// scratch outside the streaming loops): left alone it takes 74 and runs 6 waves, which costs
+// more than the spills (R-MAT scale 21: 23.7 against 25.3 ms).
 template <int kThreads, int U, bool kPerEdge>
-__global__ __launch_bounds__(kThreads) void tc_batched(BatchArgs a) {
+__global__ __launch_bounds__(kThreads, kPerEdge ? 8 : 1) void tc_batched(BatchArgs a) {
     constexpr int kWaves = kThreads / 64;
     extern __shared__ uint32_t table[];
     __shared__ int32_t ex_all[kWaves][65];
