@@ -1098,6 +1098,11 @@ int autotune(rtg_ctx *ctx, rtg_tc_plan *p, BatchTune *best) {
 int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p) {
     using Clk = std::chrono::steady_clock;
     const auto ms_since = [](Clk::time_point t) { return std::chrono::duration<double, std::milli>(Clk::now() - t).count(); };
+    // Long partner lists are read through buffer resources sized in bytes by a 32-bit int, and
+    // lane offsets are int byte offsets: rows are bounded to 2^29 - 1 elements (a degree-ordered
+    // CSR's out-degrees stay below sqrt(2m), far under it).
+    if (p->g->max_row > (1 << 29) - 1)
+        return rtg::set_error(RTG_EUNSUPPORTED, "batched TC: a row longer than 2^29 - 1 elements");
     auto t0 = Clk::now();
     RTG_TRY(plan_graph(ctx, p));
     p->t_plan_graph_ms = ms_since(t0);
