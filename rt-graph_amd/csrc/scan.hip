@@ -1,8 +1,9 @@
 // Device-wide exclusive prefix scan (reduce-then-scan) for uint32 and uint64 values.
 //
 // Level 0 splits the input into tiles of 256 threads x 16 items; a reduce kernel writes one
-// sum per tile, the tile sums are scanned recursively with the same kernels, and a scan
-// kernel rescans each tile locally and adds its tile offset. Traffic: 2 reads + 1 write of
+// sum per tile, the tile sums are scanned recursively with the same kernels (or, for at most
+// 4096 tiles, added up by each block of the final kernel itself), and a scan kernel rescans
+// each tile locally and adds its tile offset. Traffic: 2 reads + 1 write of
 // the input size, plus the (1/4096-sized) tile-sum levels.
 #include "device.hpp"
 
@@ -51,12 +52,29 @@ __global__ __launch_bounds__(kThreads) void tile_reduce(const T *__restrict__ in
     }
 }
 
-// partial == nullptr: single tile, offset 0.
-template <class T>
+// partial == nullptr: single tile, offset 0. kPrefixed: partial holds the unscanned tile sums
+// (at most kTile of them) and each block adds up those before its own tile, which saves the
+// separate scan of the tile sums and the launch it would need.
+template <class T, bool kPrefixed>
 __global__ __launch_bounds__(kThreads) void tile_scan(const T *__restrict__ in, T *__restrict__ out,
                                                       int64_t n, const T *__restrict__ partial,
                                                       T *__restrict__ d_total) {
     __shared__ T lds[kThreads / 64];
+    __shared__ T off_s;
+    if constexpr (kPrefixed) {
+        T s = 0;
+        for (int64_t t = threadIdx.x; t < (int64_t)blockIdx.x; t += kThreads) s += partial[t];
+        if constexpr (sizeof(T) == 8) s = rtgd::wave_sum_u64(s);
+        else s = rtgd::wave_sum_u32(s);
+        if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            T t = 0;
+            for (int w = 0; w < kThreads / 64; ++w) t += lds[w];
+            off_s = t;
+        }
+        __syncthreads();  // lds is reused by the tile's own scan
+    }
     const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
     T v[kItems];
     T local = 0;
@@ -68,7 +86,9 @@ __global__ __launch_bounds__(kThreads) void tile_scan(const T *__restrict__ in, 
     }
     T tile_total;
     T run = block_exclusive_sum<T>(local, lds, &tile_total);
-    const T off = partial ? partial[blockIdx.x] : (T)0;
+    T off = 0;
+    if constexpr (kPrefixed) off = off_s;
+    else if (partial) off = partial[blockIdx.x];
     run += off;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
@@ -98,7 +118,7 @@ int scan_impl(rtg_ctx *ctx, const T *in, T *out, int64_t n, T *d_total, char *wo
     }
     const int64_t tiles = rtg::ceil_div(n, kTile);
     if (tiles == 1) {
-        tile_scan<T><<<1, kThreads, 0, ctx->stream>>>(in, out, n, nullptr, d_total);
+        tile_scan<T, false><<<1, kThreads, 0, ctx->stream>>>(in, out, n, nullptr, d_total);
         HIP_TRY(hipGetLastError());
         return RTG_OK;
     }
@@ -106,8 +126,13 @@ int scan_impl(rtg_ctx *ctx, const T *in, T *out, int64_t n, T *d_total, char *wo
     char *next = work + rtg::Arena::round(sizeof(T) * (size_t)tiles);
     tile_reduce<T><<<(unsigned)tiles, kThreads, 0, ctx->stream>>>(in, n, partial);
     HIP_TRY(hipGetLastError());
+    if (tiles <= kTile) {  // each block adds up the tile sums before it (at most kTile - 1)
+        tile_scan<T, true><<<(unsigned)tiles, kThreads, 0, ctx->stream>>>(in, out, n, partial, d_total);
+        HIP_TRY(hipGetLastError());
+        return RTG_OK;
+    }
     RTG_TRY(scan_impl<T>(ctx, partial, partial, tiles, (T *)nullptr, next));
-    tile_scan<T><<<(unsigned)tiles, kThreads, 0, ctx->stream>>>(in, out, n, partial, d_total);
+    tile_scan<T, false><<<(unsigned)tiles, kThreads, 0, ctx->stream>>>(in, out, n, partial, d_total);
     HIP_TRY(hipGetLastError());
     return RTG_OK;
 }
