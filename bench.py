@@ -285,7 +285,8 @@ def config_model_leg(rtg, ctx, steps: int, tune_cache, autotune: bool) -> dict:
     """Batched TC on the power-law configuration-model graph (n = 3,000,000, exponent 2.1,
     degrees up to 30,000, triangle closing 0.3, seed 1; ~117M undirected edges), relabelled and
     degree-oriented as the count's own input: plan (tuned once, then from the cache), median
-    kernel time of `steps` runs, and the plain wave-per-edge kernel once, whose count must match."""
+    kernel time of `steps` runs, and the plain wave-per-edge kernel once, whose count must match
+    (as must the golden count of the oracle's run on the same graph, tests/golden/golden.json)."""
     t0 = time.perf_counter()
     deg = rtg.gen_powerlaw_degrees(3_000_000, 2.1, 30_000.0, 50.0, seed=1)
     d_pairs = rtg.gen_config_model_device(ctx, deg, 0.3, seed=1)
@@ -315,8 +316,10 @@ def config_model_leg(rtg, ctx, steps: int, tune_cache, autotune: bool) -> dict:
            "batched_over_plain": s_plain["t_count_ms"] / t_ms, "plain_count_equal": c_plain == cnt,
            "plan": {k: desc[k] for k in ("threads", "unroll", "item", "source") if k in desc}}
     g.close()
-    if c_plain != cnt:
-        raise RuntimeError(f"config-model counts disagree: batched {cnt}, plain wave {c_plain}")
+    gold = golden_triangles("plcm_n3000000")
+    out["golden_count_equal"] = None if gold is None else cnt == gold
+    if c_plain != cnt or (gold is not None and cnt != gold):
+        raise RuntimeError(f"config-model counts disagree: batched {cnt}, plain wave {c_plain}, golden {gold}")
     log(f"config model: n={out['n']} m_oriented={out['m_oriented']} (generated and built in {t_gen:.1f}s), "
         f"batched {t_ms:.3f} ms ({out['edges_per_s']:.4g} edges/s, {out['achieved_gbs']:.0f} GB/s algorithmic), "
         f"plain wave {s_plain['t_count_ms']:.2f} ms, triangles {cnt}")
