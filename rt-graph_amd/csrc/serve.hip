@@ -73,7 +73,7 @@ struct Slot {
     uint32_t *z_pair_out = nullptr;
     int32_t *z_lookup_out = nullptr;
     std::vector<Req *> reqs;
-    bool busy = false;
+    bool busy = false;  // dispatcher thread only; other threads read rtg_server::busy_slots
     int64_t fake_deadline = 0;
 };
 
@@ -146,6 +146,7 @@ struct rtg_server {
     // statistics (under mu)
     int64_t admitted = 0, rejected = 0, completed = 0, cancelled = 0, failed = 0, batches = 0;
     int64_t max_batch = 0, max_busy = 0, batched_requests = 0;
+    int64_t busy_slots = 0;  // slots holding a batch (taken and not yet completed)
     int64_t t_first_arrival = 0, t_last_done = 0;
     double device_ms = 0, pack_ms = 0, unpack_ms = 0, slot_ms = 0;  // summed over batches
     double max_device_ms = 0, max_slot_ms = 0;
@@ -166,13 +167,16 @@ void release(rtg_server *s, Req *r, int32_t status) {
 }
 
 // Takes requests from the queue into `slot` while it has room; expired requests are
-// cancelled instead. Caller holds s->mu.
+// cancelled instead. Caller holds s->mu. Each request taken is claimed here, PENDING ->
+// RUNNING, under the lock that rtg_server_cancel also holds: a cancel either happened before
+// (the request is CANCELLED and dropped) or fails afterwards, so a request is never both
+// cancelled and launched.
 void take_batch(rtg_server *s, Slot &slot, int64_t now) {
     int64_t np = 0, nk = 0, ni = 0;
     slot.reqs.clear();
     while (!s->pending.empty() && (int64_t)slot.reqs.size() < s->opts.max_batch_requests) {
         Req *r = s->pending.front();
-        if (r->status.load(std::memory_order_acquire) == RTG_REQ_CANCELLED) {  // cancelled while queued
+        if (r->status.load(std::memory_order_acquire) != RTG_REQ_PENDING) {  // cancelled while queued
             s->pending.pop_front();
             continue;
         }
@@ -184,14 +188,22 @@ void take_batch(rtg_server *s, Slot &slot, int64_t now) {
         }
         if (r->kind == RTG_REQ_PAIR) {
             if (np + 1 > s->opts.max_batch_pairs || ni + r->items > s->max_items) break;
+        } else {
+            if (nk + r->nkeys > s->opts.max_batch_keys) break;
+        }
+        int32_t expected = RTG_REQ_PENDING;
+        if (!r->status.compare_exchange_strong(expected, RTG_REQ_RUNNING, std::memory_order_acq_rel)) {
+            s->pending.pop_front();
+            continue;
+        }
+        s->pending.pop_front();
+        if (r->kind == RTG_REQ_PAIR) {
             r->pos = np++;
             ni += r->items;
         } else {
-            if (nk + r->nkeys > s->opts.max_batch_keys) break;
             r->pos = nk;
             nk += r->nkeys;
         }
-        s->pending.pop_front();
         slot.reqs.push_back(r);
     }
 }
@@ -210,7 +222,6 @@ int launch_slot(rtg_server *s, Slot &slot) {
     int64_t np = 0, nk = 0, ni = 0;
     const int64_t t_pack = now_ns();
     for (Req *r : slot.reqs) {
-        r->status.store(RTG_REQ_RUNNING, std::memory_order_release);
         if (r->kind == RTG_REQ_PAIR) {
             slot.h_pairs[2 * r->pos] = (uint32_t)r->u;
             slot.h_pairs[2 * r->pos + 1] = (uint32_t)r->v;
@@ -300,6 +311,7 @@ void complete_slot(rtg_server *s, Slot &slot, bool ok) {
         }
     }
     slot.reqs.clear();
+    --s->busy_slots;
     slot.busy = false;
 }
 
@@ -338,6 +350,7 @@ void dispatcher(rtg_server *s) {
                 take_batch(s, slot, now_ns());
                 if (slot.reqs.empty()) continue;
                 ++s->batches;
+                ++s->busy_slots;
                 s->batched_requests += (int64_t)slot.reqs.size();
                 s->max_batch = std::max<int64_t>(s->max_batch, (int64_t)slot.reqs.size());
             }
@@ -650,12 +663,11 @@ RTG_API int rtg_server_destroy(rtg_server *s) {
     if (!s) return RTG_OK;
     // Let in-flight batches finish, then stop the dispatcher.
     for (int spin = 0; spin < 200000; ++spin) {
-        bool idle = true;
+        bool idle;
         {
             std::lock_guard<std::mutex> lock(s->mu);
-            if (!s->pending.empty()) idle = false;
+            idle = s->pending.empty() && s->busy_slots == 0;
         }
-        for (Slot &slot : s->slots) idle &= !slot.busy;
         if (idle) break;
         std::this_thread::sleep_for(std::chrono::microseconds(50));
     }

@@ -118,3 +118,50 @@ def test_request_table_across_chunks():
         s.poll(ids[0])
     assert s.stats()["completed"] == 70_000
     s.close()
+
+
+def test_cancel_races_dispatch():
+    """Cancels issued while the dispatcher is taking batches: every request ends exactly once,
+    as CANCELLED when its cancel succeeded and DONE when the cancel found it already claimed,
+    and the counters add up (a request is never both cancelled and launched)."""
+    import threading
+
+    s = server(slots=4, null_latency_us=30, max_batch_requests=16, mem_budget_bytes=16 * 4096)
+    n = 6000
+    ids = [0] * n
+    cancelled_ok = [False] * n
+    submitted = threading.Semaphore(0)
+
+    def canceller():
+        for i in range(n):
+            submitted.acquire()
+            try:
+                s.cancel(ids[i])
+                cancelled_ok[i] = True
+            except rtg.RTGError as e:
+                assert e.code == 6  # RTG_EAGAIN: already claimed by a batch or finished
+
+    t = threading.Thread(target=canceller)
+    t.start()
+    for i in range(n):
+        while True:
+            try:
+                ids[i] = s.submit_pair(i % 97, i % 89)
+                break
+            except rtg.RTGError as e:  # budget full: let the batches drain
+                assert e.code == 6
+                time.sleep(50e-6)
+        submitted.release()
+    t.join()
+    finals = [s.wait(r)[0] for r in ids]
+    for ok, st in zip(cancelled_ok, finals):
+        assert st == (rtg.REQ_CANCELLED if ok else rtg.REQ_DONE)
+    st = s.stats()
+    assert st["cancelled"] == sum(cancelled_ok)
+    assert st["completed"] == n - sum(cancelled_ok)
+    assert st["completed"] + st["cancelled"] == st["admitted"] == n
+    # the admitted bytes all returned to the budget: a full budget's worth is admitted again
+    s2_ids = [s.submit_pair(0, 1) for _ in range(4096)]
+    for r in s2_ids:
+        assert s.wait(r)[0] == rtg.REQ_DONE
+    s.close()
