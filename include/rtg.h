@@ -227,6 +227,20 @@ typedef struct rtg_si_stats {
 int rtg_si_pairs(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements,
                  int64_t nlists, const uint32_t *d_pairs, int64_t npairs, int32_t method,
                  uint32_t *d_out, rtg_si_stats *stats);
+/* Allocation-free form of rtg_si_pairs, for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It has no
+   counterpart in the reference. Same inputs, outputs and results as rtg_si_pairs for the
+   methods PLAIN, BATCHED, REFERENCE, MERGE, HASH and BITMAP (AUTO is refused: its tuning keeps
+   host-side state), at most RTG_SI_NOALLOC_MAX_PAIRS pairs (more: RTG_EINVAL), no statistics
+   beyond the kernel time. Its scratch (per-pair unit offsets, scan sums, flags) is one
+   file-scope device array of about 16.1 MiB per GPU, held by the library until the process
+   ends and shared by every call on that GPU: calls that overlap on one GPU (two contexts or
+   threads on the same device) race and must be serialised by the caller. The call returns
+   after the result is complete.                                                             */
+#define RTG_SI_NOALLOC_MAX_PAIRS (1 << 21)
+int rtg_si_pairs_noalloc(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements,
+                         int64_t nlists, const uint32_t *d_pairs, int64_t npairs, int32_t method,
+                         uint32_t *d_out, double *t_kernel_ms /* may be NULL */);
 /* The reference's A x B tile mode: sets [0,num_a) against [num_a, num_a+num_b);
    d_counts[a*num_b + b] = |A_a & B_b| (BinarySearchBasedIntersection, bs.cu:91-135).        */
 int rtg_si_tile(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements,

@@ -547,6 +547,20 @@ def si_pairs(ctx: Context, d_offsets: DeviceArray, d_elements: DeviceArray, nlis
     return d_out, st.as_dict()
 
 
+SI_NOALLOC_MAX_PAIRS = 1 << 21  # RTG_SI_NOALLOC_MAX_PAIRS
+
+
+def si_pairs_noalloc(ctx: Context, d_offsets: DeviceArray, d_elements: DeviceArray, nlists: int,
+                     d_pairs: DeviceArray, npairs: int, method: int, d_out: DeviceArray) -> float:
+    """rtg_si_pairs_noalloc: rtg_si_pairs without any device allocation (scratch in the
+    library's fixed per-GPU array); d_out is the caller's. Returns the kernel time in ms."""
+    ms = C.c_double()
+    _check(_lib.rtg_si_pairs_noalloc(ctx.handle, _ptr(d_offsets), _ptr(d_elements), C.c_int64(nlists),
+                                     _ptr(d_pairs), C.c_int64(npairs), C.c_int32(method), _ptr(d_out),
+                                     C.byref(ms)))
+    return ms.value
+
+
 def si_tile(ctx: Context, d_offsets: DeviceArray, d_elements: DeviceArray, num_a: int, num_b: int,
             method: int = SI_BATCHED, d_out: DeviceArray | None = None):
     """rtg_si_tile: counts[a * num_b + b] = |A_a & B_b| for sets [0, num_a) and [num_a, num_a + num_b)."""

@@ -78,6 +78,8 @@ RTG_API int rtg_create(int device, rtg_ctx **out) {
         if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { st = rtg_fail(RTG_EHIP, "hipStreamCreate failed"); break; }
         for (auto &e : ctx->ev)
             if (hipEventCreate(&e) != hipSuccess) { st = rtg_fail(RTG_EHIP, "hipEventCreate failed"); break; }
+        if (st != RTG_OK) break;
+        (void)rtg::si_noalloc_init(device);  // status kept per device; rtg_si_pairs_noalloc reports it
     } while (0);
     if (st != RTG_OK) {
         rtg_destroy(ctx);

@@ -57,6 +57,11 @@ int ctx_enter(rtg_ctx *ctx);
 // Returns at least `bytes` of device scratch owned by the context (contents undefined).
 int ctx_scratch(rtg_ctx *ctx, size_t bytes, void **out);
 
+// Looks up, once per device, the device address of the set-intersection scratch array used by
+// rtg_si_pairs_noalloc (si.hip). Called by rtg_create with `device` current; a failure is
+// recorded and reported by that entry point.
+int si_noalloc_init(int device);
+
 // Carves aligned sub-buffers out of one allocation.
 struct Arena {
     char *base = nullptr;

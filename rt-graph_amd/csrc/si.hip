@@ -816,6 +816,19 @@ int autotune(rtg_ctx *ctx, Lists L, PairSrc P, int64_t npairs, const BatchBufs &
     return RTG_OK;
 }
 
+// The counting launch of a method (kind: its long-pair strategy, or kSampled when AUTO picked
+// the block-per-pair form); B is unused by PLAIN and REFERENCE.
+int launch_method(rtg_ctx *ctx, Lists L, PairSrc P, int64_t npairs, int32_t method, int kind, const BatchBufs &B,
+                  uint32_t *d_out) {
+    if (method == RTG_SI_PLAIN) {
+        si_plain<<<grid_for(ctx, npairs * 64, 32), kThreads, 0, ctx->stream>>>(L, P, npairs, d_out);
+        HIP_TRY(hipGetLastError());
+        return RTG_OK;
+    }
+    if (method == RTG_SI_REFERENCE || kind == kSampled) return launch_sampled(ctx, L, P, npairs, d_out);
+    return launch_batched(ctx, kind, L, P, npairs, B, d_out);
+}
+
 // nlists: lists in d_off; pairs naming an id at or above it count 0 and make the call fail
 // with RTG_EINVAL after it ran (the kernels never read past the offsets).
 int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, uint32_t nlists, PairSrc P, int64_t npairs,
@@ -866,14 +879,7 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, uint32_t n
         }
     }
     HIP_TRY(hipEventRecord(ctx->ev[0], st));
-    if (method == RTG_SI_PLAIN) {
-        si_plain<<<grid_for(ctx, npairs * 64, 32), kThreads, 0, st>>>(L, P, npairs, d_out);
-    } else if (method == RTG_SI_REFERENCE || kind == kSampled) {
-        RTG_TRY(launch_sampled(ctx, L, P, npairs, d_out));
-    } else {
-        RTG_TRY(launch_batched(ctx, kind, L, P, npairs, B, d_out));
-    }
-    HIP_TRY(hipGetLastError());
+    RTG_TRY(launch_method(ctx, L, P, npairs, method, kind, B, d_out));
     HIP_TRY(hipEventRecord(ctx->ev[1], st));
     uint32_t bad = 0;
     if (stats) {
@@ -912,6 +918,97 @@ int si_run(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, uint32_t n
 
 }  // namespace
 
+// ---- allocation-free entry point ---------------------------------------------------------
+// rtg_si_pairs_noalloc keeps its scratch — the out-of-range flag, the unit total, the per-pair
+// unit offsets and the scan's tile sums — in this file-scope device array instead of memory
+// allocated per call. Every GPU holds its own copy (loaded with the code object, held until the
+// process ends); host code reaches it only through the address hipGetSymbolAddress returns for
+// the current device, never through the array's name, which in host code is only a stand-in.
+// Layout: [0, 4) flag, [64, 96) unit total, [256, 256 + 8 * RTG_SI_NOALLOC_MAX_PAIRS) unit
+// offsets, then the scan work.
+namespace {
+constexpr size_t kNoallocOffsets = 256;
+constexpr size_t kNoallocScanWork = 64 * 1024;
+constexpr size_t kNoallocBytes = kNoallocOffsets + 8 * (size_t)RTG_SI_NOALLOC_MAX_PAIRS + kNoallocScanWork;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+namespace {
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoallocMaxDevices];      // device address of `scratch`, per device
+int g_noalloc_status[kNoallocMaxDevices];     // 0: not looked up yet, 1: ok, 2: lookup failed
+
+__global__ void noalloc_clear_flag(uint32_t *flag) {
+    if (threadIdx.x == 0) *flag = 0u;
+}
+
+}  // namespace
+
+namespace rtg {
+
+// Called by rtg_create with `device` current: looks up this device's copy of the scratch array
+// once per device and keeps its address.
+int si_noalloc_init(int device) {
+    if (device < 0 || device >= kNoallocMaxDevices) return RTG_OK;  // the entry point will refuse
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_status[device] != 0) return RTG_OK;
+    static_assert(kNoallocScanWork >= 4096, "scan work");
+    if (scan_work_bytes(RTG_SI_NOALLOC_MAX_PAIRS) > kNoallocScanWork) {
+        g_noalloc_status[device] = 2;
+        return set_error(RTG_EHIP, "set-intersection scratch: scan work does not fit");
+    }
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        g_noalloc_status[device] = 2;
+        return set_error(RTG_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_noalloc_ptr[device] = ptr;
+    g_noalloc_status[device] = 1;
+    return RTG_OK;
+}
+
+}  // namespace rtg
+
+namespace {
+
+int si_run_noalloc(rtg_ctx *ctx, const uint32_t *d_off, const uint32_t *d_el, uint32_t nlists, PairSrc P,
+                   int64_t npairs, int32_t method, uint32_t *d_out, double *t_kernel_ms) {
+    using namespace rtg;
+    hipStream_t st = ctx->stream;
+    char *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (ctx->device < kNoallocMaxDevices && g_noalloc_status[ctx->device] == 1)
+            base = static_cast<char *>(g_noalloc_ptr[ctx->device]);
+    }
+    if (!base) return set_error(RTG_EHIP, "rtg_si_pairs_noalloc: the scratch array of this device is unavailable");
+    uint32_t *d_bad = reinterpret_cast<uint32_t *>(base);
+    BatchBufs B{reinterpret_cast<uint64_t *>(base + kNoallocOffsets), reinterpret_cast<uint64_t *>(base + 64),
+                base + kNoallocOffsets + 8 * (size_t)RTG_SI_NOALLOC_MAX_PAIRS};
+    const int kind = long_of(method);
+    Lists L{d_off, d_el, nlists, d_bad};
+    // The flag keeps the previous call's value: cleared by a kernel (no memset node, so the
+    // sequence can also be recorded into a graph by a caller).
+    noalloc_clear_flag<<<1, 64, 0, st>>>(d_bad);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ctx->ev[0], st));
+    RTG_TRY(launch_method(ctx, L, P, npairs, method, kind, B, d_out));
+    HIP_TRY(hipEventRecord(ctx->ev[1], st));
+    uint32_t bad = 0;
+    HIP_TRY(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (t_kernel_ms) *t_kernel_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+    if (bad) return set_error(RTG_EINVAL, "rtg_si_pairs_noalloc: a pair names a list id at or above the number of lists (counted as 0)");
+    return RTG_OK;
+}
+
+}  // namespace
+
 namespace rtg {
 
 int si_launch_items(rtg_ctx *ctx, hipStream_t st, const uint32_t *d_off, const uint32_t *d_el, const uint32_t *d_pairs,
@@ -941,6 +1038,27 @@ RTG_API int rtg_si_pairs(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t
     std::lock_guard<std::mutex> lock(ctx->mu);
     PairSrc P{d_pairs, 0, 1};
     return si_run(ctx, d_offsets, d_elements, (uint32_t)nlists, P, npairs, method, d_out, stats);
+}
+
+RTG_API int rtg_si_pairs_noalloc(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements, int64_t nlists,
+                                 const uint32_t *d_pairs, int64_t npairs, int32_t method, uint32_t *d_out,
+                                 double *t_kernel_ms) {
+    RTG_TRY(rtg::ctx_enter(ctx));
+    if (t_kernel_ms) *t_kernel_ms = 0.0;
+    if (nlists < 0 || npairs < 0 || !d_offsets || (npairs > 0 && (!d_pairs || !d_out)))
+        return rtg_fail(RTG_EINVAL, "rtg_si_pairs_noalloc: bad arguments");
+    if (((uintptr_t)d_pairs & 7) != 0) return rtg_fail(RTG_EINVAL, "rtg_si_pairs_noalloc: d_pairs must be 8-byte aligned");
+    if (nlists > (int64_t)0xFFFFFFFFu)
+        return rtg_fail(RTG_EINVAL, "rtg_si_pairs_noalloc: list ids are 32-bit (at most 2^32 - 1 lists)");
+    if (npairs > RTG_SI_NOALLOC_MAX_PAIRS)
+        return rtg_fail(RTG_EINVAL, "rtg_si_pairs_noalloc: more than RTG_SI_NOALLOC_MAX_PAIRS (" +
+                                        std::to_string((long long)RTG_SI_NOALLOC_MAX_PAIRS) + ") pairs");
+    if (method != RTG_SI_PLAIN && method != RTG_SI_REFERENCE && long_of(method) < 0)
+        return rtg_fail(RTG_EINVAL, "rtg_si_pairs_noalloc: method must be PLAIN, BATCHED, REFERENCE, MERGE, HASH or BITMAP");
+    if (npairs == 0) return RTG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    PairSrc P{d_pairs, 0, 1};
+    return si_run_noalloc(ctx, d_offsets, d_elements, (uint32_t)nlists, P, npairs, method, d_out, t_kernel_ms);
 }
 
 RTG_API int rtg_si_tile(rtg_ctx *ctx, const uint32_t *d_offsets, const uint32_t *d_elements, uint32_t num_a,
