@@ -140,12 +140,37 @@ def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
     srv = rtg.Server(ctx, g, d_arr, 1 << 28, slots=2)
     warm = min(1 << 16, nqueries // 8)
     st, _, _ = srv.replay(rate_qps=rate, nqueries=nqueries + warm, warmup=warm, pair_frac=0.7, max_keys=4096, seed=1)
+    # Answers: a bounded stream of the same mix (2^18 queries, seed 2) replayed with its results
+    # kept; every pair answer and a sample of lookup batches are checked against the oracle.
+    nchk = 1 << 18
+    st_c, pr, ls = srv.replay(rate_qps=rate, nqueries=nchk, warmup=0, pair_frac=0.7, max_keys=4096, seed=2,
+                              want_results=True)
     srv.close()
+    orc = oracle_module()
+    qs = rtg.stream_queries(g, rate, nchk, pair_frac=0.7, max_keys=4096, seed=2)
+    off, adj = g.download()
+    isp = qs["kind"] == rtg.REQ_PAIR
+    uv = np.stack([qs["u"][isp], qs["v"][isp]], 1).astype(np.uint32).reshape(-1)
+    exp = orc.si_pairs(off.astype(np.uint32), adj.astype(np.uint32), uv, ORACLE_THREADS[0])
+    del off, adj
+    arr = d_arr.numpy()
+    lk = np.flatnonzero(~isp)
+    sample = np.random.default_rng(2).choice(lk, size=min(2000, lk.size), replace=False)
+    bad = int(np.count_nonzero(pr[isp] != exp))
+    for i in sample:
+        keys = qs["pool"][qs["key_off"][i]: qs["key_off"][i] + qs["nkeys"][i]]
+        bad += int(ls[i]) != int((orc.lookup(arr, keys).astype(np.int64) + 2).sum())
+    del arr
+    if bad or st_c["completed"] != nchk:
+        raise RuntimeError(f"serving: {bad} answers differ from the oracle (completed {st_c['completed']} of {nchk})")
+    checked = {"queries": nchk, "pairs": int(isp.sum()), "lookup_batches": int(sample.size),
+               "lookup_keys": int(qs["nkeys"][sample].sum()), "p99_us": st_c["p99_us"]}
+    log(f"serving answers checked: {checked}")
     log(f"serving: offered {rate:.3g} q/s, sustained {st['throughput_qps']:.3g} q/s, p50 {st['p50_us']:.1f} us, "
         f"p99 {st['p99_us']:.1f} us, admitted {st['admitted']} rejected {st['rejected']}, "
         f"mean batch {st['mean_batch_requests']:.1f} requests, busy slots up to {st['max_busy_slots']}")
     out = {"graph": {"n": g.n, "undirected_edges": g.m // 2, "generated_pairs": npairs, "max_degree": g.max_row},
-           "array_keys": 1 << 28, "queries": nqueries, "warmup": warm,
+           "array_keys": 1 << 28, "queries": nqueries, "warmup": warm, "answers_checked": checked,
            "lookup_index_keys_per_s": nk / (lk_ms * 1e-3),
            **{k: st[k] for k in ("offered_qps", "throughput_qps", "p50_us", "p99_us", "p999_us", "mean_us", "max_us",
                                   "admitted", "rejected", "completed", "cancelled", "failed", "batches",
@@ -155,6 +180,64 @@ def serving_leg(rtg, ctx, rate: float, nqueries: int) -> dict:
     g.close()
     d_arr.free()
     return out
+
+
+# The reference's only published timings (rt_search/output/q24_unsorted_data.txt:225,269,302 and
+# q16_unsorted_data.txt:225,302; thrust sort of 2^27 keys at q24_unsorted_data.txt:302), on its
+# workload (rt_search/measure.cpp:24-58): N int32 uniform in [0, 10N] with duplicates, Q
+# unsorted uniform queries from the same range; kernel-only lookup time.
+REFERENCE_LOOKUP_MS = {(20, 24): 2.560, (24, 24): 6.573, (27, 24): 15.576, (20, 16): 0.0164, (27, 16): 0.1096}
+REFERENCE_SORT_MS = {27: 18.0}
+
+
+def lookup_ref_leg(rtg, ctx) -> list:
+    """rtg_sort_i32 + rtg_lookup_i32 on the reference's lookup workload: N in {2^20, 2^24, 2^27},
+    Q in {2^16, 2^24}, seed 1; median of 5 device times each, every result checked on a sample
+    of 2^16 queries against the host-sorted array."""
+    rng = np.random.default_rng(1)
+    rows = []
+    for ln in (20, 24, 27):
+        n = 1 << ln
+        data = rng.integers(0, 10 * n + 1, size=n, dtype=np.int64).astype(np.int32)
+        d_src = ctx.to_device(data)
+        d_dat = rtg.DeviceArray(ctx, (n,), np.int32)
+        sorts = []
+        for _ in range(5):
+            rtg._check(rtg._lib.rtg_memcpy_d2d(ctx.handle, rtg.C.c_void_p(d_dat.ptr), rtg.C.c_void_p(d_src.ptr),
+                                               rtg.C.c_size_t(4 * n)))
+            sorts.append(rtg.sort_device(ctx, d_dat, n, signed=True))
+        d_src.free()
+        host_sorted = np.sort(data)
+        del data
+        sample = rng.integers(0, n, size=1 << 16)
+        if not np.array_equal(d_dat.numpy()[sample], host_sorted[sample]):
+            raise RuntimeError(f"device sort of 2^{ln} int32 differs from the host sort")
+        for lq in (16, 24):
+            q = 1 << lq
+            query = rng.integers(0, 10 * n + 1, size=q, dtype=np.int64).astype(np.int32)
+            d_q = ctx.to_device(query)
+            d_out = rtg.DeviceArray(ctx, (q,), np.int32)
+            ts = [rtg.lookup(ctx, d_dat, n, d_q, q, d_out=d_out, signed=True)[1] for _ in range(5)]
+            got = d_out.numpy()
+            k = np.arange(q) if q <= (1 << 16) else rng.integers(0, q, size=1 << 16)
+            exp = oracle_module().lookup(host_sorted, query[k], signed=True)
+            if not np.array_equal(got[k], exp):
+                raise RuntimeError(f"lookup N=2^{ln} Q=2^{lq}: results differ from the oracle")
+            t = sorted(ts)[2]
+            ref = REFERENCE_LOOKUP_MS.get((ln, lq))
+            row = {"n": n, "q": q, "lookup_ms": t, "keys_per_s": q / (t * 1e-3), "reference_ms": ref,
+                   "speedup_vs_reference": (ref / t) if ref else None, "checked": int(k.size)}
+            if lq == 24:
+                row["sort_ms"] = sorted(sorts)[2]
+                row["sort_keys_per_s"] = n / (row["sort_ms"] * 1e-3)
+                row["reference_sort_ms"] = REFERENCE_SORT_MS.get(ln)
+            rows.append(row)
+            log(f"lookup N=2^{ln} Q=2^{lq}: {t:.4f} ms" + (f" (reference {ref} ms, {ref / t:.1f}x)" if ref else "") +
+                (f"; sort {row['sort_ms']:.3f} ms" if "sort_ms" in row else ""))
+            d_q.free()
+            d_out.free()
+        d_dat.free()
+    return rows
 
 
 def si_leg(rtg, ctx) -> dict:
@@ -390,18 +473,24 @@ def copy_peak_gbs(rtg, ctx, nbytes: int = 1 << 30) -> float:
     return 2 * nbytes / best / 1e9
 
 
-def read_traffic(kernel: str, workload: str):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+def read_traffic(kernel: str, workload: str, revision: str | None):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary of this kernel and workload,
+    with the file's name and kernel revision; (None, reason) when the file is missing or was
+    taken from another revision of the kernel than the one that ran."""
     p = ROOT / "profiles" / f"{kernel}_{workload}_pmc.txt"
     if not p.exists():
-        return None
+        return None, f"no {p.name}"
+    fields = {}
     for line in p.read_text().splitlines():
-        if line.startswith("hbm_bytes_per_launch:"):
-            try:
-                return float(line.split(":", 1)[1])
-            except ValueError:
-                return None
-    return None
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            fields[k.strip()] = v.strip()
+    if revision is None or fields.get("kernel_revision") != revision:
+        return None, f"{p.name} is from kernel revision {fields.get('kernel_revision')}, this run used {revision}"
+    try:
+        return float(fields["hbm_bytes_per_launch"]), f"{p.name} (kernel revision {revision})"
+    except (KeyError, ValueError):
+        return None, f"{p.name} has no hbm_bytes_per_launch"
 
 
 def main() -> int:
@@ -420,6 +509,8 @@ def main() -> int:
     ap.add_argument("--no-si", action="store_true", help="skip the set-intersection microbench")
     ap.add_argument("--si-only", action="store_true",
                     help="run only the set-intersection legs and print their JSON (not the bench line)")
+    ap.add_argument("--no-lookup-ref", action="store_true",
+                    help="skip the device sort + lookup at the reference's published configurations")
     ap.add_argument("--serve-queries", type=int, default=1 << 24)
     ap.add_argument("--serve-rate", type=float, default=2.0e6)
     ap.add_argument("--tune-cache", default=str(ROOT / "rt-graph_amd" / "tune_cache.json"),
@@ -470,10 +561,13 @@ def main() -> int:
         log(f"COUNT MISMATCH: {cnt} != golden {gold}")
         return 1
     value = g.m / (t_ms / 1e3)
-    alg = st["alg_bytes"]
-    achieved = alg / (t_ms / 1e3) / 1e9
+    alg = st["alg_bytes"]          # per-edge model, no reuse across edges
+    staged = st["staged_bytes"]    # what the owner-staged launch must read at least
+    achieved = staged / (t_ms / 1e3) / 1e9
+    achieved_noreuse = alg / (t_ms / 1e3) / 1e9
     log(f"batched: {t_ms:.3f} ms/step (min {min(times):.3f}), {value:.4g} edges/s, count={cnt}, "
-        f"alg {alg / 1e9:.1f} GB -> {achieved:.0f} GB/s")
+        f"staged {staged / 1e9:.1f} GB -> {achieved:.0f} GB/s; per-edge model {alg / 1e9:.1f} GB -> "
+        f"{achieved_noreuse:.0f} GB/s")
 
     plain = {}
     if not args.no_plain:
@@ -503,8 +597,10 @@ def main() -> int:
     if not args.no_serving:
         serving = serving_leg(rtg, ctx, args.serve_rate, args.serve_queries)
     ba_rows = ba_rows_leg(rtg, ctx)
+    lookup_ref = None if args.no_lookup_ref else lookup_ref_leg(rtg, ctx)
 
-    traffic = read_traffic("tc_batched", args.workload)
+    traffic, traffic_src = read_traffic("tc_batched", args.workload, desc.get("rev"))
+    log(f"traffic: {traffic_src}")
     copy_gbs = copy_peak_gbs(rtg, ctx)
     log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
     # the build's model traffic: 8-byte keys read and written per radix pass, 16 B per raw pair
@@ -518,7 +614,12 @@ def main() -> int:
                    "method": "batched: owner-grouped two-choice cuckoo LDS tables, persistent work queue, hipGraph",
                    "plan": desc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "measured_copy_gbs": copy_gbs},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "traffic_frac": (traffic / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                     "bytes_model": "staged: 4*min(d+u,d+v) + 8 per work edge, each owner list once per item, "
+                                    "queue and small-owner offsets",
+                     "staged_bytes": staged, "alg_bytes_noreuse": alg, "achieved_noreuse": achieved_noreuse,
+                     "measured_copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "plain_ms": plain, "batched_over_plain": (plain_ms / t_ms) if plain_ms else None,
         "t_build_ms": bst.t_total_ms, "build_model_bytes": build_bytes,
@@ -527,7 +628,7 @@ def main() -> int:
         "per_edge": per_edge, "ba_100k": ba_rows, "config_model_3m": config_model,
         "set_intersection": si,
         "wall_s_timed": wall, "alg_bytes": alg, "merge_bytes": st["merge_bytes"],
-        "serving": serving,
+        "serving": serving, "lookup_reference_configs": lookup_ref,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -164,6 +164,10 @@ typedef struct rtg_tc_stats {
     double bucket_ms[RTG_TC_MAX_BUCKETS];
     int32_t kernel_launches;
     int32_t reserved;
+    uint64_t staged_bytes;   /* batched: bytes the owner-staged launch must read at least — each
+                                partner list once per edge (4*min(d+(u), d+(v))), each owner's
+                                list once per work item, the partner spans (8 B per edge), the
+                                queue and the small-owner offsets; 0 for the plain methods     */
 } rtg_tc_stats;
 
 int rtg_tc_count(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *opts, uint64_t *count,
@@ -254,6 +258,11 @@ int rtg_lookup_u32(rtg_ctx *ctx, const uint32_t *d_data, int64_t n, const uint32
                    int64_t q, int32_t *d_out, double *t_kernel_ms);
 int rtg_lookup_i32(rtg_ctx *ctx, const int32_t *d_data, int64_t n, const int32_t *d_keys,
                    int64_t q, int32_t *d_out, double *t_kernel_ms);
+/* Ascending device sort of n 32-bit keys in place (the thrust::sort of the lookup array in
+   BSAccessCountFunction, rt_search/simple_bs.cu:99-108): LSD radix sort, 8-bit digits, with a
+   temporary of n keys. t_ms (may be NULL) receives the device time of the sort.             */
+int rtg_sort_u32(rtg_ctx *ctx, uint32_t *d_data, int64_t n, double *t_ms);
+int rtg_sort_i32(rtg_ctx *ctx, int32_t *d_data, int64_t n, double *t_ms);
 /* The same lookup through a search index built once per array, for callers that look up in
    one array many times (the reference re-searches its array per call, simple_bs.cu:80-142;
    this is the analogue of its build-once search structure). The index is a 32-ary tree of

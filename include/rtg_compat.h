@@ -32,9 +32,12 @@ void CountOnGPU(int *adjs, int *offsets, int nodes, int edges, int device_id);
 int rtg_count_on_gpu(const int *adjs, const int *offsets, int nodes, int64_t edges, int device_id,
                      int method, uint64_t *count, double *ms);
 
-/* Replaces rt_search/simple_bs.h:5 (rt_search/simple_bs.cu:80-142): sorts `data` on the host
-   in place (as the reference does at :92), looks every query up on the device and writes the
-   index of the first equal element, or -1, to bs_results. */
+/* Replaces rt_search/simple_bs.h:5 (rt_search/simple_bs.cu:80-142): copies `data` and `query`
+   to the device, sorts `data` on the host in place (as the reference does at :92) and on the
+   device (:99-108), looks every query up on the device and writes the index of the first
+   equal element, or -1, to bs_results. Prints the reference's lines "data size = %d",
+   "cpu sort time = %.3f ms" and "Sorting time = %f ms, Binary search time = %f ms, Total
+   time = %f ms" (:88, :95, :137-139); the times are device times of the sort and the lookup. */
 void BSAccessCountFunction(int *data, int data_size, int *query, int query_size, int *bs_results);
 
 /* C form of BinarySearchBasedIntersection (set_intersection/baseline/bs.cu:91-135): sets

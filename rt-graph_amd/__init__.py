@@ -107,7 +107,7 @@ class TcStats(C.Structure):
                 ("merge_bytes", C.c_uint64), ("edges", C.c_int64), ("nbuckets", C.c_int32),
                 ("strategy", C.c_int32 * TC_MAX_BUCKETS), ("bucket_items", C.c_int64 * TC_MAX_BUCKETS),
                 ("bucket_ms", C.c_double * TC_MAX_BUCKETS), ("kernel_launches", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("staged_bytes", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {
@@ -115,7 +115,7 @@ class TcStats(C.Structure):
             "merge_bytes": self.merge_bytes, "edges": self.edges, "nbuckets": self.nbuckets,
             "strategy": list(self.strategy[: self.nbuckets]),
             "bucket_items": list(self.bucket_items[: self.nbuckets]),
-            "kernel_launches": self.kernel_launches,
+            "kernel_launches": self.kernel_launches, "staged_bytes": self.staged_bytes,
         }
 
 
@@ -648,17 +648,33 @@ class LookupIndex:
             pass
 
 
+def sort_device(ctx: Context, d_data: DeviceArray, n: int, signed: bool = False) -> float:
+    """rtg_sort_u32 / rtg_sort_i32: ascending in-place device sort of n 32-bit keys; returns the
+    device time in ms."""
+    ms = C.c_double()
+    fn = _lib.rtg_sort_i32 if signed else _lib.rtg_sort_u32
+    _check(fn(ctx.handle, _ptr(d_data), C.c_int64(n), C.byref(ms)))
+    return ms.value
+
+
 def BSAccessCountFunction(data: np.ndarray, query: np.ndarray, device_id: int = 0) -> np.ndarray:
-    """rt_search/simple_bs.cu:80-142: sorts ``data`` (in place, as the reference does at :92)
-    and returns, per query, the index of the first equal element or -1."""
-    data.sort()
+    """rt_search/simple_bs.cu:80-142: the unsorted ``data`` goes to the device, is sorted there
+    (:99-108) and on the host in place (:92), and every query is looked up on the device.
+    Returns, per query, the index of the first equal element of the sorted array, or -1, and
+    prints the reference's lines."""
+    import time
     d = np.ascontiguousarray(data, dtype=np.int32)
     qv = np.ascontiguousarray(query, dtype=np.int32)
     with Context(device_id) as ctx:
         dd, dq = ctx.to_device(d if d.size else np.zeros(1, np.int32)), ctx.to_device(qv if qv.size else np.zeros(1, np.int32))
+        print(f"data size = {d.size}")
+        t0 = time.process_time()
+        data.sort()
+        print(f"cpu sort time = {(time.process_time() - t0) * 1e3:.3f} ms")
+        sort_ms = sort_device(ctx, dd, d.size, signed=True)
         out, ms = lookup(ctx, dd, d.size, dq, qv.size, signed=True)
         res = out.numpy()[: qv.size]
-        print(f"Binary search time = {ms:f} ms")
+        print(f"Sorting time = {sort_ms:f} ms, Binary search time = {ms:f} ms, Total time = {sort_ms + ms:f} ms")
         for a in (out, dq, dd):
             a.free()
     return res

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <ctime>
 #include <vector>
 
 namespace {
@@ -64,24 +65,30 @@ RTG_API void CountOnGPU(int *adjs, int *offsets, int nodes, int edges, int devic
 }
 
 RTG_API void BSAccessCountFunction(int *data, int data_size, int *query, int query_size, int *bs_results) {
-    if (data_size < 0 || query_size < 0) {
-        rtg_fail(RTG_EINVAL, "negative size");
+    if (data_size < 0 || query_size < 0 || (data_size > 0 && !data) || (query_size > 0 && (!query || !bs_results))) {
+        rtg_fail(RTG_EINVAL, "bad arguments");
         die("BSAccessCountFunction");
     }
-    std::sort(data, data + data_size);
     Ctx ctx;
     if (rtg_create(0, &ctx.c) != RTG_OK) die("BSAccessCountFunction");
     Dev dd(ctx.c), dq(ctx.c), dr(ctx.c);
     const size_t nb = sizeof(int) * (size_t)std::max(data_size, 1), qb = sizeof(int) * (size_t)std::max(query_size, 1);
+    // the unsorted array goes to the device first; the host copy is sorted separately (:85-95)
     if (rtg_malloc(ctx.c, nb, &dd.p) || rtg_malloc(ctx.c, qb, &dq.p) || rtg_malloc(ctx.c, qb, &dr.p) ||
         rtg_memcpy_h2d(ctx.c, dd.p, data, sizeof(int) * (size_t)data_size) ||
         rtg_memcpy_h2d(ctx.c, dq.p, query, sizeof(int) * (size_t)query_size))
         die("BSAccessCountFunction");
-    double ms = 0.0;
-    if (rtg_lookup_i32(ctx.c, (const int32_t *)dd.p, data_size, (const int32_t *)dq.p, query_size, (int32_t *)dr.p, &ms) ||
+    std::printf("data size = %d\n", data_size);
+    const std::clock_t begin = std::clock();
+    std::sort(data, data + data_size);
+    const std::clock_t end = std::clock();
+    std::printf("cpu sort time = %.3f ms\n", double(end - begin) / CLOCKS_PER_SEC * 1000.0);
+    double sort_ms = 0.0, bs_ms = 0.0;
+    if (rtg_sort_i32(ctx.c, (int32_t *)dd.p, data_size, &sort_ms) ||
+        rtg_lookup_i32(ctx.c, (const int32_t *)dd.p, data_size, (const int32_t *)dq.p, query_size, (int32_t *)dr.p, &bs_ms) ||
         rtg_memcpy_d2h(ctx.c, bs_results, dr.p, sizeof(int) * (size_t)query_size))
         die("BSAccessCountFunction");
-    std::printf("Binary search time = %f ms\n", ms);
+    std::printf("Sorting time = %f ms, Binary search time = %f ms, Total time = %f ms\n", sort_ms, bs_ms, sort_ms + bs_ms);
 }
 
 RTG_API int rtg_set_intersection_tile_host(const uint32_t *set_offsets, const uint32_t *elements, uint32_t num_a,

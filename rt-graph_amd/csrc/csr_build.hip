@@ -2,11 +2,16 @@
 // Graph::ReadBinaryEdges relabel + Graph::TCPreprocessing + Graph::ConvertToCSR
 // (tc/graph/graph.cpp:40-62, 104-177, 71-102) as data-parallel passes over HBM:
 //
-//   relabel   max/zero detection, presence marks, exclusive scan -> dense order-preserving ids
-//   degree    raw degree by atomics (duplicates and self-loops count, graph.cpp:109-114)
+//   degree    raw degree of every input id (duplicates and self-loops count,
+//             graph.cpp:109-114) by a partitioned LDS histogram: endpoints grouped by id range,
+//             each range counted in LDS, one global add per non-zero bin and block
+//   renumber  the relabel (present = degree > 0, dense and order-preserving, graph.cpp:40-62)
+//             and the degree<=1 removal (x - removes[x], graph.cpp:126-152) are both
+//             order-preserving renumberings, so the final id of a kept vertex is one exclusive
+//             scan of (degree > 1) over the input ids
 //   orient    src->dst iff deg[src] < deg[dst] or (equal and src < dst) (graph.cpp:116-124),
-//             degree<=1 vertices removed and the rest renumbered x - removes[x] (:126-152),
-//             surviving pairs packed into 64-bit keys (src << b | dst), b = bits(n')
+//             on the input ids (same order as the relabelled ones), kept pairs packed with
+//             their final ids into 64-bit keys (src << b | dst), b = bits(n')
 //   sort      stable LSD radix sort on 2b bits (replaces ThrustSortEdgesInBatch)
 //   csr       drop self-loops and repeats of the previous key (graph.cpp:159-173), compact
 //             by exclusive scan, write adjs and the row offsets (graph.cpp:71-102)
@@ -46,29 +51,119 @@ __global__ void minmax_kernel(const int32_t *__restrict__ v, int64_t n, MinMax *
     }
 }
 
-__global__ void mark_present(const int32_t *__restrict__ v, int64_t n, uint32_t *__restrict__ present) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        present[v[i]] = 1u;
-}
-
-__global__ void apply_map(const int32_t *__restrict__ in, int32_t *__restrict__ out, int64_t n,
-                          const uint32_t *__restrict__ newid) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = (int32_t)newid[in[i]];
-}
-
 __global__ void raw_degree(const int32_t *__restrict__ v, int64_t n, uint32_t *__restrict__ deg) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         atomicAdd(&deg[v[i]], 1u);
 }
 
-__global__ void removal_flags(const uint32_t *__restrict__ deg, int32_t n, uint32_t *__restrict__ rm) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        rm[i] = deg[i] <= 1u ? 1u : 0u;
+// ---- raw degree by a partitioned LDS histogram ----
+// The endpoints are first grouped by id >> kPartBits (one counting-sort pass: per-tile bin
+// histograms in LDS, a scan, a scatter through LDS cursors), then each block counts a
+// contiguous run of one partition's endpoints into an LDS histogram of the partition's
+// 2^kPartBits ids and adds its non-zero bins to the global degrees. One device-scope atomic per
+// (block, bin) instead of one per endpoint: R-MAT hubs, whose same-address atomics serialise,
+// are counted in LDS.
+constexpr int kPartBits = 14;                 // 16,384 ids (64 KiB of LDS counters) per partition
+constexpr int64_t kMaxPartitions = 1024;      // domains up to 2^24 ids; larger ones use raw_degree
+constexpr int kPartItems = 32;                // endpoints per thread in the partitioning tiles
+constexpr int64_t kPartTile = (int64_t)kT * kPartItems;
+constexpr int64_t kCountChunk = 1 << 17;      // endpoints per block in the counting pass
+
+__global__ __launch_bounds__(kT) void part_hist(const int32_t *__restrict__ v, int64_t n, int32_t nparts,
+                                                uint32_t *__restrict__ hist, uint32_t ntiles) {
+    __shared__ uint32_t h[kMaxPartitions];
+    for (int p = threadIdx.x; p < nparts; p += kT) h[p] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kPartTile;
+#pragma unroll 8
+    for (int i = 0; i < kPartItems; ++i) {
+        const int64_t idx = base + (int64_t)i * kT + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(uint32_t)v[idx] >> kPartBits], 1u);
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < nparts; p += kT) hist[(size_t)p * ntiles + blockIdx.x] = h[p];
+}
+
+// Scatters the tile's endpoints to their partitions' runs (starts = scanned histogram); the
+// order inside a run is arbitrary, which a histogram does not mind.
+__global__ __launch_bounds__(kT) void part_scatter(const int32_t *__restrict__ v, int64_t n, int32_t nparts,
+                                                   const uint32_t *__restrict__ starts, uint32_t ntiles,
+                                                   int32_t *__restrict__ out) {
+    __shared__ uint32_t cur[kMaxPartitions];
+    for (int p = threadIdx.x; p < nparts; p += kT) cur[p] = starts[(size_t)p * ntiles + blockIdx.x];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kPartTile;
+#pragma unroll 8
+    for (int i = 0; i < kPartItems; ++i) {
+        const int64_t idx = base + (int64_t)i * kT + threadIdx.x;
+        if (idx < n) {
+            const int32_t x = v[idx];
+            out[atomicAdd(&cur[(uint32_t)x >> kPartBits], 1u)] = x;
+        }
+    }
+}
+
+// part_start[p] = first position of partition p in the grouped array, part_start[nparts] = n.
+__global__ void part_starts(const uint32_t *__restrict__ scanned, uint32_t ntiles, int32_t nparts, int64_t n,
+                            uint32_t *__restrict__ part_start) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= nparts; p += gridDim.x * blockDim.x)
+        part_start[p] = p < nparts ? scanned[(size_t)p * ntiles] : (uint32_t)n;
+}
+
+// Block b counts the grouped endpoints [b * kCountChunk, (b + 1) * kCountChunk), one partition
+// run at a time, in an LDS histogram, and adds each non-zero bin to deg (sized nparts << kPartBits).
+__global__ __launch_bounds__(kT) void part_count(const int32_t *__restrict__ grouped, int64_t n,
+                                                 const uint32_t *__restrict__ part_start, int32_t nparts,
+                                                 uint32_t *__restrict__ deg) {
+    __shared__ uint32_t h[1 << kPartBits];
+    const int64_t c0 = (int64_t)blockIdx.x * kCountChunk;
+    const int64_t c1 = min(n, c0 + kCountChunk);
+    // first partition whose run reaches past c0 (uniform: every thread computes the same)
+    int32_t lo = 0, hi = nparts;  // invariant: part_start[lo] <= c0 < part_start[hi]
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if ((int64_t)part_start[mid] <= c0) lo = mid;
+        else hi = mid;
+    }
+    for (int32_t p = (int32_t)rtgd::uniform_u32((uint32_t)lo); p < nparts; ++p) {
+        const int64_t s0 = part_start[p], s1 = part_start[p + 1];
+        const int64_t r0 = rtgd::uniform_u64((uint64_t)max(c0, s0)), r1 = rtgd::uniform_u64((uint64_t)min(c1, s1));
+        if (s0 >= c1) break;
+        if (r1 <= r0) continue;
+        for (int j = threadIdx.x; j < (1 << kPartBits); j += kT) h[j] = 0u;
+        __syncthreads();
+        for (int64_t i = r0 + threadIdx.x; i < r1; i += kT) atomicAdd(&h[(uint32_t)grouped[i] & ((1u << kPartBits) - 1u)], 1u);
+        __syncthreads();
+        uint32_t *dst = deg + ((size_t)p << kPartBits);
+        for (int j = threadIdx.x; j < (1 << kPartBits); j += kT) {
+            const uint32_t c = h[j];
+            if (c) atomicAdd(&dst[j], c);
+        }
+        __syncthreads();  // h is cleared for the next run
+    }
+}
+
+// Vertex map of the original id domain: vinfo[x] = {raw degree, new id}, new id = exclusive
+// scan of keep; keep = degree > 1 when orienting (the relabel and the degree<=1 removal
+// of graph.cpp:40-62, 126-152 in one renumbering: both are order-preserving), else degree > 0
+// (relabel only). present counts ids of degree > 0 (the relabelled vertex count).
+__global__ void keep_flags(const uint32_t *__restrict__ deg, int64_t n, uint32_t min_deg, uint32_t *__restrict__ keep,
+                           unsigned long long *present) {
+    uint64_t cnt = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = deg[i];
+        keep[i] = d >= min_deg ? 1u : 0u;
+        cnt += d > 0 ? 1u : 0u;
+    }
+    cnt = rtgd::wave_sum_u64(cnt);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(present, (unsigned long long)cnt);
+}
+
+__global__ void vertex_info(const uint32_t *__restrict__ deg, const uint32_t *__restrict__ newid, int64_t n,
+                            uint2 *__restrict__ vinfo) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        vinfo[i] = make_uint2(deg[i], newid[i]);
 }
 
 // Block-aggregated append: each thread holds up to kPer keys; one exclusive scan over the block
@@ -103,11 +198,12 @@ __device__ __forceinline__ void block_append(const uint64_t (&keys)[kMax], uint3
     __syncthreads();  // wsum and base_s are reused by the next tile
 }
 
-// Orientation, removal and key packing (TCPreprocessing). removes_ex is the exclusive scan of
-// the removal flags, so removes[x] (inclusive, graph.cpp:128-131) = removes_ex[x] for kept x.
+// Orientation, removal and key packing (TCPreprocessing) on the input's own ids: vinfo[x] =
+// {raw degree, final id}. The orientation compares (degree, id) — the relabel preserves order,
+// so the input ids order ties as the relabelled ones do (graph.cpp:116-124) — and the kept
+// endpoints (degree > 1) are renumbered by their final ids (graph.cpp:126-152).
 __global__ __launch_bounds__(kPackThreads) void orient_pack(const int32_t *__restrict__ pairs, int64_t np,
-                                                            const uint32_t *__restrict__ deg,
-                                                            const uint32_t *__restrict__ removes_ex, int bits,
+                                                            const uint2 *__restrict__ vinfo, int bits,
                                                             uint64_t *__restrict__ keys, unsigned long long *counter) {
     const int64_t tile = (int64_t)kPackThreads * kPer;
     for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < np; t0 += (int64_t)gridDim.x * tile) {
@@ -121,15 +217,14 @@ __global__ __launch_bounds__(kPackThreads) void orient_pack(const int32_t *__res
             if (idx < np) {
                 const int2 pr = reinterpret_cast<const int2 *>(pairs)[idx];
                 int32_t s = pr.x, d = pr.y;
-                uint32_t ds = deg[s], dd = deg[d];
-                if (ds > dd || (ds == dd && s > d)) {
-                    int32_t t = s; s = d; d = t;
-                    uint32_t u = ds; ds = dd; dd = u;
+                uint2 is = vinfo[s], id = vinfo[d];
+                if (is.x > id.x || (is.x == id.x && s > d)) {
+                    const uint2 t = is;
+                    is = id;
+                    id = t;
                 }
-                if (ds > 1u && dd > 1u) {
-                    const uint64_t ns = (uint64_t)(uint32_t)(s - (int32_t)removes_ex[s]);
-                    const uint64_t nd = (uint64_t)(uint32_t)(d - (int32_t)removes_ex[d]);
-                    k[i] = (ns << bits) | nd;
+                if (is.x > 1u && id.x > 1u) {
+                    k[i] = ((uint64_t)is.y << bits) | (uint64_t)id.y;
                     mask |= 1u << i;
                 }
             }
@@ -138,8 +233,10 @@ __global__ __launch_bounds__(kPackThreads) void orient_pack(const int32_t *__res
     }
 }
 
-// Symmetric mode: both directions of every non-loop pair.
-__global__ __launch_bounds__(kPackThreads) void symmetric_pack(const int32_t *__restrict__ pairs, int64_t np, int bits,
+// Symmetric mode: both directions of every non-loop pair, ids mapped through newid when given
+// (the relabel), else kept.
+__global__ __launch_bounds__(kPackThreads) void symmetric_pack(const int32_t *__restrict__ pairs, int64_t np,
+                                                               const uint32_t *__restrict__ newid, int bits,
                                                                uint64_t *__restrict__ keys, unsigned long long *counter) {
     const int64_t tile = (int64_t)kPackThreads * kPer;
     for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < np; t0 += (int64_t)gridDim.x * tile) {
@@ -152,8 +249,8 @@ __global__ __launch_bounds__(kPackThreads) void symmetric_pack(const int32_t *__
             k[2 * i] = k[2 * i + 1] = 0;
             if (idx < np) {
                 const int2 pr = reinterpret_cast<const int2 *>(pairs)[idx];
-                const uint64_t s = (uint32_t)pr.x, d = (uint32_t)pr.y;
-                if (s != d) {
+                if (pr.x != pr.y) {
+                    const uint64_t s = newid ? newid[pr.x] : (uint32_t)pr.x, d = newid ? newid[pr.y] : (uint32_t)pr.y;
                     k[2 * i] = (s << bits) | d;
                     k[2 * i + 1] = (d << bits) | s;
                     mask |= 3u << (2 * i);
@@ -318,7 +415,7 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
                                (flags & RTG_PAIRS_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
 
-    // ---- relabel (graph.cpp:40-62) ----
+    // ---- domain, raw degree, renumbering (graph.cpp:40-62, 109-114, 126-152) ----
     MinMax mm{0, -1};
     if (npairs > 0) {
         void *s = nullptr;
@@ -331,36 +428,70 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
         HIP_TRY(hipStreamSynchronize(st));
         if (mm.mn < 0) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: negative vertex id");
     }
-    int64_t nodes = npairs > 0 ? (int64_t)mm.mx + 1 : 0;
-    if (nodes >= ((int64_t)1 << 31) - 1) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: vertex ids too large");
-    const int64_t domain = std::max<int64_t>(nodes, 1);
-    DevBuf vbuf;  // vertex-sized scratch: present/newid, degree, removal flags/scan
-    const size_t vbytes = Arena::round(4 * (size_t)(domain + 1)) * 3 + scan_work_bytes(domain + 1) + 4096;
+    const int64_t domain_ids = npairs > 0 ? (int64_t)mm.mx + 1 : 0;  // the input's id range
+    if (domain_ids >= ((int64_t)1 << 31) - 1) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: vertex ids too large");
+    const int64_t domain = std::max<int64_t>(domain_ids, 1);
+    const bool relabel = (flags & RTG_PAIRS_RELABEL) != 0;
+    const bool need_degree = orient || relabel;  // a symmetric CSR without relabel keeps its ids
+    const int64_t nparts = ceil_div(domain, (int64_t)1 << kPartBits);
+    const bool partitioned = need_degree && npairs > 0 && nparts <= kMaxPartitions;
+    const int64_t deg_len = partitioned ? (nparts << kPartBits) : domain;
+    const uint32_t ntiles = (uint32_t)std::max<int64_t>(1, ceil_div(nv, kPartTile));
+    const int64_t nhist = partitioned ? nparts * (int64_t)ntiles : 1;
+    DevBuf vbuf;  // vertex-sized scratch: degree, keep flags / new ids, vertex info; endpoint grouping
+    const size_t vbytes = Arena::round(4 * (size_t)deg_len) + Arena::round(4 * (size_t)(domain + 1)) * 2 +
+                          Arena::round(8 * (size_t)domain) + scan_work_bytes(std::max<int64_t>(domain + 1, nhist)) +
+                          (partitioned ? Arena::round(4 * (size_t)nv) + Arena::round(4 * (size_t)nhist) +
+                                             Arena::round(4 * (size_t)(nparts + 1))
+                                       : 0) +
+                          8192;
     HIP_TRY(hipMalloc(&vbuf.p, vbytes));
     Arena va{(char *)vbuf.p, 0, vbytes};
-    uint32_t *present = va.take<uint32_t>(domain + 1);
-    uint32_t *deg = va.take<uint32_t>(domain + 1);
-    uint32_t *rmx = va.take<uint32_t>(domain + 1);
-    uint32_t *d_small = va.take<uint32_t>(16);
-    void *vscan = va.take<char>(scan_work_bytes(domain + 1));
-
-    if ((flags & RTG_PAIRS_RELABEL) && npairs > 0) {
-        HIP_TRY(hipMemsetAsync(present, 0, 4 * (size_t)(domain + 1), st));
-        mark_present<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, nv, present);
+    uint32_t *deg = va.take<uint32_t>((size_t)deg_len);
+    uint32_t *keep = va.take<uint32_t>((size_t)domain + 1);
+    uint32_t *newid = va.take<uint32_t>((size_t)domain + 1);
+    uint2 *vinfo = va.take<uint2>((size_t)domain);
+    unsigned long long *d_small = va.take<unsigned long long>(8);  // [0] present ids, [1] kept (u32)
+    void *vscan = va.take<char>(scan_work_bytes(std::max<int64_t>(domain + 1, nhist)));
+    int64_t nodes = domain_ids, n_out = domain_ids;
+    if (need_degree && npairs > 0) {
+        HIP_TRY(hipMemsetAsync(deg, 0, 4 * (size_t)deg_len, st));
+        HIP_TRY(hipMemsetAsync(d_small, 0, sizeof(unsigned long long) * 8, st));
+        if (partitioned) {
+            int32_t *grouped = va.take<int32_t>((size_t)nv);
+            uint32_t *hist = va.take<uint32_t>((size_t)nhist);
+            uint32_t *pstart = va.take<uint32_t>((size_t)nparts + 1);
+            part_hist<<<ntiles, kT, 0, st>>>(d_pairs, nv, (int32_t)nparts, hist, ntiles);
+            HIP_TRY(hipGetLastError());
+            RTG_TRY(scan_exclusive_u32_w(ctx, hist, hist, nhist, nullptr, vscan));
+            part_scatter<<<ntiles, kT, 0, st>>>(d_pairs, nv, (int32_t)nparts, hist, ntiles, grouped);
+            HIP_TRY(hipGetLastError());
+            part_starts<<<(unsigned)ceil_div(nparts + 1, 256), 256, 0, st>>>(hist, ntiles, (int32_t)nparts, nv, pstart);
+            HIP_TRY(hipGetLastError());
+            part_count<<<(unsigned)ceil_div(nv, kCountChunk), kT, 0, st>>>(grouped, nv, pstart, (int32_t)nparts, deg);
+            HIP_TRY(hipGetLastError());
+        } else {
+            raw_degree<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, nv, deg);
+            HIP_TRY(hipGetLastError());
+        }
+        keep_flags<<<grid_for(ctx, domain), kT, 0, st>>>(deg, domain, orient ? 2u : 1u, keep, d_small);
         HIP_TRY(hipGetLastError());
-        RTG_TRY(scan_exclusive_u32_w(ctx, present, present, domain, d_small, vscan));
-        uint32_t cnt = 0;
-        HIP_TRY(hipMemcpyAsync(&cnt, d_small, 4, hipMemcpyDeviceToHost, st));
-        apply_map<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, d_pairs, nv, present);
-        HIP_TRY(hipGetLastError());
+        RTG_TRY(scan_exclusive_u32_w(ctx, keep, newid, domain, reinterpret_cast<uint32_t *>(d_small + 1), vscan));
+        if (orient) {
+            vertex_info<<<grid_for(ctx, domain), kT, 0, st>>>(deg, newid, domain, vinfo);
+            HIP_TRY(hipGetLastError());
+        }
+        unsigned long long hs[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(hs, d_small, sizeof(hs), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        nodes = cnt;
+        const int64_t present = (int64_t)hs[0], kept = (int64_t)(uint32_t)hs[1];
+        nodes = relabel ? present : domain_ids;
+        n_out = orient ? kept : present;
     }
     HIP_TRY(hipEventRecord(ev[1], st));
 
-    // ---- degree, orientation, removal (graph.cpp:104-152) ----
-    int64_t n_out = nodes;
-    int bits = 1;
+    // ---- orientation and key packing (graph.cpp:116-124, 126-152), or both directions ----
+    int bits = bits_for(std::max<int64_t>(n_out, 2));
     DevBuf kbuf;
     const int64_t kcap = std::max<int64_t>(max_keys, 1);
     const size_t kbytes = Arena::round(8 * (size_t)kcap) * 2 + radix_sort_work_bytes(kcap) +
@@ -378,32 +509,13 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
     void *kscan = ka.take<char>(scan_work_bytes(kcap));
 
     HIP_TRY(hipMemsetAsync(counter, 0, sizeof(unsigned long long), st));
-    if (orient) {
-        HIP_TRY(hipMemsetAsync(deg, 0, 4 * (size_t)(domain + 1), st));
-        if (npairs > 0) {
-            raw_degree<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, nv, deg);
-            HIP_TRY(hipGetLastError());
-        }
-        removal_flags<<<grid_for(ctx, nodes), kT, 0, st>>>(deg, (int32_t)nodes, rmx);
+    if (npairs > 0) {
+        const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(npairs, (int64_t)kPackThreads * kPer), (int64_t)ctx->cu_count * 8);
+        if (orient)
+            orient_pack<<<pgrid, kPackThreads, 0, st>>>(d_pairs, npairs, vinfo, bits, keys0, counter);
+        else
+            symmetric_pack<<<pgrid, kPackThreads, 0, st>>>(d_pairs, npairs, relabel ? newid : nullptr, bits, keys0, counter);
         HIP_TRY(hipGetLastError());
-        RTG_TRY(scan_exclusive_u32_w(ctx, rmx, rmx, nodes, d_small, vscan));
-        uint32_t removed = 0;
-        HIP_TRY(hipMemcpyAsync(&removed, d_small, 4, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        n_out = nodes - removed;
-        bits = bits_for(std::max<int64_t>(n_out, 2));
-        if (npairs > 0) {
-            orient_pack<<<(unsigned)std::min<int64_t>(ceil_div(npairs, (int64_t)kPackThreads * kPer), (int64_t)ctx->cu_count * 8),
-                          kPackThreads, 0, st>>>(d_pairs, npairs, deg, rmx, bits, keys0, counter);
-            HIP_TRY(hipGetLastError());
-        }
-    } else {
-        bits = bits_for(std::max<int64_t>(n_out, 2));
-        if (npairs > 0) {
-            symmetric_pack<<<(unsigned)std::min<int64_t>(ceil_div(npairs, (int64_t)kPackThreads * kPer), (int64_t)ctx->cu_count * 8),
-                             kPackThreads, 0, st>>>(d_pairs, npairs, bits, keys0, counter);
-            HIP_TRY(hipGetLastError());
-        }
     }
     unsigned long long nkeys = 0;
     HIP_TRY(hipMemcpyAsync(&nkeys, counter, sizeof(nkeys), hipMemcpyDeviceToHost, st));

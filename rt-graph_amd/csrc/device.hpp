@@ -99,6 +99,10 @@ int scan_exclusive_u64_w(rtg_ctx *ctx, const uint64_t *in, uint64_t *out, int64_
 size_t radix_sort_work_bytes(int64_t n);
 int radix_sort_u64(rtg_ctx *ctx, uint64_t *keys[2], int64_t n, int bits, int *which,
                    int *passes, void *work);
+// The same for 32-bit keys, all 32 bits; is_signed orders them as int32 (the bit patterns are
+// moved, not converted). Same work size.
+int radix_sort_32(rtg_ctx *ctx, uint32_t *keys[2], int64_t n, bool is_signed, int *which,
+                  int *passes, void *work);
 
 }  // namespace rtg
 

@@ -406,6 +406,50 @@ RTG_API int rtg_lookup_i32(rtg_ctx *ctx, const int32_t *d_data, int64_t n, const
     return lookup_api<int32_t>(ctx, d_data, n, d_keys, q, d_out, t_kernel_ms);
 }
 
+namespace {
+// Sorts n 32-bit keys in place on the device: LSD radix sort, four 8-bit passes (even, so the
+// result ends in the caller's buffer); *t_ms gets the device time of the passes.
+int sort32_api(rtg_ctx *ctx, uint32_t *d_data, int64_t n, bool is_signed, double *t_ms) {
+    RTG_TRY(rtg::ctx_enter(ctx));
+    if (t_ms) *t_ms = 0.0;
+    if (n < 0 || (n > 0 && !d_data)) return rtg_fail(RTG_EINVAL, "rtg_sort: bad arguments");
+    if (n >= ((int64_t)1 << 32)) return rtg_fail(RTG_EINVAL, "rtg_sort: more than 2^32 - 1 keys");
+    if (n <= 1) return RTG_OK;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const size_t wb = rtg::radix_sort_work_bytes(n);
+    const size_t bytes = rtg::Arena::round(sizeof(uint32_t) * (size_t)n) + rtg::Arena::round(wb) + 256;
+    void *buf = nullptr;
+    if (hipMalloc(&buf, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return rtg_fail(RTG_ENOMEM, "rtg_sort: temporary of " + std::to_string(bytes) + " bytes failed");
+    }
+    rtg::Arena ar{(char *)buf, 0, bytes};
+    uint32_t *keys[2] = {d_data, ar.take<uint32_t>((size_t)n)};
+    void *work = ar.take<char>(wb);
+    int which = 0, passes = 0, rc = RTG_OK;
+    hipStream_t st = ctx->stream;
+    if (hipEventRecord(ctx->ev[0], st) != hipSuccess) rc = rtg_fail(RTG_EHIP, "rtg_sort: event record failed");
+    if (rc == RTG_OK) rc = rtg::radix_sort_32(ctx, keys, n, is_signed, &which, &passes, work);
+    if (rc == RTG_OK && which != 0 &&
+        hipMemcpyAsync(d_data, keys[1], sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        rc = rtg_fail(RTG_EHIP, "rtg_sort: copy back failed");
+    if (rc == RTG_OK && (hipEventRecord(ctx->ev[1], st) != hipSuccess || hipEventSynchronize(ctx->ev[1]) != hipSuccess))
+        rc = rtg_fail(RTG_EHIP, "rtg_sort: synchronisation failed");
+    if (rc == RTG_OK && t_ms) *t_ms = rtg::elapsed_ms(ctx->ev[0], ctx->ev[1]);
+    (void)hipStreamSynchronize(st);  // the passes may still read the temporary after a failure
+    (void)hipFree(buf);
+    return rc;
+}
+}  // namespace
+
+RTG_API int rtg_sort_u32(rtg_ctx *ctx, uint32_t *d_data, int64_t n, double *t_ms) {
+    return sort32_api(ctx, d_data, n, false, t_ms);
+}
+
+RTG_API int rtg_sort_i32(rtg_ctx *ctx, int32_t *d_data, int64_t n, double *t_ms) {
+    return sort32_api(ctx, reinterpret_cast<uint32_t *>(d_data), n, true, t_ms);
+}
+
 RTG_API int rtg_lookup_index_create_u32(rtg_ctx *ctx, const uint32_t *d_data, int64_t n, rtg_lookup_index **out) {
     if (!out) return rtg_fail(RTG_EINVAL, "rtg_lookup_index_create: null out");
     *out = nullptr;

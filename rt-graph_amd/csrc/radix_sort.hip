@@ -1,13 +1,15 @@
 // Stable LSD radix sort of uint64 keys, 8-bit digits, for the CSR build (sorting the packed
 // (src << b | dst) edge keys, the role thrust::sort plays in ThrustSortEdgesInBatch,
-// tc/graph/thrust_helper.cu:16-42).
+// tc/graph/thrust_helper.cu:16-42), and of 32-bit keys, unsigned or signed (the device sort of
+// the lookup array, thrust::sort in BSAccessCountFunction, rt_search/simple_bs.cu:99-108; a
+// signed key's top digit has its sign bit flipped, so negative values sort first).
 //
 // Per pass: (1) every 4096-key tile writes its 256-bin digit histogram, digit-major;
 // (2) one exclusive scan over the histograms gives each (digit, tile) its global start;
 // (3) every tile ranks its keys stably — each wave64 owns 1024 consecutive keys, peers with
 // the same digit are found with eight ballots, and per-wave LDS counters carry the running
 // count — then reorders the tile in LDS by digit and writes it out in digit runs, so that
-// stores are mostly contiguous. Traffic per pass: 2 reads + 1 write of 8 bytes per key.
+// stores are mostly contiguous. Traffic per pass: 2 reads + 1 write of the key size per key.
 #include "device.hpp"
 
 namespace {
@@ -18,8 +20,10 @@ constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;  // 4096 keys
 constexpr int kBins = 256;
 
-__global__ __launch_bounds__(kThreads) void digit_histogram(const uint64_t *__restrict__ in,
-                                                            int64_t n, int shift,
+// dxor: XORed into every digit of the pass (0x80 on a signed key's top digit, else 0).
+template <class K>
+__global__ __launch_bounds__(kThreads) void digit_histogram(const K *__restrict__ in,
+                                                            int64_t n, int shift, uint32_t dxor,
                                                             uint32_t *__restrict__ hist,
                                                             uint32_t ntiles) {
     __shared__ uint32_t h[kBins];
@@ -29,22 +33,23 @@ __global__ __launch_bounds__(kThreads) void digit_histogram(const uint64_t *__re
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         int64_t idx = base + (int64_t)i * kThreads + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(uint32_t)(in[idx] >> shift) & 0xFFu], 1u);
+        if (idx < n) atomicAdd(&h[((uint32_t)(in[idx] >> shift) & 0xFFu) ^ dxor], 1u);
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(kThreads) void rank_scatter(const uint64_t *__restrict__ in,
-                                                         uint64_t *__restrict__ out, int64_t n,
-                                                         int shift,
+template <class K>
+__global__ __launch_bounds__(kThreads) void rank_scatter(const K *__restrict__ in,
+                                                         K *__restrict__ out, int64_t n,
+                                                         int shift, uint32_t dxor,
                                                          const uint32_t *__restrict__ starts,
                                                          uint32_t ntiles) {
     __shared__ uint32_t wave_cnt[kWaves][kBins];
     __shared__ uint32_t digit_start[kBins];
     __shared__ uint32_t global_minus_local[kBins];
     __shared__ uint32_t wsum[kWaves];
-    __shared__ uint64_t tile_keys[kTile];
+    __shared__ K tile_keys[kTile];
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -54,14 +59,14 @@ __global__ __launch_bounds__(kThreads) void rank_scatter(const uint64_t *__restr
     const int64_t tile_base = (int64_t)blockIdx.x * kTile;
     const int64_t wave_base = tile_base + (int64_t)wave * (64 * kItems);
     const uint64_t lt = rtgd::lanemask_lt();
-    uint64_t key[kItems];
+    K key[kItems];
     uint32_t rank[kItems];
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
         const int64_t idx = wave_base + (int64_t)i * 64 + lane;
         const bool valid = idx < n;
-        const uint64_t k = valid ? in[idx] : 0ull;
-        const uint32_t dg = (uint32_t)(k >> shift) & 0xFFu;
+        const K k = valid ? in[idx] : (K)0;
+        const uint32_t dg = ((uint32_t)(k >> shift) & 0xFFu) ^ dxor;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -106,15 +111,15 @@ __global__ __launch_bounds__(kThreads) void rank_scatter(const uint64_t *__restr
     for (int i = 0; i < kItems; ++i) {
         const int64_t idx = wave_base + (int64_t)i * 64 + lane;
         if (idx < n) {
-            const uint32_t dg = (uint32_t)(key[i] >> shift) & 0xFFu;
+            const uint32_t dg = ((uint32_t)(key[i] >> shift) & 0xFFu) ^ dxor;
             tile_keys[digit_start[dg] + wave_cnt[wave][dg] + rank[i]] = key[i];
         }
     }
     __syncthreads();
     const int64_t count = (n - tile_base) < kTile ? (n - tile_base) : kTile;
     for (int j = threadIdx.x; j < count; j += kThreads) {
-        const uint64_t k = tile_keys[j];
-        const uint32_t dg = (uint32_t)(k >> shift) & 0xFFu;
+        const K k = tile_keys[j];
+        const uint32_t dg = ((uint32_t)(k >> shift) & 0xFFu) ^ dxor;
         out[(size_t)(global_minus_local[dg] + (uint32_t)j)] = k;
     }
 }
@@ -129,30 +134,42 @@ size_t radix_sort_work_bytes(int64_t n) {
     return Arena::round(sizeof(uint32_t) * (size_t)h) + scan_work_bytes(h) + 256;
 }
 
-int radix_sort_u64(rtg_ctx *ctx, uint64_t *keys[2], int64_t n, int bits, int *which,
-                   int *passes, void *work) {
+template <class K>
+int radix_sort_t(rtg_ctx *ctx, K *keys[2], int64_t n, int bits, bool sign_top, int *which, int *passes,
+                 void *work) {
     *which = 0;
     if (passes) *passes = 0;
     if (n <= 1 || bits <= 0) return RTG_OK;
-    if (n >= (int64_t)1 << 32) return set_error(RTG_EINVAL, "radix_sort_u64: more than 2^32 keys");
+    if (n >= (int64_t)1 << 32) return set_error(RTG_EINVAL, "radix sort: more than 2^32 keys");
     const int64_t tiles = ceil_div(n, kTile);
     const int64_t h = tiles * kBins;
     uint32_t *hist = reinterpret_cast<uint32_t *>(work);
     char *scan_work = reinterpret_cast<char *>(work) + Arena::round(sizeof(uint32_t) * (size_t)h);
     int cur = 0, np = 0;
     for (int shift = 0; shift < bits; shift += 8, ++np) {
-        digit_histogram<<<(unsigned)tiles, kThreads, 0, ctx->stream>>>(keys[cur], n, shift, hist,
-                                                                       (uint32_t)tiles);
+        const uint32_t dxor = (sign_top && shift + 8 >= bits) ? 0x80u : 0u;
+        digit_histogram<K><<<(unsigned)tiles, kThreads, 0, ctx->stream>>>(keys[cur], n, shift, dxor, hist,
+                                                                          (uint32_t)tiles);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u32_w(ctx, hist, hist, h, nullptr, scan_work));
-        rank_scatter<<<(unsigned)tiles, kThreads, 0, ctx->stream>>>(keys[cur], keys[cur ^ 1], n,
-                                                                    shift, hist, (uint32_t)tiles);
+        rank_scatter<K><<<(unsigned)tiles, kThreads, 0, ctx->stream>>>(keys[cur], keys[cur ^ 1], n, shift, dxor,
+                                                                       hist, (uint32_t)tiles);
         HIP_TRY(hipGetLastError());
         cur ^= 1;
     }
     *which = cur;
     if (passes) *passes = np;
     return RTG_OK;
+}
+
+int radix_sort_u64(rtg_ctx *ctx, uint64_t *keys[2], int64_t n, int bits, int *which,
+                   int *passes, void *work) {
+    return radix_sort_t<uint64_t>(ctx, keys, n, bits, false, which, passes, work);
+}
+
+int radix_sort_32(rtg_ctx *ctx, uint32_t *keys[2], int64_t n, bool is_signed, int *which, int *passes,
+                  void *work) {
+    return radix_sort_t<uint32_t>(ctx, keys, n, 32, is_signed, which, passes, work);
 }
 
 }  // namespace rtg

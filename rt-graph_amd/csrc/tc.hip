@@ -164,11 +164,16 @@ __host__ __device__ __forceinline__ uint32_t cuckoo_half(int32_t d) {
 
 __device__ __forceinline__ uint32_t hmul(uint32_t x, uint32_t k, uint32_t t) { return __umulhi(x * k, t); }
 
+// Probe modes of an owner table: two-choice cuckoo hash, linear probing (the hash's fallback),
+// a bitmap over the owner's value range, the owner's sorted list.
+constexpr uint32_t kCuckoo = 0, kLinearProbe = 1, kBitmap = 2, kSorted = 3;
+
 struct Table {
     uint32_t *tab;
-    uint32_t t;      // slots per half (cuckoo) / half the slots (linear)
-    uint32_t k1, k2;
-    uint32_t linear; // 0: cuckoo, 1: linear probing over 2t slots
+    uint32_t t;      // cuckoo: slots per half; linear: half the slots; bitmap: value range
+                     // (bits); sorted: list length
+    uint32_t k1, k2; // hash multipliers; bitmap: k1 = the owner's smallest value
+    uint32_t mode;   // kCuckoo, kLinearProbe, kBitmap, kSorted
 };
 
 __device__ __forceinline__ bool cuckoo_insert(const Table &c, uint32_t x) {
@@ -200,6 +205,27 @@ __device__ __forceinline__ uint32_t cuckoo_has(const Table &c, uint32_t x) {
     return (uint32_t)(a == x) | (uint32_t)(b == x);
 }
 
+__device__ __forceinline__ uint32_t bitmap_has(const Table &c, uint32_t x) {
+    const uint32_t o = x - c.k1;
+    return o < c.t ? (c.tab[o >> 5] >> (o & 31u)) & 1u : 0u;
+}
+
+// First index of tab[lo, lo + len) whose value is not below x (lo + len if none): binary lifting
+// whose trip count depends on len only, so the lanes of a wave stay together.
+__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *tab, uint32_t lo, uint32_t len, uint32_t x) {
+    uint32_t pos = lo;
+    const uint32_t end = lo + len;
+    for (uint32_t step = len ? 1u << (31 - __clz(len)) : 0u; step; step >>= 1)
+        if (pos + step <= end && tab[pos + step - 1] < x) pos += step;
+    return pos;
+}
+
+// x in the sorted list tab[0, t) (t >= 1).
+__device__ __forceinline__ uint32_t sorted_has(const Table &c, uint32_t x) {
+    const uint32_t p = lds_lower_bound(c.tab, 0, c.t, x);
+    return (p < c.t && c.tab[min(p, c.t - 1)] == x) ? 1u : 0u;
+}
+
 __device__ __forceinline__ uint32_t linear_has(const Table &c, uint32_t x) {
     const uint32_t size = 2u * c.t;
     uint32_t h = hmul(x, c.k1, size);
@@ -221,7 +247,7 @@ __device__ __forceinline__ void table_seed(Table &c, int r) {
 template <int kThreads>
 __device__ Table build_block_table(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d,
                                    uint32_t *fail_flag) {
-    Table c{tab, cuckoo_half(d), 0, 0, 0};
+    Table c{tab, cuckoo_half(d), 0, 0, kCuckoo};
     for (int r = 0; r < kRetries; ++r) {
         table_seed(c, r);
         for (uint32_t i = threadIdx.x; i < 2u * c.t; i += kThreads) tab[i] = kEmpty;
@@ -234,7 +260,7 @@ __device__ Table build_block_table(uint32_t *tab, const int32_t *__restrict__ ad
         if (rtgd::uniform_u32(*fail_flag) == 0u) return c;
         __syncthreads();  // everyone has read the flag before it is reset
     }
-    c.linear = 1u;
+    c.mode = kLinearProbe;
     for (uint32_t i = threadIdx.x; i < 2u * c.t; i += kThreads) tab[i] = kEmpty;
     __syncthreads();
     for (int32_t i = threadIdx.x; i < d; i += kThreads) linear_insert(c, (uint32_t)adj[xb + i]);
@@ -242,10 +268,35 @@ __device__ Table build_block_table(uint32_t *tab, const int32_t *__restrict__ ad
     return c;
 }
 
+// Block-wide bitmap of adj[xb .. xb+d) over its value range [adj[xb], adj[xb+d-1]] (the caller
+// has checked that the range's words fit the table).
+template <int kThreads>
+__device__ Table build_block_bitmap(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d) {
+    const uint32_t lo = (uint32_t)adj[xb], hi = (uint32_t)adj[xb + d - 1];
+    const Table c{tab, hi - lo + 1u, lo, 0u, kBitmap};
+    const uint32_t words = ((hi - lo) >> 5) + 1u;
+    for (uint32_t i = threadIdx.x; i < words; i += kThreads) tab[i] = 0u;
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < d; i += kThreads) {
+        const uint32_t o = (uint32_t)adj[xb + i] - lo;
+        atomicOr(&tab[o >> 5], 1u << (o & 31u));
+    }
+    __syncthreads();
+    return c;
+}
+
+// Block-wide copy of the sorted list adj[xb .. xb+d) (d fits the table).
+template <int kThreads>
+__device__ Table build_block_sorted(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d) {
+    for (int32_t i = threadIdx.x; i < d; i += kThreads) tab[i] = (uint32_t)adj[xb + i];
+    __syncthreads();
+    return Table{tab, (uint32_t)d, 0u, 0u, kSorted};
+}
+
 // Wave-private build (small owners).
 __device__ Table build_wave_table(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d) {
     const int lane = __lane_id();
-    Table c{tab, cuckoo_half(d), 0, 0, 0};
+    Table c{tab, cuckoo_half(d), 0, 0, kCuckoo};
     for (int r = 0; r < kRetries; ++r) {
         table_seed(c, r);
         for (uint32_t i = lane; i < 2u * c.t; i += 64) tab[i] = kEmpty;
@@ -255,7 +306,7 @@ __device__ Table build_wave_table(uint32_t *tab, const int32_t *__restrict__ adj
         rtgd::wave_sync_lds();
         if (__ballot(!ok) == 0ull) return c;
     }
-    c.linear = 1u;
+    c.mode = kLinearProbe;
     for (uint32_t i = lane; i < 2u * c.t; i += 64) tab[i] = kEmpty;
     rtgd::wave_sync_lds();
     for (int32_t i = lane; i < d; i += 64) linear_insert(c, (uint32_t)adj[xb + i]);
@@ -263,12 +314,37 @@ __device__ Table build_wave_table(uint32_t *tab, const int32_t *__restrict__ adj
     return c;
 }
 
-// Probe modes: the owner's list in a cuckoo table or in a linear-probing table.
-constexpr int kCuckoo = 0, kLinearProbe = 1;
+// Wave-private bitmap or sorted copy (the caller has checked that it fits `cap` words; else it
+// builds the hash table).
+__device__ Table build_wave_other(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d, uint32_t mode,
+                                  uint32_t cap) {
+    const int lane = __lane_id();
+    if (mode == kBitmap) {
+        const uint32_t lo = (uint32_t)adj[xb], hi = (uint32_t)adj[xb + d - 1];
+        const uint32_t words = ((hi - lo) >> 5) + 1u;
+        if (words <= cap) {
+            for (uint32_t i = lane; i < words; i += 64) tab[i] = 0u;
+            rtgd::wave_sync_lds();
+            for (int32_t i = lane; i < d; i += 64) {
+                const uint32_t o = (uint32_t)adj[xb + i] - lo;
+                atomicOr(&tab[o >> 5], 1u << (o & 31u));
+            }
+            rtgd::wave_sync_lds();
+            return Table{tab, hi - lo + 1u, lo, 0u, kBitmap};
+        }
+    } else if (mode == kSorted && (uint32_t)d <= cap) {
+        for (int32_t i = lane; i < d; i += 64) tab[i] = (uint32_t)adj[xb + i];
+        rtgd::wave_sync_lds();
+        return Table{tab, (uint32_t)d, 0u, 0u, kSorted};
+    }
+    return build_wave_table(tab, adj, xb, d);
+}
 
-template <int kMode>
+template <uint32_t kMode>
 __device__ __forceinline__ uint32_t probe(const Table &c, uint32_t x) {
     if constexpr (kMode == kLinearProbe) return linear_has(c, x);
+    else if constexpr (kMode == kBitmap) return bitmap_has(c, x);
+    else if constexpr (kMode == kSorted) return sorted_has(c, x);
     else return cuckoo_has(c, x);
 }
 
@@ -312,7 +388,11 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
 // cursor, so that lane j of a round handles element j of the concatenation.
 // kPerEdge: every span's count is also stored at eout[wedge[pi]] — a long span's by a wave
 // sum, a concatenated short span's through its LDS counter pe[ci].
-template <int kMode, int U, bool kPerEdge>
+// kSorted (the owner's sorted list in LDS) merges a long span with the owner's list: both
+// ascend, so a round of 64*U consecutive elements can only match the owner-list window between
+// the lower bound of its first element and the upper bound of its last (two searches the wave
+// shares), each lane searches only that window, and the next round starts where it ended.
+template <uint32_t kMode, int U, bool kPerEdge>
 __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                  int64_t p0, int64_t p1, const Table &c, int32_t long_min,
                                                  int32_t *ex, int32_t *st, const uint32_t *__restrict__ wedge,
@@ -339,18 +419,39 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<int32_t *>(adj + s0), (short)0, l0 * 4, kBufferWord3);
             int32_t jb = 0;
+            uint32_t w0 = 0;  // kSorted: owner-list position the span has reached
             for (; jb < l0; jb += 64 * U) {
                 const int vo = (jb + lane) * 4;
                 uint32_t x[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) x[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, vo + u * 256, 0, 0);
+                if constexpr (kMode == kSorted) {
+                    const int32_t last = min(64 * U, l0 - jb) - 1;  // the round's last element
+                    uint32_t xl = x[0];
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if constexpr (kPerEdge) sh += probe<kMode>(c, x[u]);
-                    else hits += probe<kMode>(c, x[u]);
+                    for (int u = 1; u < U; ++u)
+                        if ((last >> 6) == u) xl = x[u];
+                    const uint32_t first_v = __builtin_amdgcn_readfirstlane(x[0]);
+                    const uint32_t last_v = __builtin_amdgcn_readlane(xl, last & 63);
+                    const uint32_t wa = rtgd::uniform_u32(lds_lower_bound(c.tab, w0, c.t - w0, first_v));
+                    const uint32_t wb = rtgd::uniform_u32(lds_lower_bound(c.tab, wa, c.t - wa, last_v + 1u));
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const uint32_t q = lds_lower_bound(c.tab, wa, wb - wa, x[u]);
+                        const uint32_t h = (jb + u * 64 + lane < l0 && q < wb && c.tab[min(q, c.t - 1)] == x[u]) ? 1u : 0u;
+                        if constexpr (kPerEdge) sh += h;
+                        else hits += h;
+                    }
+                    w0 = wb;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if constexpr (kPerEdge) sh += probe<kMode>(c, x[u]);
+                        else hits += probe<kMode>(c, x[u]);
+                    }
                 }
             }
-            {
+            if constexpr (kMode != kSorted) {
                 // This lane's loads past the end in the last round each probed 0 and hit iff
                 // the owner's list holds 0 (has0): take them back.
                 const int32_t r = l0 - (jb - 64 * U) - lane;
@@ -414,6 +515,35 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
     return hits;
 }
 
+// The bitmap and sorted-list strategies out of line: the hash path's code stays as compact as
+// before (instruction cache), and these run only for the buckets the planner gave them.
+template <uint32_t kMode, int U, bool kPerEdge>
+__device__ __noinline__ uint32_t stream_spans_ool(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
+                                                  int64_t p0, int64_t p1, Table c, int32_t long_min, int32_t *ex,
+                                                  int32_t *st, const uint32_t *__restrict__ wedge,
+                                                  uint32_t *__restrict__ eout, uint32_t *pe) {
+    return stream_spans<kMode, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+}
+
+// Partners [p0, p1) against the owner table c, in the probe mode the table was built in. kAll:
+// the bitmap and sorted-list modes are compiled in (else only the hash modes exist).
+template <int U, bool kPerEdge, bool kAll>
+__device__ __forceinline__ uint32_t stream_table(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
+                                                 int64_t p0, int64_t p1, const Table &c, int32_t long_min, int32_t *ex,
+                                                 int32_t *st, const uint32_t *__restrict__ wedge,
+                                                 uint32_t *__restrict__ eout, uint32_t *pe) {
+    const uint32_t mode = rtgd::uniform_u32(c.mode);
+    if (mode == kCuckoo) return stream_spans<kCuckoo, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+    if (!kAll || mode == kLinearProbe)
+        return stream_spans<kLinearProbe, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+    if constexpr (kAll) {
+        if (mode == kBitmap)
+            return stream_spans_ool<kBitmap, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+        return stream_spans_ool<kSorted, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+    }
+    return 0u;
+}
+
 struct BatchArgs {
     const int32_t *off;
     const int32_t *adj;
@@ -434,8 +564,12 @@ struct BatchArgs {
 // The per-edge instantiation is held to 8 waves per SIMD (at most 64 VGPRs, a few spilled to
 // scratch outside the streaming loops): left alone it takes 74 and runs 6 waves, which costs
 // more than the spills (R-MAT scale 21: 23.7 against 25.3 ms).
-template <int kThreads, int U, bool kPerEdge>
-__global__ __launch_bounds__(kThreads, kPerEdge ? 8 : 1) void tc_batched(BatchArgs a) {
+// kAll: every strategy compiled in. Plans whose items are all hash, search or hash wave items
+// run the instantiation without the bitmap and sorted-list code (the out-of-line calls would
+// hold more registers across the hash path and cost it occupancy); there, bitmap and sorted
+// items would be searched and wave items hashed, which gives the same counts.
+template <int kThreads, int U, bool kPerEdge, bool kAll>
+__global__ __launch_bounds__(kThreads, (kPerEdge || kAll) ? 8 : 1) void tc_batched(BatchArgs a) {
     constexpr int kWaves = kThreads / 64;
     extern __shared__ uint32_t table[];
     __shared__ int32_t ex_all[kWaves][65];
@@ -462,11 +596,22 @@ __global__ __launch_bounds__(kThreads, kPerEdge ? 8 : 1) void tc_batched(BatchAr
     uint64_t it_idx = rtgd::uniform_u64(item_slot);
     while (it_idx < (uint64_t)a.n_items) {
         const TcItem it = a.items[it_idx];
-        if (it.kind == 0) {
-            // Block item: owner it.a, partners [it.b, it.c).
-            const int32_t x = it.a;
+        uint32_t kind = rtgd::uniform_u32((uint32_t)it.kind);
+        const int32_t x = it.a;
+        if (kind == kItemBitmap || kind == kItemSorted) {
+            // The planner only gives these strategies to owners whose bitmap or list fits the
+            // table; anything else is searched in HBM rather than written past it.
             const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
-            const Table c = build_block_table<kThreads>(table, a.adj, xb, d, &fail_flag);
+            const uint32_t need = kind == kItemSorted ? (uint32_t)d : (((uint32_t)(a.adj[xb + d - 1] - a.adj[xb])) >> 5) + 1u;
+            if (!kAll || rtgd::uniform_u32(need) > (uint32_t)a.max_table) kind = kItemSearch;
+        }
+        if (kind == kItemHash || kind == kItemBitmap || kind == kItemSorted) {
+            // Block item: owner it.a staged in the block's table, partners [it.b, it.c).
+            const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
+            Table c;
+            if (!kAll || kind == kItemHash) c = build_block_table<kThreads>(table, a.adj, xb, d, &fail_flag);
+            else if (kind == kItemBitmap) c = build_block_bitmap<kThreads>(table, a.adj, xb, d);
+            else c = build_block_sorted<kThreads>(table, a.adj, xb, d);
             while (true) {
                 int32_t g = 0;
                 if (lane == 0) g = atomicAdd(&next_batch, a.grab);
@@ -474,17 +619,11 @@ __global__ __launch_bounds__(kThreads, kPerEdge ? 8 : 1) void tc_batched(BatchAr
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
-                if (c.linear)
-                    hits += stream_spans<kLinearProbe, U, kPerEdge>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
-                                                                    a.eout, pe);
-                else
-                    hits += stream_spans<kCuckoo, U, kPerEdge>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
-                                                               a.eout, pe);
+                hits += stream_table<U, kPerEdge, kAll>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge, a.eout, pe);
             }
-        } else if (it.kind == 2) {
-            // Search item: an owner whose list is too long for an LDS table; partners
-            // [it.b, it.c) are binary-searched in its sorted list in HBM.
-            const int32_t x = it.a;
+        } else if (kind == kItemSearch) {
+            // Search item: partners [it.b, it.c) are binary-searched in the owner's sorted
+            // list in HBM (owners too long for an LDS table, or with too little reuse).
             const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
             while (true) {
                 int32_t g = 0;
@@ -496,18 +635,17 @@ __global__ __launch_bounds__(kThreads, kPerEdge ? 8 : 1) void tc_batched(BatchAr
                 hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d, a.wedge, a.eout);
             }
         } else {
-            // Wave items: small owners small_list[it.b, it.c), one per wave at a time.
+            // Wave items: small owners small_list[it.b, it.c), one per wave at a time, each in a
+            // wave-private table of the group's strategy it.a (kCuckoo, kBitmap or kSorted).
             uint32_t *tab = table + wave * a.wave_table;
+            const uint32_t mode = kAll ? rtgd::uniform_u32((uint32_t)it.a) : kCuckoo;
             for (int32_t s = it.b + wave; s < it.c; s += kWaves) {
-                const int32_t x = a.small_list[s];
-                const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
-                const Table c = build_wave_table(tab, a.adj, xb, d);
-                if (c.linear)
-                    hits += stream_spans<kLinearProbe, U, kPerEdge>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min,
-                                                                    ex, st, a.wedge, a.eout, pe);
-                else
-                    hits += stream_spans<kCuckoo, U, kPerEdge>(a.adj, a.wspan, a.wofs[x], a.wofs[x + 1], c, a.long_min, ex,
-                                                               st, a.wedge, a.eout, pe);
+                const int32_t xs = a.small_list[s];
+                const int32_t xb = a.off[xs], d = a.off[xs + 1] - xb;
+                const Table c = mode == kCuckoo ? build_wave_table(tab, a.adj, xb, d)
+                                                : build_wave_other(tab, a.adj, xb, d, mode, (uint32_t)a.wave_table);
+                hits += stream_table<U, kPerEdge, kAll>(a.adj, a.wspan, a.wofs[xs], a.wofs[xs + 1], c, a.long_min, ex,
+                                                        st, a.wedge, a.eout, pe);
                 rtgd::wave_sync_lds();
             }
         }
@@ -525,20 +663,22 @@ __global__ __launch_bounds__(kThreads, kPerEdge ? 8 : 1) void tc_batched(BatchAr
 
 using BatchKernel = void (*)(BatchArgs);
 
-// Instantiated (block size, unroll) pairs of the batched kernel, counting or per-edge.
-template <bool kPerEdge>
+// Instantiated (block size, unroll) pairs of the batched kernel, counting or per-edge, with the
+// hash strategies only or with every strategy.
+template <bool kPerEdge, bool kAll>
 BatchKernel batched_kernel_t(int threads, int unroll) {
-    if (threads == 256 && unroll == 2) return tc_batched<256, 2, kPerEdge>;
-    if (threads == 256 && unroll == 4) return tc_batched<256, 4, kPerEdge>;
-    if (threads == 256 && unroll == 8) return tc_batched<256, 8, kPerEdge>;
-    if (threads == 512 && unroll == 2) return tc_batched<512, 2, kPerEdge>;
-    if (threads == 512 && unroll == 4) return tc_batched<512, 4, kPerEdge>;
-    if (threads == 512 && unroll == 8) return tc_batched<512, 8, kPerEdge>;
+    if (threads == 256 && unroll == 2) return tc_batched<256, 2, kPerEdge, kAll>;
+    if (threads == 256 && unroll == 4) return tc_batched<256, 4, kPerEdge, kAll>;
+    if (threads == 256 && unroll == 8) return tc_batched<256, 8, kPerEdge, kAll>;
+    if (threads == 512 && unroll == 2) return tc_batched<512, 2, kPerEdge, kAll>;
+    if (threads == 512 && unroll == 4) return tc_batched<512, 4, kPerEdge, kAll>;
+    if (threads == 512 && unroll == 8) return tc_batched<512, 8, kPerEdge, kAll>;
     return nullptr;
 }
 
-BatchKernel batched_kernel(int threads, int unroll, bool per_edge = false) {
-    return per_edge ? batched_kernel_t<true>(threads, unroll) : batched_kernel_t<false>(threads, unroll);
+BatchKernel batched_kernel(int threads, int unroll, bool per_edge, bool all_strategies) {
+    if (per_edge) return all_strategies ? batched_kernel_t<true, true>(threads, unroll) : batched_kernel_t<true, false>(threads, unroll);
+    return all_strategies ? batched_kernel_t<false, true>(threads, unroll) : batched_kernel_t<false, false>(threads, unroll);
 }
 
 void launch_batched(BatchKernel k, int grid, int threads, size_t lds, hipStream_t st, const BatchArgs &a) {
@@ -652,8 +792,14 @@ int dev_alloc(T **p, size_t count) {
 }
 
 // Tunables of the batched plan, written as a comma list of key=value (keys threads, unroll,
-// item, wave_item, wave_max, long, grab); chosen by the autotuner, read from the tuning cache or
-// set by the RTG_TC_TUNE environment variable.
+// item, wave_item, wave_max, long, grab, table_cap, strat); chosen by the autotuner, read from
+// the tuning cache or set by the RTG_TC_TUNE environment variable. `strat` holds one letter per
+// owner bucket (h hash, b bitmap, m merge with the sorted list, s search in HBM).
+constexpr int kBuckets = 4;
+constexpr int kStratHash = 0, kStratSearch = 1, kStratBitmap = 2, kStratMerge = 3;
+constexpr char kStratLetter[] = {'h', 's', 'b', 'm'};
+const char *const kStratName[] = {"hash", "search", "bitmap", "merge"};
+
 struct BatchTune {
     int32_t block_threads = 256;
     int32_t unroll = 4;
@@ -663,13 +809,21 @@ struct BatchTune {
     int32_t long_min = 128;
     int32_t grab = 64;
     int32_t table_cap = 0;         // LDS table slots per block (0: what a workgroup can hold)
+    int32_t strat[kBuckets] = {kStratHash, kStratHash, kStratHash, kStratSearch};
 };
+
+std::string strat_string(const int32_t *strat) {
+    std::string s;
+    for (int b = 0; b < kBuckets; ++b) s.push_back(kStratLetter[strat[b]]);
+    return s;
+}
 
 std::string tune_string(const BatchTune &t) {
     return "threads=" + std::to_string(t.block_threads) + ",unroll=" + std::to_string(t.unroll) +
            ",item=" + std::to_string(t.item_work) + ",wave_item=" + std::to_string(t.wave_item_work) +
            ",wave_max=" + std::to_string(t.wave_owner_max) + ",long=" + std::to_string(t.long_min) +
-           ",grab=" + std::to_string(t.grab) + (t.table_cap > 0 ? ",table_cap=" + std::to_string(t.table_cap) : "");
+           ",grab=" + std::to_string(t.grab) + (t.table_cap > 0 ? ",table_cap=" + std::to_string(t.table_cap) : "") +
+           ",strat=" + strat_string(t.strat);
 }
 
 void tune_from_string(BatchTune &t, const std::string &s) {
@@ -681,7 +835,8 @@ void tune_from_string(BatchTune &t, const std::string &s) {
         const size_t eq = kv.find('=');
         if (eq != std::string::npos) {
             const std::string k = kv.substr(0, eq);
-            const long long v = std::atoll(kv.c_str() + eq + 1);
+            const std::string sv = kv.substr(eq + 1);
+            const long long v = std::atoll(sv.c_str());
             if (k == "threads") t.block_threads = (int32_t)v;
             else if (k == "unroll") t.unroll = (int32_t)v;
             else if (k == "item") t.item_work = std::max<long long>(1, v);
@@ -690,17 +845,45 @@ void tune_from_string(BatchTune &t, const std::string &s) {
             else if (k == "long") t.long_min = (int32_t)std::max<long long>(1, v);
             else if (k == "grab") t.grab = (int32_t)std::max<long long>(1, std::min<long long>(64, v));
             else if (k == "table_cap") t.table_cap = (int32_t)std::max<long long>(0, std::min<long long>(kMaxTableSlots, v));
+            else if (k == "strat")
+                for (int b = 0; b < kBuckets && b < (int)sv.size(); ++b)
+                    for (int c = 0; c < 4; ++c)
+                        if (sv[(size_t)b] == kStratLetter[c]) t.strat[b] = c;
         }
         pos = e + 1;
     }
 }
 
-// Graph-level part of the batched plan: owner assignment, partner spans grouped by owner, the
-// byte models, and host copies of the per-owner counts that item building needs.
-// Groups the range's work edges by owner without atomics: owner keys, a radix sort on the
-// owner bits (stable, so each owner's partners stay in edge order), spans and run boundaries,
-// then per-owner counts, work and partner offsets; the host gets row offsets, partner offsets
-// and work for the queue.
+// ---- device planning: partner offsets, validation, fingerprint ----
+
+// flag[0] = 1 unless wofs[0..n] is a partition of the nw partner spans (starts at 0, ends at nw,
+// never decreases): the counting kernel reads spans through these offsets.
+__global__ void plan_check_offsets(const int32_t *__restrict__ wofs, int32_t n, int64_t nw, uint32_t *flag) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
+        const bool bad = wofs[x] > wofs[x + 1] || (x == 0 && wofs[0] != 0) || (x == n - 1 && (int64_t)wofs[n] != nw);
+        if (bad) atomicOr(flag, 1u);
+    }
+}
+
+// A fingerprint of the row offsets (index-mixed, so order matters) for the tuning cache key.
+__global__ void offsets_fingerprint(const int32_t *__restrict__ off, int32_t n, unsigned long long *h) {
+    uint64_t acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = ((uint64_t)(uint32_t)off[i] << 32 | (uint64_t)i) + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        acc += z ^ (z >> 31);
+    }
+    acc = rtgd::wave_sum_u64(acc);
+    if (__lane_id() == 0) atomicAdd(h, (unsigned long long)acc);
+}
+
+// Graph-level part of the batched plan, on the device: each work edge goes to its owner (the
+// endpoint with the longer out-list); the edges are grouped by owner with a stable radix sort on
+// the owner bits (each owner's partners stay in edge order, so plans are deterministic); then
+// partner spans, run boundaries, per-owner partner counts and streamed work, the partner
+// offsets (scanned counts), their validation and the offsets' fingerprint. Only the sums come
+// back to the host.
 int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     using namespace rtg;
     const rtg_graph *g = p->g;
@@ -711,8 +894,7 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     const int bits = (std::max(1, 32 - __builtin_clz((uint32_t)n + 1u)) + 7) / 8 * 8;
     if (ne >= ((int64_t)1 << (63 - bits))) return set_error(RTG_EINVAL, "batched TC: edge range too large for the plan keys");
     const size_t kn = (size_t)std::max<int64_t>(ne, 1);
-    const size_t bytes = Arena::round(sizeof(uint32_t) * ((size_t)n + 1)) * 2 +
-                         Arena::round(sizeof(unsigned long long) * ((size_t)n + 1)) + Arena::round(64) +
+    const size_t bytes = Arena::round(sizeof(uint32_t) * ((size_t)n + 1)) * 2 + Arena::round(64) +
                          Arena::round(sizeof(uint64_t) * kn) * 4 + Arena::round(radix_sort_work_bytes((int64_t)kn)) +
                          Arena::round(std::max(scan_work_bytes((int64_t)kn), scan_work_bytes(n + 1))) + 4096;
     char *buf = nullptr;
@@ -722,16 +904,16 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     Arena ar{buf, 0, bytes};
     uint32_t *d_cnt = ar.take<uint32_t>((size_t)n + 1);
     uint32_t *d_first = ar.take<uint32_t>((size_t)n + 1);
-    unsigned long long *d_work = ar.take<unsigned long long>((size_t)n + 1);
-    unsigned long long *d_bytes = ar.take<unsigned long long>(4);
+    unsigned long long *d_bytes = ar.take<unsigned long long>(8);  // alg, merge, edges, flag, fingerprint
     uint64_t *keys[2] = {ar.take<uint64_t>(kn), ar.take<uint64_t>(kn)};
     uint64_t *lens = ar.take<uint64_t>(kn);
     uint64_t *lens_ex = ar.take<uint64_t>(kn);
     void *sort_work = ar.take<char>(radix_sort_work_bytes((int64_t)kn));
     void *scan_work = ar.take<char>(std::max(scan_work_bytes((int64_t)kn), scan_work_bytes(n + 1)));
+    RTG_TRY(dev_alloc(&p->d_work, (size_t)n + 1));
     HIP_TRY(hipMemsetAsync(d_cnt, 0, sizeof(uint32_t) * (size_t)(n + 1), st));
-    HIP_TRY(hipMemsetAsync(d_work, 0, sizeof(unsigned long long) * (size_t)(n + 1), st));
-    HIP_TRY(hipMemsetAsync(d_bytes, 0, sizeof(unsigned long long) * 4, st));
+    HIP_TRY(hipMemsetAsync(p->d_work, 0, sizeof(unsigned long long) * (size_t)(n + 1), st));
+    HIP_TRY(hipMemsetAsync(d_bytes, 0, sizeof(unsigned long long) * 8, st));
     int which = 0;
     if (ne > 0) {
         plan_keys<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end, bits,
@@ -740,12 +922,15 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
         int passes = 0;
         RTG_TRY(radix_sort_u64(ctx, keys, ne, bits, &which, &passes, sort_work));
     }
-    unsigned long long hb[4] = {0, 0, 0, 0};
+    offsets_fingerprint<<<grid_cap(ctx, (int64_t)n + 1, 256, 4), 256, 0, st>>>(g->d_offsets, n, d_bytes + 4);
+    HIP_TRY(hipGetLastError());
+    unsigned long long hb[8] = {};
     HIP_TRY(hipMemcpyAsync(hb, d_bytes, sizeof(hb), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     p->alg_bytes = hb[0];
     p->merge_bytes = hb[1];
     p->n_work_edges = (int64_t)hb[2];
+    p->offsets_fingerprint = hb[4];
     const int64_t nw = p->n_work_edges;
     RTG_TRY(dev_alloc(&p->d_wspan, (size_t)std::max<int64_t>(nw, 1)));
     if (p->opts.per_edge) RTG_TRY(dev_alloc(&p->d_wedge, (size_t)std::max<int64_t>(nw, 1)));
@@ -754,83 +939,217 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
                                                               nw, bits, p->d_wspan, p->d_wedge, lens, d_first);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u64_w(ctx, lens, lens_ex, nw, nullptr, scan_work));
-        plan_runs<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(keys[which], nw, bits, d_first, lens, lens_ex, d_cnt, d_work);
+        plan_runs<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(keys[which], nw, bits, d_first, lens, lens_ex, d_cnt,
+                                                             p->d_work);
         HIP_TRY(hipGetLastError());
     }
     RTG_TRY(dev_alloc(&p->d_wofs, (size_t)n + 1));
     RTG_TRY(scan_exclusive_u32_w(ctx, d_cnt, (uint32_t *)p->d_wofs, n + 1, nullptr, scan_work));
-    p->hoff.resize((size_t)n + 1);
-    p->hcnt.resize((size_t)n + 1);
-    p->hwork.resize((size_t)n + 1);
-    HIP_TRY(hipMemcpyAsync(p->hoff.data(), g->d_offsets, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(p->hwork.data(), d_work, sizeof(unsigned long long) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(p->hcnt.data(), p->d_wofs, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyDeviceToHost, st));
+    uint32_t *d_flag = reinterpret_cast<uint32_t *>(d_bytes + 3);
+    if (n > 0) {
+        plan_check_offsets<<<grid_cap(ctx, n, 256, 4), 256, 0, st>>>(p->d_wofs, n, nw, d_flag);
+        HIP_TRY(hipGetLastError());
+    }
+    uint32_t flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, d_flag, sizeof(flag), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    // The counting kernel reads partner spans through these offsets: refuse a plan whose
-    // offsets are not a partition of the nw spans.
-    bool ok = p->hcnt[0] == 0 && (int64_t)p->hcnt[(size_t)n] == nw;
-    for (int32_t x = 0; ok && x < n; ++x) ok = p->hcnt[(size_t)x] <= p->hcnt[(size_t)x + 1];
-    if (!ok) return set_error(RTG_EHIP, "batched TC: inconsistent partner offsets in the plan");
+    if (flag) return set_error(RTG_EHIP, "batched TC: inconsistent partner offsets in the plan");
     return RTG_OK;
 }
 
-// Indices 0 .. key.size()-1 in descending order of key, keys within 1/16 of an octave of each
-// other (and ties) in index order: a counting sort on 16*log2(key), linear in the count. The
-// work queue only needs its heaviest items roughly first.
-std::vector<int64_t> order_heaviest_first(const std::vector<uint64_t> &key) {
-    constexpr int kSub = 16, kBuckets = 1 + 64 * kSub;
-    const auto bucket = [](uint64_t k) -> int {
-        if (k == 0) return 0;
-        const int e = 63 - __builtin_clzll(k);
-        const int frac = e >= 4 ? (int)((k >> (e - 4)) & 15u) : (int)((k << (4 - e)) & 15u);
-        return 1 + e * kSub + frac;
-    };
-    // Per-range histograms, then bucket starts (heaviest bucket first; within a bucket by range,
-    // so by index: the order does not depend on the thread count), then a parallel scatter.
-    const int64_t n = (int64_t)key.size();
-    const int64_t nr = std::max<int64_t>(1, std::min<int64_t>(rtg::host_threads(), n / 65536));
-    std::vector<std::vector<int64_t>> start((size_t)nr, std::vector<int64_t>(kBuckets, 0));
-    std::vector<uint16_t> bk((size_t)n);
-    rtg::parallel_for(nr, [&](int64_t r0, int64_t r1) {
-        for (int64_t r = r0; r < r1; ++r)
-            for (int64_t i = n * r / nr; i < n * (r + 1) / nr; ++i) {
-                bk[(size_t)i] = (uint16_t)bucket(key[(size_t)i]);
-                ++start[(size_t)r][bk[(size_t)i]];
-            }
-    }, 1);
-    int64_t acc = 0;
-    for (int b = kBuckets - 1; b >= 0; --b)
-        for (int64_t r = 0; r < nr; ++r) {
-            const int64_t c = start[(size_t)r][b];
-            start[(size_t)r][b] = acc;
-            acc += c;
-        }
-    std::vector<int64_t> out((size_t)n);
-    rtg::parallel_for(nr, [&](int64_t r0, int64_t r1) {
-        for (int64_t r = r0; r < r1; ++r)
-            for (int64_t i = n * r / nr; i < n * (r + 1) / nr; ++i) out[(size_t)start[(size_t)r][bk[(size_t)i]]++] = i;
-    }, 1);
-    return out;
+// ---- device planning: the work queue ----
+
+// Owner buckets: 0 wave owners (small), 1 block owners of out-degree <= 1024, 2 larger block
+// owners whose hash table fits the LDS, 3 owners too long for an LDS hash table.
+__host__ __device__ __forceinline__ int owner_bucket(int32_t d, int32_t wave_owner_max, int32_t fit_block) {
+    return d <= wave_owner_max ? 0 : d > fit_block ? 3 : d <= 1024 ? 1 : 2;
 }
 
-// Tune-dependent part: table sizes, the work queue (heaviest first) and the kernel instance.
-int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
-    using namespace rtg;
-    const rtg_graph *g = p->g;
-    const int32_t n = g->n;
-    hipStream_t st = ctx->stream;
-    const std::vector<int32_t> &hoff = p->hoff, &hcnt = p->hcnt;
-    const std::vector<unsigned long long> &hwork = p->hwork;
-    const int threads = tune.block_threads;
-    if (threads != 256 && threads != 512) return set_error(RTG_EINVAL, "batched TC: threads must be 256 or 512");
-    const int waves = threads / 64;
-    const int32_t max_d = g->max_row;
+// Order key of a cost: 16 steps per octave, heavier first once sorted ascending.
+__device__ __forceinline__ uint32_t cost_key(uint64_t k) {
+    if (k == 0) return 0xFFFFu;
+    const int e = 63 - __clzll((long long)k);
+    const uint32_t frac = e >= 4 ? (uint32_t)((k >> (e - 4)) & 15u) : (uint32_t)((k << (4 - e)) & 15u);
+    return 0xFFFFu - (1u + (uint32_t)e * 16u + frac);
+}
+
+struct QueueParams {
+    const int32_t *off;
+    const int32_t *adj;
+    const int32_t *wofs;
+    const unsigned long long *work;  // streamed partner elements per owner
+    int32_t n;
+    int32_t wave_owner_max, fit_block, max_table, wave_table;
+    int32_t only_bucket;             // -1: every owner; else only this bucket's (autotuning)
+    int64_t item_work;
+    int32_t strat[kBuckets];
+};
+
+// What the queue does with owner x: its bucket, item kind and number of block items (0 for a
+// small owner or an owner without partners), the wave mode of a small owner.
+struct OwnerPlan {
+    int32_t bucket, d, c;
+    uint32_t kind;
+    int64_t chunks;
+    uint64_t w;
+    bool small;
+};
+
+__device__ __forceinline__ OwnerPlan owner_plan(const QueueParams &q, int32_t x) {
+    OwnerPlan o{};
+    o.c = q.wofs[x + 1] - q.wofs[x];
+    if (o.c == 0) return o;
+    o.d = q.off[x + 1] - q.off[x];
+    o.w = q.work[x];
+    o.bucket = owner_bucket(o.d, q.wave_owner_max, q.fit_block);
+    if (q.only_bucket >= 0 && o.bucket != q.only_bucket) {
+        o.c = 0;
+        return o;
+    }
+    if (o.bucket == 0) {
+        o.small = true;
+        return o;
+    }
+    const int32_t s = q.strat[o.bucket];
+    const uint32_t words = (((uint32_t)(q.adj[q.off[x] + o.d - 1] - q.adj[q.off[x]])) >> 5) + 1u;
+    if (s == kStratBitmap && words <= (uint32_t)q.max_table) o.kind = kItemBitmap;
+    else if (s == kStratMerge && o.d <= q.max_table) o.kind = kItemSorted;
+    else if (s == kStratSearch || o.d > q.fit_block) o.kind = kItemSearch;
+    else o.kind = kItemHash;
+    int64_t chunks = (int64_t)((o.w + (uint64_t)q.item_work / 2) / (uint64_t)q.item_work);
+    o.chunks = std::max<int64_t>(1, std::min<int64_t>(chunks, o.c));
+    return o;
+}
+
+// Per owner: block items (nblk) or small-owner slot (nsmall), and the sums acc[0] staged bytes,
+// acc[1 + b] items of bucket b (owners for bucket 0), acc[5] search items, acc[6] items with a
+// bitmap or sorted-list table (including small owners whose wave mode is one of them).
+__global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *__restrict__ nblk,
+                                                      uint32_t *__restrict__ nsmall, unsigned long long *acc) {
+    uint64_t v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < q.n; x += (int64_t)gridDim.x * blockDim.x) {
+        const OwnerPlan o = owner_plan(q, (int32_t)x);
+        uint32_t nb = 0, ns = 0;
+        if (o.c > 0) {
+            v[0] += 4ull * o.w + 8ull * (uint64_t)o.c + 8ull;  // partner elements and spans, owner row offsets
+            if (o.small) {
+                ns = 1;
+                v[0] += 4ull + 8ull + 4ull * (uint64_t)o.d;  // small-list entry, partner offsets, owner list
+                v[1] += 1;
+                v[6] += q.strat[0] == kStratBitmap || q.strat[0] == kStratMerge ? 1 : 0;
+            } else {
+                nb = (uint32_t)o.chunks;
+                v[0] += (uint64_t)o.chunks * (16ull + 4ull * (uint64_t)o.d);  // queue entry, owner list per item
+                v[1 + o.bucket] += (uint64_t)o.chunks;
+                v[5] += o.kind == kItemSearch ? (uint64_t)o.chunks : 0;
+                v[6] += (o.kind == kItemBitmap || o.kind == kItemSorted) ? (uint64_t)o.chunks : 0;
+            }
+        }
+        nblk[x] = nb;
+        nsmall[x] = ns;
+    }
+    __shared__ uint64_t part[4][7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const uint64_t t = rtgd::wave_sum_u64(v[k]);
+        if (__lane_id() == 0) part[threadIdx.x >> 6][k] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        uint64_t t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w][threadIdx.x];
+        if (t) atomicAdd(&acc[threadIdx.x], (unsigned long long)t);
+    }
+}
+
+// Writes each owner's block items (partners split evenly into o.chunks runs) and small-owner
+// entry, with their order keys (position << 16 | heaviness key).
+__global__ __launch_bounds__(256) void queue_emit(QueueParams q, const uint32_t *__restrict__ blk_off,
+                                                  const uint32_t *__restrict__ small_off, TcItem *__restrict__ items,
+                                                  uint64_t *__restrict__ ikeys, int32_t *__restrict__ small,
+                                                  uint64_t *__restrict__ swork, uint64_t *__restrict__ skeys) {
+    for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < q.n; x += (int64_t)gridDim.x * blockDim.x) {
+        const OwnerPlan o = owner_plan(q, (int32_t)x);
+        if (o.c == 0) continue;
+        if (o.small) {
+            const uint32_t j = small_off[x];
+            small[j] = (int32_t)x;
+            swork[j] = o.w + (uint64_t)o.d;
+            skeys[j] = ((uint64_t)j << 16) | cost_key(o.w + (uint64_t)o.d);
+            continue;
+        }
+        const int32_t base = q.wofs[x];
+        const uint64_t per = o.kind == kItemSearch ? (uint64_t)(32 - __clz((uint32_t)o.d)) / 2 : 1;  // ~log2(d) reads per element
+        for (int64_t k = 0; k < o.chunks; ++k) {
+            const uint32_t idx = blk_off[x] + (uint32_t)k;
+            const int32_t b = base + (int32_t)((int64_t)o.c * k / o.chunks);
+            const int32_t e = base + (int32_t)((int64_t)o.c * (k + 1) / o.chunks);
+            items[idx] = TcItem{(int32_t)o.kind, (int32_t)x, b, e};
+            ikeys[idx] = ((uint64_t)idx << 16) | cost_key(o.w / (uint64_t)o.chunks * per + (uint64_t)o.d);
+        }
+    }
+}
+
+// Heaviest first: dst[i] = src[key_i >> 16] over the keys sorted on their low 16 bits (stable,
+// so equal heaviness keeps owner order).
+__global__ void queue_gather_items(const uint64_t *__restrict__ keys, int64_t n, const TcItem *__restrict__ src,
+                                   TcItem *__restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[keys[i] >> 16];
+}
+
+__global__ void queue_gather_small(const uint64_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ src,
+                                   const uint64_t *__restrict__ wsrc, int32_t *__restrict__ dst, uint64_t *__restrict__ wdst) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i] >> 16;
+        dst[i] = src[k];
+        wdst[i] = wsrc[k];
+    }
+}
+
+// Wave items: consecutive small owners (heaviest first) grouped by
+// g(i) = min(P(i) / target, i / waves), P the exclusive prefix of their work — a group holds
+// `waves` owners when they are heavy and about `target` work when they are light. flag[i] = 1
+// where a group starts.
+__global__ void queue_group_flags(const uint64_t *__restrict__ pw, int64_t n, uint64_t target, int waves,
+                                  uint32_t *__restrict__ flag) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t gi = min(pw[i] / target, (uint64_t)i / (uint64_t)waves);
+        const uint64_t gp = i ? min(pw[i - 1] / target, (uint64_t)(i - 1) / (uint64_t)waves) : ~0ull;
+        flag[i] = (i == 0 || gi != gp) ? 1u : 0u;
+    }
+}
+
+__global__ void queue_group_starts(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ gid, int64_t n,
+                                   uint32_t *__restrict__ starts) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (flag[i]) starts[gid[i]] = (uint32_t)i;
+}
+
+__global__ void queue_wave_items(const uint32_t *__restrict__ starts, const uint32_t *__restrict__ ngroups_p, int64_t nsmall,
+                                 int32_t mode, TcItem *__restrict__ queue) {
+    const int64_t ng = *ngroups_p;
+    for (int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < ng; gi += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t b = (int32_t)starts[gi];
+        const int32_t e = gi + 1 < ng ? (int32_t)starts[gi + 1] : (int32_t)nsmall;
+        queue[gi] = TcItem{(int32_t)kItemWave, mode, b, e};
+    }
+}
+
+// Table sizes of a tune: LDS words per block, per wave, the largest owner whose hash table fits
+// a block, the largest wave owner.
+struct TableSizes {
+    int32_t max_table, wave_table, fit_block, wave_owner_max;
+};
+
+TableSizes table_sizes(const BatchTune &tune, int32_t max_d) {
+    const int waves = tune.block_threads / 64;
     // Block table: both cuckoo halves of the largest owner that fits, at least 2048 slots, a
     // multiple of 64 * waves so that the wave-private tables split it evenly, at most the cap.
-    // Owners whose table would exceed the cap become search items.
+    // Owners whose table would exceed the cap are searched (or given a bitmap or list).
     const int32_t quantum = 64 * waves;
     const int32_t cap = (tune.table_cap > 0 ? tune.table_cap : kMaxTableSlots) / quantum * quantum;
-    int32_t fit_block = 0;  // largest out-degree whose table fits the cap
+    int32_t fit_block = 0;
     {
         int32_t lo = 0, hi = std::max<int32_t>(max_d, 0);
         while (lo < hi) {
@@ -846,119 +1165,136 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune) {
     int32_t fit = 0;  // largest out-degree whose table fits a wave's share
     while (2 * (int32_t)cuckoo_half(fit + 1) <= wave_table) ++fit;
     const int32_t wave_owner_max = tune.wave_owner_max >= 0 ? std::min(tune.wave_owner_max, fit) : fit;
+    return {max_table, wave_table, fit_block, wave_owner_max};
+}
 
-    // Owners are classified over contiguous ranges in parallel; the ranges' lists are joined in
-    // order, so the lists are the same as a serial pass would give.
-    struct Part {
-        std::vector<TcItem> items;
-        std::vector<uint64_t> cost;
-        std::vector<std::pair<uint64_t, int32_t>> small;  // (work, owner)
-        int64_t search = 0;
-    };
-    const int64_t nparts = std::max<int64_t>(1, std::min<int64_t>(host_threads(), ((int64_t)n + 65535) / 65536));
-    std::vector<Part> parts((size_t)nparts);
-    parallel_for(nparts, [&](int64_t c0, int64_t c1) {
-        for (int64_t cpart = c0; cpart < c1; ++cpart) {
-            Part &pt = parts[(size_t)cpart];
-            const int32_t x0 = (int32_t)((int64_t)n * cpart / nparts), x1 = (int32_t)((int64_t)n * (cpart + 1) / nparts);
-            for (int32_t x = x0; x < x1; ++x) {
-                const int32_t c = hcnt[x + 1] - hcnt[x];
-                if (c == 0) continue;
-                const int32_t d = hoff[x + 1] - hoff[x];
-                const uint64_t w = hwork[x];
-                if (d <= wave_owner_max) {
-                    pt.small.push_back({w + (uint64_t)d, x});
-                    continue;
-                }
-                int64_t chunks = std::max<int64_t>(1, (int64_t)((w + tune.item_work / 2) / tune.item_work));
-                chunks = std::min<int64_t>(chunks, c);
-                const bool search = d > fit_block;  // no LDS table: binary searches, ~log2(d) reads each
-                const uint64_t per = search ? (uint64_t)(32 - __builtin_clz((uint32_t)d)) / 2 : 1;
-                for (int64_t k = 0; k < chunks; ++k) {
-                    const int32_t b = hcnt[x] + (int32_t)(c * k / chunks), e = hcnt[x] + (int32_t)(c * (k + 1) / chunks);
-                    pt.items.push_back(TcItem{search ? 2 : 0, x, b, e});
-                    pt.cost.push_back(w / chunks * per + (uint64_t)d);
-                    pt.search += search ? 1 : 0;
-                }
-            }
-        }
-    }, 1);
-    std::vector<TcItem> items;
-    std::vector<std::pair<uint64_t, int32_t>> small;  // (work, owner)
-    std::vector<uint64_t> item_cost;
-    {
-        size_t ni = 0, ns = 0;
-        for (const Part &pt : parts) {
-            ni += pt.items.size();
-            ns += pt.small.size();
-        }
-        items.reserve(ni);
-        item_cost.reserve(ni);
-        small.reserve(ns);
-        p->n_search_items = 0;
-        for (const Part &pt : parts) {
-            items.insert(items.end(), pt.items.begin(), pt.items.end());
-            item_cost.insert(item_cost.end(), pt.cost.begin(), pt.cost.end());
-            small.insert(small.end(), pt.small.begin(), pt.small.end());
-            p->n_search_items += pt.search;
-        }
-    }
-    // Heaviest first.
-    std::vector<TcItem> queue;
-    queue.reserve(items.size() + small.size() / 4 + 1);
-    for (int64_t i : order_heaviest_first(item_cost)) queue.push_back(items[i]);
-    {
-        std::vector<uint64_t> sw(small.size());
-        for (size_t i = 0; i < small.size(); ++i) sw[i] = small[i].first;
-        const std::vector<int64_t> so = order_heaviest_first(sw);
-        std::vector<std::pair<uint64_t, int32_t>> sorted(small.size());
-        for (size_t i = 0; i < small.size(); ++i) sorted[i] = small[so[i]];
-        small.swap(sorted);
-    }
-    std::vector<int32_t> small_list(small.size());
-    for (size_t i = 0; i < small.size(); ++i) small_list[i] = small[i].second;
-    // Wave items: groups of `waves` or more consecutive small owners with bounded total work.
-    size_t i = 0;
-    while (i < small.size()) {
-        size_t j = i;
-        uint64_t w = 0;
-        while (j < small.size() && (j - i < (size_t)waves || w < (uint64_t)tune.wave_item_work * waves)) {
-            w += small[j].first;
-            ++j;
-        }
-        queue.push_back(TcItem{1, 0, (int32_t)i, (int32_t)j});
-        i = j;
-    }
+// Tune-dependent part, on the device: owners classified into buckets with their strategies, the
+// block items and small owners ordered heaviest first (radix sort on a log-scale key), small
+// owners grouped into wave items, and the kernel instance. only_bucket >= 0 builds the queue of
+// that bucket alone (autotuning). Two small downloads: the sizes, then the number of wave items.
+int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_bucket = -1) {
+    using namespace rtg;
+    const rtg_graph *g = p->g;
+    const int32_t n = g->n;
+    hipStream_t st = ctx->stream;
+    const int threads = tune.block_threads;
+    if (threads != 256 && threads != 512) return set_error(RTG_EINVAL, "batched TC: threads must be 256 or 512");
+    const int waves = threads / 64;
+    const TableSizes ts = table_sizes(tune, g->max_row);
+    QueueParams q{g->d_offsets, g->d_adjs, p->d_wofs, p->d_work, n, ts.wave_owner_max, ts.fit_block, ts.max_table,
+                  ts.wave_table, only_bucket, tune.item_work, {}};
+    for (int b = 0; b < kBuckets; ++b) q.strat[b] = tune.strat[b];
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->d_small) (void)hipFree(p->d_small);
     p->d_items = nullptr;
     p->d_small = nullptr;
-    p->n_items = (int64_t)queue.size();
-    p->n_small = (int64_t)small_list.size();
-    p->n_block_items = (int64_t)items.size();
-    RTG_TRY(dev_alloc(&p->d_items, queue.size()));
-    RTG_TRY(dev_alloc(&p->d_small, small_list.size()));
-    if (!queue.empty())
-        HIP_TRY(hipMemcpyAsync(p->d_items, queue.data(), sizeof(TcItem) * queue.size(), hipMemcpyHostToDevice, st));
-    if (!small_list.empty())
-        HIP_TRY(hipMemcpyAsync(p->d_small, small_list.data(), sizeof(int32_t) * small_list.size(), hipMemcpyHostToDevice, st));
+
+    // pass 1: classification, counts, sums
+    const size_t nn = (size_t)n + 1;
+    const size_t b1 = Arena::round(sizeof(uint32_t) * nn) * 4 + Arena::round(sizeof(unsigned long long) * 16) +
+                      Arena::round(scan_work_bytes((int64_t)nn)) + 4096;
+    char *buf1 = nullptr;
+    RTG_TRY(dev_alloc(&buf1, b1));
+    DevFree f1;
+    f1.p = buf1;
+    Arena a1{buf1, 0, b1};
+    uint32_t *nblk = a1.take<uint32_t>(nn), *nsmall = a1.take<uint32_t>(nn);
+    uint32_t *blk_off = a1.take<uint32_t>(nn), *small_off = a1.take<uint32_t>(nn);
+    unsigned long long *acc = a1.take<unsigned long long>(16);  // [0..6] sums, [8] totals (two u32)
+    void *scan1 = a1.take<char>(scan_work_bytes((int64_t)nn));
+    HIP_TRY(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * 16, st));
+    uint32_t *totals = reinterpret_cast<uint32_t *>(acc + 8);
+    if (n > 0) {
+        queue_classify<<<grid_cap(ctx, n, 256, 8), 256, 0, st>>>(q, nblk, nsmall, acc);
+        HIP_TRY(hipGetLastError());
+        RTG_TRY(scan_exclusive_u32_w(ctx, nblk, blk_off, n, totals, scan1));
+        RTG_TRY(scan_exclusive_u32_w(ctx, nsmall, small_off, n, totals + 1, scan1));
+    }
+    unsigned long long h[10] = {};
+    HIP_TRY(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    const int64_t nb = (int64_t)(uint32_t)(h[8] & 0xFFFFFFFFull), ns = (int64_t)(uint32_t)(h[8] >> 32);
+    p->staged_bytes = h[0] + 16ull * (uint64_t)ns;  // a wave item per small owner at most
+    for (int b = 0; b < kBuckets; ++b) p->bucket_items[b] = (int64_t)h[1 + b];
+    p->n_search_items = (int64_t)h[5];
+    p->all_strategies = h[6] > 0;
+    p->n_block_items = nb;
+    p->n_small = ns;
+
+    // pass 2: items, small owners, order, wave groups
+    RTG_TRY(dev_alloc(&p->d_items, (size_t)std::max<int64_t>(nb + ns, 1)));
+    RTG_TRY(dev_alloc(&p->d_small, (size_t)std::max<int64_t>(ns, 1)));
+    const int64_t mx = std::max<int64_t>(std::max(nb, ns), 1);
+    const size_t b2 = Arena::round(sizeof(TcItem) * (size_t)mx) + Arena::round(sizeof(uint64_t) * (size_t)mx) * 6 +
+                      Arena::round(sizeof(int32_t) * (size_t)mx) + Arena::round(sizeof(uint32_t) * (size_t)mx) * 3 +
+                      Arena::round(radix_sort_work_bytes(mx)) + Arena::round(scan_work_bytes(mx)) + 4096;
+    char *buf2 = nullptr;
+    RTG_TRY(dev_alloc(&buf2, b2));
+    DevFree f2;
+    f2.p = buf2;
+    Arena a2{buf2, 0, b2};
+    TcItem *items_tmp = a2.take<TcItem>((size_t)mx);
+    uint64_t *k0 = a2.take<uint64_t>((size_t)mx), *k1 = a2.take<uint64_t>((size_t)mx);
+    uint64_t *sk0 = a2.take<uint64_t>((size_t)mx), *sk1 = a2.take<uint64_t>((size_t)mx);
+    uint64_t *swork = a2.take<uint64_t>((size_t)mx), *sw = a2.take<uint64_t>((size_t)mx);
+    int32_t *small_tmp = a2.take<int32_t>((size_t)mx);
+    uint32_t *flag = a2.take<uint32_t>((size_t)mx), *gid = a2.take<uint32_t>((size_t)mx);
+    uint32_t *starts = a2.take<uint32_t>((size_t)mx);
+    void *sort2 = a2.take<char>(radix_sort_work_bytes(mx));
+    void *scan2 = a2.take<char>(scan_work_bytes(mx));
+    uint32_t *ngroups = totals + 2;
+    HIP_TRY(hipMemsetAsync(ngroups, 0, sizeof(uint32_t), st));
+    if (nb + ns > 0) {
+        queue_emit<<<grid_cap(ctx, n, 256, 8), 256, 0, st>>>(q, blk_off, small_off, items_tmp, k0, small_tmp, swork, sk0);
+        HIP_TRY(hipGetLastError());
+    }
+    if (nb > 0) {
+        uint64_t *kk[2] = {k0, k1};
+        int which = 0, passes = 0;
+        RTG_TRY(radix_sort_u64(ctx, kk, nb, 16, &which, &passes, sort2));
+        queue_gather_items<<<grid_cap(ctx, nb, 256, 8), 256, 0, st>>>(kk[which], nb, items_tmp, p->d_items);
+        HIP_TRY(hipGetLastError());
+    }
+    if (ns > 0) {
+        uint64_t *kk[2] = {sk0, sk1};
+        int which = 0, passes = 0;
+        RTG_TRY(radix_sort_u64(ctx, kk, ns, 16, &which, &passes, sort2));
+        queue_gather_small<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(kk[which], ns, small_tmp, swork, p->d_small, sw);
+        HIP_TRY(hipGetLastError());
+        RTG_TRY(scan_exclusive_u64_w(ctx, sw, swork, ns, nullptr, scan2));  // exclusive work prefix, into swork
+        const uint64_t target = (uint64_t)std::max<int64_t>(1, tune.wave_item_work * waves);
+        queue_group_flags<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(swork, ns, target, waves, flag);
+        HIP_TRY(hipGetLastError());
+        RTG_TRY(scan_exclusive_u32_w(ctx, flag, gid, ns, ngroups, scan2));
+        queue_group_starts<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(flag, gid, ns, starts);
+        HIP_TRY(hipGetLastError());
+        const int32_t mode = tune.strat[0] == kStratBitmap ? (int32_t)kBitmap : tune.strat[0] == kStratMerge ? (int32_t)kSorted
+                                                                                                         : (int32_t)kCuckoo;
+        queue_wave_items<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(starts, ngroups, ns, mode, p->d_items + nb);
+        HIP_TRY(hipGetLastError());
+    }
+    uint32_t ng = 0;
+    HIP_TRY(hipMemcpyAsync(&ng, ngroups, sizeof(ng), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    p->n_items = nb + (int64_t)ng;
+    for (int b = 0; b < kBuckets; ++b) p->strat[b] = tune.strat[b];
 
     // Persistent grid: as many blocks as fit on the device at once.
-    const size_t lds = sizeof(uint32_t) * (size_t)max_table;
-    BatchKernel k = batched_kernel(threads, tune.unroll);
+    const size_t lds = sizeof(uint32_t) * (size_t)ts.max_table;
+    BatchKernel k = batched_kernel(threads, tune.unroll, false, p->all_strategies);
     if (!k) return set_error(RTG_EINVAL, "batched TC: unsupported (threads, unroll) = (" + std::to_string(threads) + ", " +
                                              std::to_string(tune.unroll) + ")");
     int per_cu = 0;
     HIP_TRY(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, lds));
     if (per_cu < 1) per_cu = 1;
-    p->max_table = max_table;
+    p->max_table = ts.max_table;
     p->block_threads = threads;
-    p->wave_owner_max = wave_owner_max;
+    p->wave_owner_max = ts.wave_owner_max;
+    p->fit_block = ts.fit_block;
     p->kernel = (void *)k;
     if (p->opts.per_edge) {
-        BatchKernel ke = batched_kernel(threads, tune.unroll, true);
+        BatchKernel ke = batched_kernel(threads, tune.unroll, true, p->all_strategies);
         int per_cu_e = 0;
         HIP_TRY(hipFuncSetAttribute((const void *)ke, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_e, ke, threads, lds));
@@ -989,38 +1325,25 @@ const char *kCandidates[] = {
     "threads=512,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
 };
 
+// Strategies the autotuner times per bucket (the first is the default).
+const int32_t kBucketCandidates[kBuckets][4] = {
+    {kStratHash, kStratBitmap, kStratMerge, -1},
+    {kStratHash, kStratBitmap, kStratMerge, kStratSearch},
+    {kStratHash, kStratBitmap, kStratMerge, kStratSearch},
+    {kStratSearch, kStratMerge, kStratBitmap, -1},
+};
+
 // Revision of the batched kernel's code, part of the cache key: raised when a change to the
 // kernel can change which candidate is fastest, so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 2;
+constexpr int kTuneRevision = 3;
 
-// FNV-1a over 64-bit little-endian words (a trailing partial word zero-padded): one multiply
-// per 8 bytes, so hashing the row offsets of a graph with millions of vertices takes well under
-// a millisecond.
-uint64_t fnv1a(const void *data, size_t bytes, uint64_t h = 1469598103934665603ull) {
-    const unsigned char *c = (const unsigned char *)data;
-    size_t i = 0;
-    for (; i + 8 <= bytes; i += 8) {
-        uint64_t w;
-        std::memcpy(&w, c + i, 8);
-        h = (h ^ w) * 1099511628211ull;
-    }
-    if (i < bytes) {
-        uint64_t w = 0;
-        std::memcpy(&w, c + i, bytes - i);
-        h = (h ^ w) * 1099511628211ull;
-    }
-    return h;
-}
-
-// Cache key: device architecture and CU count, kernel revision, graph shape, a hash of the row
-// offsets and the edge range.
+// Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
+// fingerprint, the edge range and whether the plan is for per-edge counts.
 std::string tune_key(rtg_ctx *ctx, const rtg_tc_plan *p) {
-    uint64_t h = fnv1a(p->hoff.data(), sizeof(int32_t) * p->hoff.size());
-    int64_t r[2] = {p->e_begin, p->e_end};
-    h = fnv1a(r, sizeof(r), h);
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "%s/cu%d/r%d/n%d/m%lld/%016llx", ctx->prop.gcnArchName, ctx->cu_count,
-                  kTuneRevision, p->g->n, (long long)p->g->m, (unsigned long long)h);
+    std::snprintf(buf, sizeof(buf), "%s/cu%d/r%d/n%d/m%lld/%016llx/e%lld-%lld%s", ctx->prop.gcnArchName, ctx->cu_count,
+                  kTuneRevision, p->g->n, (long long)p->g->m, (unsigned long long)p->offsets_fingerprint,
+                  (long long)p->e_begin, (long long)p->e_end, p->opts.per_edge ? "/pe" : "");
     return buf;
 }
 
@@ -1060,35 +1383,75 @@ int write_tune_cache(const char *path, const std::vector<std::pair<std::string, 
 }
 
 int launch_count(rtg_ctx *ctx, rtg_tc_plan *p);
+int launch_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *eout);
 
-// Times every candidate on the plan (median of three runs after one warm-up) and returns the
-// fastest in *best.
+// Median of three timed launches after one warm-up, of the plan as it stands (the kernel the
+// plan will run: the per-edge instantiation into a scratch output for per-edge plans).
+int time_plan(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *pe_out, float *median_ms) {
+    float ms[3];
+    for (int r = -1; r < 3; ++r) {
+        HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+        if (pe_out) RTG_TRY(launch_per_edge(ctx, p, pe_out));
+        else RTG_TRY(launch_count(ctx, p));
+        HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev[3]));
+        if (r >= 0) ms[r] = rtg::elapsed_ms(ctx->ev[2], ctx->ev[3]);
+    }
+    std::sort(ms, ms + 3);
+    *median_ms = ms[1];
+    return RTG_OK;
+}
+
+// Two stages: every tile-shape candidate with the default strategies, timed on the whole plan;
+// then, for the fastest shape, each bucket's strategy candidates timed on a queue of that bucket
+// alone (a strategy no owner of the bucket can use is skipped). The combination is kept if it
+// is at least as fast as the default strategies on the whole plan.
 int autotune(rtg_ctx *ctx, rtg_tc_plan *p, BatchTune *best) {
+    uint32_t *pe_out = nullptr;
+    DevFree fpe;
+    if (p->opts.per_edge) {
+        RTG_TRY(dev_alloc(&pe_out, (size_t)std::max<int64_t>(p->g->m, 1)));
+        fpe.p = pe_out;
+    }
     float best_ms = 1e30f;
     std::string best_s;
     for (const char *cand : kCandidates) {
         BatchTune t;
         tune_from_string(t, cand);
-        if (plan_items(ctx, p, t) != RTG_OK) continue;
-        float ms[3];
-        bool ok = true;
-        for (int r = -1; r < 3 && ok; ++r) {
-            if (hipEventRecord(ctx->ev[2], ctx->stream) != hipSuccess || launch_count(ctx, p) != RTG_OK ||
-                hipEventRecord(ctx->ev[3], ctx->stream) != hipSuccess || hipEventSynchronize(ctx->ev[3]) != hipSuccess)
-                ok = false;
-            else if (r >= 0)
-                ms[r] = rtg::elapsed_ms(ctx->ev[2], ctx->ev[3]);
-        }
-        if (!ok) continue;
-        std::sort(ms, ms + 3);
-        if (ms[1] < best_ms) {
-            best_ms = ms[1];
+        float ms = 0.f;
+        if (plan_items(ctx, p, t) != RTG_OK || time_plan(ctx, p, pe_out, &ms) != RTG_OK) continue;
+        if (ms < best_ms) {
+            best_ms = ms;
             best_s = cand;
         }
     }
     if (best_s.empty()) return rtg::set_error(RTG_EHIP, "autotune: no candidate ran");
-    *best = BatchTune{};
-    tune_from_string(*best, best_s);
+    BatchTune shape;
+    tune_from_string(shape, best_s);
+    BatchTune mix = shape;
+    for (int b = 0; b < kBuckets; ++b) {
+        float bucket_best = 1e30f;
+        for (int ci = 0; ci < 4; ++ci) {
+            const int32_t s = kBucketCandidates[b][ci];
+            if (s < 0) continue;
+            BatchTune t = shape;
+            t.strat[b] = s;
+            float ms = 0.f;
+            if (plan_items(ctx, p, t, b) != RTG_OK) continue;
+            if (p->bucket_items[b] == 0) break;  // empty bucket: keep the default
+            if ((s == kStratBitmap || s == kStratMerge) && !p->all_strategies) continue;  // no owner can use it
+            if (time_plan(ctx, p, pe_out, &ms) != RTG_OK) continue;
+            if (ms < bucket_best) {
+                bucket_best = ms;
+                mix.strat[b] = s;
+            }
+        }
+    }
+    *best = shape;
+    if (std::memcmp(mix.strat, shape.strat, sizeof(mix.strat)) != 0) {
+        float ms = 0.f;
+        if (plan_items(ctx, p, mix) == RTG_OK && time_plan(ctx, p, pe_out, &ms) == RTG_OK && ms <= best_ms) *best = mix;
+    }
     p->autotuned = true;
     return RTG_OK;
 }
@@ -1245,13 +1608,16 @@ static void fill_stats(rtg_tc_plan *p, rtg_tc_stats *s) {
     s->t_plan_ms = p->t_plan_ms;
     s->alg_bytes = p->alg_bytes;
     s->merge_bytes = p->merge_bytes;
+    s->staged_bytes = p->opts.method == RTG_TC_BATCHED ? p->staged_bytes : 0;
     s->edges = p->e_end - p->e_begin;
     if (p->opts.method == RTG_TC_BATCHED) {
-        s->nbuckets = 2;
-        s->strategy[0] = 0;  // block items: block-shared LDS hash
-        s->strategy[1] = 1;  // wave items: wave-private LDS hash
-        s->bucket_items[0] = p->n_block_items;
-        s->bucket_items[1] = p->n_small;
+        // owner buckets: small (wave) owners, block owners up to 1024, larger block owners whose
+        // hash table fits, owners too long for a hash table
+        s->nbuckets = kBuckets;
+        for (int b = 0; b < kBuckets; ++b) {
+            s->strategy[b] = p->strat[b];
+            s->bucket_items[b] = p->bucket_items[b];
+        }
     }
     s->kernel_launches = 2;
 }
@@ -1324,7 +1690,11 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
         d += "," + p->tune + ",grid=" + std::to_string(p->grid) + ",table_slots=" + std::to_string(p->max_table) +
              ",wave_owner_max=" + std::to_string(p->wave_owner_max) + ",source=" +
              (p->autotuned ? "autotune" : (p->tune_from_cache ? "cache" : (std::getenv("RTG_TC_TUNE") ? "env" : "default"))) +
-             ",search_items=" + std::to_string(p->n_search_items) +
+             ",search_items=" + std::to_string(p->n_search_items) + ",rev=" + std::to_string(kTuneRevision) +
+             ",buckets=" + std::to_string(p->bucket_items[0]) + "/" + std::to_string(p->bucket_items[1]) + "/" +
+             std::to_string(p->bucket_items[2]) + "/" + std::to_string(p->bucket_items[3]) + ",bucket_strategy=" +
+             kStratName[p->strat[0]] + "/" + kStratName[p->strat[1]] + "/" + kStratName[p->strat[2]] + "/" +
+             kStratName[p->strat[3]] + ",all_strategies=" + (p->all_strategies ? "1" : "0") +
              ",plan_graph_ms=" + std::to_string(p->t_plan_graph_ms) + ",plan_tune_ms=" + std::to_string(p->t_plan_tune_ms) +
              ",plan_items_ms=" + std::to_string(p->t_plan_items_ms);
     std::snprintf(buf, len, "%s", d.c_str());
@@ -1336,7 +1706,7 @@ RTG_API int rtg_tc_plan_destroy(rtg_tc_plan *p) {
     if (p->ctx) (void)hipSetDevice(p->ctx->device);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    void *bufs[] = {p->d_row, p->d_wofs, p->d_wspan, p->d_wedge, p->d_small, p->d_items, p->d_count};
+    void *bufs[] = {p->d_row, p->d_wofs, p->d_wspan, p->d_wedge, p->d_small, p->d_items, p->d_count, p->d_work};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;

@@ -6,15 +6,22 @@
 #include <string>
 #include <vector>
 
-// A work item of the batched kernel. kind 0: block item — owner vertex `a` whose out-list is
-// staged in the block's LDS hash table, partners wpart[b .. c). kind 1: wave item — small
-// owners small_list[b .. c), one owner at a time per wave, each with a wave-private table.
+// A work item of the batched kernel: `kind` names the strategy.
+//   kItemHash    block item: owner vertex `a`'s out-list in the block's LDS cuckoo hash table,
+//                partners wspan[b .. c) streamed against it;
+//   kItemWave    wave item: small owners small_list[b .. c), one at a time per wave, each in a
+//                wave-private table whose strategy is `a` (0 hash, 2 bitmap, 3 sorted list);
+//   kItemSearch  block item whose partners' elements are binary-searched in the owner's list in
+//                HBM (no LDS table);
+//   kItemBitmap  block item with the owner's list as an LDS bitmap over its value range;
+//   kItemSorted  block item with the owner's sorted list in LDS, merged with each partner list.
 struct TcItem {
     int32_t kind;
     int32_t a;
     int32_t b;
     int32_t c;
 };
+constexpr uint32_t kItemHash = 0, kItemWave = 1, kItemSearch = 2, kItemBitmap = 3, kItemSorted = 4;
 
 struct rtg_tc_plan {
     rtg_ctx *ctx = nullptr;
@@ -28,6 +35,7 @@ struct rtg_tc_plan {
     uint32_t *d_wedge = nullptr;     // batched, per-edge plans: oriented edge of each span
     int32_t *d_small = nullptr;      // batched: small owners, heaviest first
     TcItem *d_items = nullptr;       // batched: work queue, heaviest first
+    unsigned long long *d_work = nullptr;  // batched: streamed partner elements per owner (n+1)
     unsigned long long *d_count = nullptr;  // [0] = triangle count, [1] = queue head
     int64_t n_items = 0;
     int64_t n_small = 0;
@@ -36,6 +44,7 @@ struct rtg_tc_plan {
     int32_t max_table = 0;           // LDS hash-table slots per block
     int32_t block_threads = 256;
     int32_t wave_owner_max = 0;      // owners with d+ <= this run as wave items
+    int32_t fit_block = 0;           // largest out-degree whose hash table fits a block
     int32_t long_min = 48;           // partner lists at least this long are streamed directly
     int32_t grab = 64;               // partners per wave grab inside a block item
     int32_t grid = 0;
@@ -43,18 +52,20 @@ struct rtg_tc_plan {
     void *kernel = nullptr;          // batched: selected kernel instantiation
     void *kernel_pe = nullptr;       // batched, per-edge plans: the per-edge instantiation
     int32_t grid_pe = 0;
-    // byte model
-    uint64_t alg_bytes = 0, merge_bytes = 0;
+    // byte models: per-edge (no reuse), merge-only, and the owner-staged launch's reads
+    uint64_t alg_bytes = 0, merge_bytes = 0, staged_bytes = 0;
     // captured launch sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double t_plan_ms = 0.0;
     double t_plan_graph_ms = 0.0, t_plan_tune_ms = 0.0, t_plan_items_ms = 0.0;  // host wall time of the stages
     // batched planning state
-    std::vector<int32_t> hoff, hcnt;        // host copies: row offsets, partner-list offsets
-    std::vector<unsigned long long> hwork;  // streamed elements per owner
+    uint64_t offsets_fingerprint = 0;  // of the row offsets (tuning-cache key)
     int64_t n_block_items = 0;
-    int64_t n_search_items = 0;  // block items of owners too long for an LDS table
+    int64_t n_search_items = 0;        // block items searched in HBM
+    int64_t bucket_items[4] = {};      // per owner bucket: block items (small owners for bucket 0)
+    int32_t strat[4] = {};             // per owner bucket: strategy (0 hash, 1 search, 2 bitmap, 3 merge)
+    bool all_strategies = false;       // the queue holds bitmap or sorted-list items
     std::string tune;                       // chosen configuration
     bool autotuned = false, tune_from_cache = false;
 };

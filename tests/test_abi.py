@@ -43,8 +43,8 @@ def test_error_reporting_without_gpu_or_bad_args():
 
 
 def test_struct_sizes_match_header():
-    # rtg_tc_stats layout: 2 doubles, 2 u64, i64, i32, 8 i32, 8 i64, 8 doubles, 2 i32
-    assert ctypes.sizeof(rtg.TcStats) == 8 * 2 + 8 * 2 + 8 + 4 + 4 * 8 + 4 + 8 * 8 + 8 * 8 + 4 * 2
+    # rtg_tc_stats layout: 2 doubles, 2 u64, i64, i32, 8 i32, 8 i64, 8 doubles, 2 i32, u64
+    assert ctypes.sizeof(rtg.TcStats) == 8 * 2 + 8 * 2 + 8 + 4 + 4 * 8 + 4 + 8 * 8 + 8 * 8 + 4 * 2 + 8
     assert ctypes.sizeof(rtg.TcOpts) == 4 + 4 + 8 + 8 + 8 + 4 + 4
 
 

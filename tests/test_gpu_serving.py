@@ -107,3 +107,42 @@ def test_device_generators_match_host(ctx):
     d = rtg.gen_config_model_device(ctx, deg, 0.3, seed=6)
     assert d.shape[0] == h.size
     assert np.array_equal(d.numpy(), h)
+
+
+def test_replay_full_size_matches_oracle(ctx):
+    """The serving workload at full size: the power-law configuration-model graph (n = 3,000,000,
+    exponent 2.1, degrees up to 30,000, triangle closing 0.3, seed 1) as a symmetric CSR and 2^28
+    sorted uint32 values; 2^20 Poisson queries at 2e6 q/s (70% degree-biased pairs, 30% lookups of
+    U{1..4096} keys) replayed through the server. Every pair answer is checked against the oracle
+    (hub pairs with lists tens of thousands long included), and 3,000 lookup batches."""
+    print("[serving full size] building the 3M-node graph and the 2^28 array", flush=True)
+    deg = rtg.gen_powerlaw_degrees(3_000_000, 2.1, 30_000.0, 50.0, seed=1)
+    d_pairs = rtg.gen_config_model_device(ctx, deg, 0.3, seed=1)
+    g = rtg.DeviceGraph.from_pairs(ctx, d_pairs, relabel=False, orient=False)
+    d_pairs.free()
+    d_arr = rtg.gen_sorted_array_device(ctx, 1 << 28, seed=1)
+    nq = 1 << 20
+    print("[serving full size] replaying 2^20 queries", flush=True)
+    srv = rtg.Server(ctx, g, d_arr, 1 << 28, slots=2)
+    st, pr, ls = srv.replay(rate_qps=2.0e6, nqueries=nq, warmup=0, pair_frac=0.7, max_keys=4096, seed=1,
+                            want_results=True)
+    srv.close()
+    assert st["completed"] == nq and st["rejected"] == 0 and st["failed"] == 0
+    print("[serving full size] checking against the oracle", flush=True)
+    q = rtg.stream_queries(g, 2.0e6, nq, pair_frac=0.7, max_keys=4096, seed=1)
+    off, adj = g.download()
+    isp = q["kind"] == rtg.REQ_PAIR
+    uv = np.stack([q["u"][isp], q["v"][isp]], 1)
+    dmin = np.minimum(np.diff(off)[uv[:, 0]], np.diff(off)[uv[:, 1]])
+    assert dmin.max() > 10_000  # hub pairs are part of the checked stream
+    exp = oracle_pairs(off, adj, uv)
+    assert np.array_equal(pr[isp], exp)
+    del off, adj
+    arr = d_arr.numpy()
+    rng = np.random.default_rng(3)
+    lk = np.flatnonzero(~isp)
+    for i in rng.choice(lk, size=3000, replace=False):
+        keys = q["pool"][q["key_off"][i]: q["key_off"][i] + q["nkeys"][i]]
+        assert int(ls[i]) == int((orc.lookup(arr, keys).astype(np.int64) + 2).sum())
+    g.close()
+    d_arr.free()

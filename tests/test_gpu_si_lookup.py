@@ -337,3 +337,90 @@ def test_set_intersection_tile_host_entry(method):
     rc = rtg._lib.rtg_set_intersection_tile_host(rtg._ptr(offs), rtg._ptr(el), C.c_uint32(5), C.c_uint32(4),
                                                  C.c_int32(method), C.c_int(0), rtg._ptr(out), C.byref(ms))
     assert rc == 0 and np.all(out == 700) and ms.value > 0
+
+
+# ---- device sort of the lookup array (rtg_sort_i32 / rtg_sort_u32) ---------------------------
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 4095, 4096, 4097, 100_003, (1 << 20) + 3])
+@pytest.mark.parametrize("signed", [True, False])
+def test_device_sort_32(ctx, n, signed):
+    rng = np.random.default_rng(n + signed)
+    if signed:
+        a = rng.integers(-(1 << 31), 1 << 31, size=n, dtype=np.int64).astype(np.int32)
+        if n > 8:
+            a[:4] = [np.iinfo(np.int32).min, np.iinfo(np.int32).max, -1, 0]
+            a[n // 2: n // 2 + 3] = a[5]  # duplicates
+    else:
+        a = rng.integers(0, 1 << 32, size=n, dtype=np.uint64).astype(np.uint32)
+        if n > 8:
+            a[:4] = [0, 0xFFFFFFFF, 0x80000000, 0x7FFFFFFF]
+            a[n // 2: n // 2 + 3] = a[5]
+    d = ctx.to_device(a if n else np.zeros(1, a.dtype))
+    rtg.sort_device(ctx, d, n, signed=signed)
+    assert np.array_equal(d.numpy()[:n], np.sort(a))
+
+
+def test_device_sort_reference_lookup_shape(ctx):
+    """The reference's lookup workload shape (rt_search/measure.cpp:32-51): int32 uniform in
+    [0, 10N] with duplicates, unsorted; device-sorted, then every query answered as the
+    host-sorted array's first occurrence."""
+    n, q = 1 << 20, 1 << 16
+    rng = np.random.default_rng(20)
+    data = rng.integers(0, 10 * n + 1, size=n).astype(np.int32)
+    query = rng.integers(0, 10 * n + 1, size=q).astype(np.int32)
+    d_data, d_q = ctx.to_device(data), ctx.to_device(query)
+    rtg.sort_device(ctx, d_data, n, signed=True)
+    ref_sorted = np.sort(data)
+    assert np.array_equal(d_data.numpy(), ref_sorted)
+    out, _ = rtg.lookup(ctx, d_data, n, d_q, q, signed=True)
+    assert np.array_equal(out.numpy(), orc.lookup(ref_sorted, query, signed=True))
+
+
+def test_bs_access_count_function_c_abi(tmp_path):
+    """The C drop-in BSAccessCountFunction (rt_search/simple_bs.h:5) called from a reference-style
+    driver: results against the oracle, `data` sorted in place, and the reference's report lines
+    (simple_bs.cu:88, :95, :137-139)."""
+    import re
+    import shutil
+    import subprocess
+    from conftest import ROOT
+    cxx = shutil.which("g++") or shutil.which("clang++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    rng = np.random.default_rng(77)
+    n, q = 50_000, 20_000
+    data = rng.integers(-5000, 500_000, size=n).astype(np.int32)
+    query = np.concatenate([rng.integers(-6000, 510_000, size=q - 2), [-5000, 499_999]]).astype(np.int32)
+    (tmp_path / "data.bin").write_bytes(data.tobytes())
+    (tmp_path / "query.bin").write_bytes(query.tobytes())
+    src = tmp_path / "driver.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <vector>
+#include "rtg_compat.h"
+int main(int, char **argv) {
+    std::vector<int> data(%d), query(%d), res(%d);
+    FILE *f = std::fopen(argv[1], "rb"); if (std::fread(data.data(), 4, data.size(), f) != data.size()) return 3; std::fclose(f);
+    f = std::fopen(argv[2], "rb"); if (std::fread(query.data(), 4, query.size(), f) != query.size()) return 3; std::fclose(f);
+    BSAccessCountFunction(data.data(), (int)data.size(), query.data(), (int)query.size(), res.data());
+    f = std::fopen(argv[3], "wb"); std::fwrite(res.data(), 4, res.size(), f); std::fwrite(data.data(), 4, data.size(), f); std::fclose(f);
+    return 0;
+}
+''' % (n, q, q))
+    lib = rtg.lib_path()
+    exe = tmp_path / "driver"
+    proc = subprocess.run([cxx, "-std=c++17", "-O1", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+                           f"-L{lib.parent}", "-lrtg", f"-Wl,-rpath,{lib.parent}"], capture_output=True, text=True)
+    assert proc.returncode == 0, proc.stderr
+    run = subprocess.run([str(exe), str(tmp_path / "data.bin"), str(tmp_path / "query.bin"), str(tmp_path / "out.bin")],
+                         capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0, run.stdout + run.stderr
+    lines = run.stdout.strip().splitlines()
+    assert lines[0] == f"data size = {n}"
+    assert re.fullmatch(r"cpu sort time = \d+\.\d{3} ms", lines[1])
+    assert re.fullmatch(r"Sorting time = \d+\.\d{6} ms, Binary search time = \d+\.\d{6} ms, Total time = \d+\.\d{6} ms",
+                        lines[2])
+    out = np.frombuffer((tmp_path / "out.bin").read_bytes(), dtype=np.int32)
+    res, sorted_data = out[:q], out[q:]
+    ref_sorted = np.sort(data)
+    assert np.array_equal(sorted_data, ref_sorted)
+    assert np.array_equal(res, orc.lookup(ref_sorted, query, signed=True))

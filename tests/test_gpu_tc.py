@@ -402,3 +402,81 @@ def test_csr_cache_save_load(ctx, tmp_path):
     s2 = rtg.DeviceGraph.load(ctx, tmp_path / "s.csr", 5)
     with pytest.raises(rtg.RTGError):  # counting needs a degree-oriented graph
         s2.tc_count()
+
+
+def _banded_csr(seed=3, n=20_000):
+    """An oriented-style CSR with locality: vertex u's out-list lies in (u, u + 4000], some
+    owners hold long runs (up to 3000 consecutive ids), so that bitmaps over the owners' value
+    ranges fit LDS and every strategy has owners it applies to in every bucket."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for u in range(n):
+        hi = min(n, u + 4001)
+        if hi <= u + 1:
+            rows.append(np.zeros(0, np.int32))
+            continue
+        if u % 97 == 0:
+            k = min(hi - u - 1, int(rng.integers(1000, 3000)))
+        elif u % 13 == 0:
+            k = min(hi - u - 1, int(rng.integers(150, 600)))
+        else:
+            k = min(hi - u - 1, int(rng.integers(0, 60)))
+        rows.append(np.sort(rng.choice(np.arange(u + 1, hi), size=k, replace=False)).astype(np.int32))
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum([r.size for r in rows])
+    return off, np.concatenate(rows).astype(np.int32)
+
+
+STRATEGY_TUNES = ["strat=hhhs", "strat=bbbb", "strat=mmmm", "strat=ssss", "strat=bmsm", "strat=mbhb"]
+
+
+@pytest.mark.parametrize("strat", STRATEGY_TUNES)
+@pytest.mark.parametrize("graph", ["banded", "hub", "rmat_s16", "zero_heavy"])
+def test_bucket_strategies(ctx, monkeypatch, strat, graph):
+    """Every per-bucket strategy (h: LDS cuckoo hash, b: LDS bitmap over the owner's value
+    range, m: the owner's sorted list in LDS merged with each partner list, s: binary search in
+    HBM), forced for all buckets and mixed: counts and per-edge counts against the oracle, with
+    long partner spans (long=1, loads past a span's end read 0) and without."""
+    if graph == "banded":
+        off, adj = _banded_csr()
+    elif graph == "zero_heavy":
+        off, adj = _zero_heavy_csr()
+    else:
+        off, adj, _ = oracle_csr(graph)
+    t = orc.tc_count_intersect(off, adj)
+    exp = orc.tc_per_edge(off, adj, 0)
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    for long_min in (1, 128):
+        monkeypatch.setenv("RTG_TC_TUNE", f"threads=256,unroll=4,item=65536,wave_item=4096,long={long_min},grab=16,{strat}")
+        p = g.tc_plan(method=rtg.TC_BATCHED)
+        d = p.describe()
+        assert d["bucket_strategy"].count("/") == 3 and d["strat"] == strat.split("=")[1]
+        if graph == "banded" and strat in ("strat=bbbb", "strat=mmmm"):
+            assert d["all_strategies"] == "1", d  # the LDS bitmap / sorted list is really used
+        assert p.run()[0] == t, (long_min, d)
+        p.close()
+        p = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=False, per_edge=True)
+        out, cnt, _ = p.run_per_edge()
+        assert cnt == t and np.array_equal(out.numpy()[: adj.size], exp), (long_min, strat)
+        out.free()
+        p.close()
+
+
+def test_autotuned_bucket_strategies(ctx, tmp_path):
+    """The autotuner picks a strategy per owner bucket and the cache records it; a plan from the
+    cache uses the same strategies and gives the oracle's count."""
+    off, adj = _banded_csr()
+    t = orc.tc_count_intersect(off, adj)
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    cache = tmp_path / "tune.json"
+    p1 = g.tc_plan(method=rtg.TC_BATCHED, autotune=True, tune_cache=str(cache))
+    d1 = p1.describe()
+    assert d1["source"] == "autotune" and len(d1["strat"]) == 4
+    assert p1.run()[0] == t
+    assert "strat=" in cache.read_text()
+    p2 = g.tc_plan(method=rtg.TC_BATCHED, autotune=True, tune_cache=str(cache))
+    d2 = p2.describe()
+    assert d2["source"] == "cache" and d2["bucket_strategy"] == d1["bucket_strategy"]
+    assert p2.run()[0] == t
+    st = p2.plan_stats
+    assert st["nbuckets"] == 4 and len(st["strategy"]) == 4
