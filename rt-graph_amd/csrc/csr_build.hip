@@ -32,11 +32,32 @@ struct MinMax {
     int32_t mx;
 };
 
+// Smallest and largest id: four ids per 16-byte load (the pair buffer is 16-byte aligned and
+// holds an even count), the odd tail of n % 4 ids by the first threads.
 __global__ void minmax_kernel(const int32_t *__restrict__ v, int64_t n, MinMax *out) {
     int32_t mn = INT32_MAX, mx = INT32_MIN;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        int32_t x = v[i];
+    const int64_t n4 = n / 4;
+    const int4 *v4 = reinterpret_cast<const int4 *>(v);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {  // four independent 16-byte loads in flight
+        int4 q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = v4[i + k * stride];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            mn = min(mn, min(min(q[k].x, q[k].y), min(q[k].z, q[k].w)));
+            mx = max(mx, max(max(q[k].x, q[k].y), max(q[k].z, q[k].w)));
+        }
+    }
+    for (; i < n4; i += stride) {
+        const int4 q = v4[i];
+        mn = min(mn, min(min(q.x, q.y), min(q.z, q.w)));
+        mx = max(mx, max(max(q.x, q.y), max(q.z, q.w)));
+    }
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n - 4 * n4) {
+        const int32_t x = v[4 * n4 + t];
         mn = min(mn, x);
         mx = max(mx, x);
     }
@@ -51,6 +72,7 @@ __global__ void minmax_kernel(const int32_t *__restrict__ v, int64_t n, MinMax *
     }
 }
 
+// Raw degree by one device-scope atomic per endpoint (id domains too large to partition).
 __global__ void raw_degree(const int32_t *__restrict__ v, int64_t n, uint32_t *__restrict__ deg) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -85,22 +107,59 @@ __global__ __launch_bounds__(kT) void part_hist(const int32_t *__restrict__ v, i
     for (int p = threadIdx.x; p < nparts; p += kT) hist[(size_t)p * ntiles + blockIdx.x] = h[p];
 }
 
-// Scatters the tile's endpoints to their partitions' runs (starts = scanned histogram); the
-// order inside a run is arbitrary, which a histogram does not mind.
+// Scatters the tile's endpoints to their partitions' runs (starts = scanned histogram): the tile
+// is first grouped by partition in LDS (ranks from LDS atomics, then the tile's own bin offsets),
+// so that each partition's share of the tile is written out as one contiguous run. The order
+// inside a run is arbitrary, which a histogram does not mind.
 __global__ __launch_bounds__(kT) void part_scatter(const int32_t *__restrict__ v, int64_t n, int32_t nparts,
                                                    const uint32_t *__restrict__ starts, uint32_t ntiles,
                                                    int32_t *__restrict__ out) {
-    __shared__ uint32_t cur[kMaxPartitions];
-    for (int p = threadIdx.x; p < nparts; p += kT) cur[p] = starts[(size_t)p * ntiles + blockIdx.x];
+    __shared__ uint32_t cnt[kMaxPartitions];     // the tile's count per partition, then its offset
+    __shared__ uint32_t gstart[kMaxPartitions];  // where the tile's run of each partition starts in out
+    __shared__ uint32_t wsum[kT / 64];
+    __shared__ int32_t keys[kPartTile];
+    constexpr int kBinsPerThread = kMaxPartitions / kT;
+    for (int p = threadIdx.x; p < kMaxPartitions; p += kT) cnt[p] = 0u;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kPartTile;
-#pragma unroll 8
+    int32_t x[kPartItems];
+    uint32_t rank[kPartItems];
+#pragma unroll
     for (int i = 0; i < kPartItems; ++i) {
         const int64_t idx = base + (int64_t)i * kT + threadIdx.x;
-        if (idx < n) {
-            const int32_t x = v[idx];
-            out[atomicAdd(&cur[(uint32_t)x >> kPartBits], 1u)] = x;
-        }
+        x[i] = idx < n ? v[idx] : -1;
+        rank[i] = x[i] >= 0 ? atomicAdd(&cnt[(uint32_t)x[i] >> kPartBits], 1u) : 0u;
+    }
+    __syncthreads();
+    // exclusive scan of the tile's counts: kBinsPerThread consecutive bins per thread
+    uint32_t c[kBinsPerThread], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kBinsPerThread; ++k) {
+        c[k] = cnt[threadIdx.x * kBinsPerThread + k];
+        sum += c[k];
+    }
+    const uint32_t incl = rtgd::wave_inclusive_scan(sum);
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint32_t off = incl - sum;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) off += wsum[w];
+#pragma unroll
+    for (int k = 0; k < kBinsPerThread; ++k) {
+        const int p = threadIdx.x * kBinsPerThread + k;
+        cnt[p] = off;
+        if (p < nparts) gstart[p] = starts[(size_t)p * ntiles + blockIdx.x];
+        off += c[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPartItems; ++i)
+        if (x[i] >= 0) keys[cnt[(uint32_t)x[i] >> kPartBits] + rank[i]] = x[i];
+    __syncthreads();
+    const int32_t count = (int32_t)min<int64_t>(kPartTile, n - base);
+    for (int32_t j = threadIdx.x; j < count; j += kT) {
+        const int32_t y = keys[j];
+        const uint32_t p = (uint32_t)y >> kPartBits;
+        out[gstart[p] + ((uint32_t)j - cnt[p])] = y;
     }
 }
 
@@ -209,10 +268,9 @@ __global__ __launch_bounds__(kPackThreads) void orient_pack(const int32_t *__res
     for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < np; t0 += (int64_t)gridDim.x * tile) {
         uint64_t k[kPer];
         uint32_t mask = 0;
-        const int64_t b = t0 + (int64_t)threadIdx.x * kPer;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            const int64_t idx = b + i;
+            const int64_t idx = t0 + (int64_t)i * kPackThreads + threadIdx.x;  // coalesced across the block
             k[i] = 0;
             if (idx < np) {
                 const int2 pr = reinterpret_cast<const int2 *>(pairs)[idx];
@@ -242,10 +300,9 @@ __global__ __launch_bounds__(kPackThreads) void symmetric_pack(const int32_t *__
     for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < np; t0 += (int64_t)gridDim.x * tile) {
         uint64_t k[2 * kPer];
         uint32_t mask = 0;
-        const int64_t b = t0 + (int64_t)threadIdx.x * kPer;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            const int64_t idx = b + i;
+            const int64_t idx = t0 + (int64_t)i * kPackThreads + threadIdx.x;  // coalesced across the block
             k[2 * i] = k[2 * i + 1] = 0;
             if (idx < np) {
                 const int2 pr = reinterpret_cast<const int2 *>(pairs)[idx];

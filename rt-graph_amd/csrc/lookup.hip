@@ -310,8 +310,9 @@ int build_levels(rtg_lookup_index *ix) {
 }
 }  // namespace
 
-int lookup_index_build(rtg_ctx *ctx, const void *d_data, int64_t n, int is_signed, rtg_lookup_index **out) {
-    auto *ix = new rtg_lookup_index();
+namespace {
+// Level sizes and offsets of the index over n entries; returns its element count.
+int64_t index_layout(rtg_lookup_index *ix, rtg_ctx *ctx, const void *d_data, int64_t n, int is_signed) {
     ix->ctx = ctx;
     ix->device = ctx->device;
     ix->is_signed = is_signed;
@@ -332,6 +333,23 @@ int lookup_index_build(rtg_ctx *ctx, const void *d_data, int64_t n, int is_signe
         total += kIndexFan;
     }
     ix->bytes = 4 * total;
+    return total;
+}
+}  // namespace
+
+// A transient index for one stateless call: its levels go into the context's grow-only scratch
+// (no allocation once the scratch is large enough), the build is only enqueued, and nothing is
+// freed afterwards (ix->d_levels is not owned).
+int lookup_index_transient(rtg_ctx *ctx, const void *d_data, int64_t n, int is_signed, rtg_lookup_index *ix) {
+    const int64_t total = index_layout(ix, ctx, d_data, n, is_signed);
+    if (total == 0) return RTG_OK;
+    RTG_TRY(ctx_scratch(ctx, (size_t)ix->bytes, &ix->d_levels));
+    return is_signed ? build_levels<int32_t>(ix) : build_levels<uint32_t>(ix);
+}
+
+int lookup_index_build(rtg_ctx *ctx, const void *d_data, int64_t n, int is_signed, rtg_lookup_index **out) {
+    auto *ix = new rtg_lookup_index();
+    const int64_t total = index_layout(ix, ctx, d_data, n, is_signed);
     int rc = RTG_OK;
     if (total > 0) {
         if (hipMalloc(&ix->d_levels, (size_t)ix->bytes) != hipSuccess) {
@@ -372,18 +390,13 @@ int lookup_api(rtg_ctx *ctx, const T *d_data, int64_t n, const T *d_keys, int64_
     // A transient search index pays off when the array fits the LDS top level (no levels to
     // build) or when the keys are many enough to amortise one pass over the array (measured on
     // 2^28 uint32: build about 0.26 ms; the two forms break even near q = n/256); the build is
-    // inside the timed region.
+    // inside the timed region. Its levels live in the context's scratch, so repeated calls
+    // allocate nothing.
     const bool aligned = ((uintptr_t)d_data & 15) == 0;
     if (aligned && q > 0 && (n <= rtg::kIndexTop || q >= n / 128)) {
-        rtg_lookup_index *ix = nullptr;
-        RTG_TRY(rtg::lookup_index_build(ctx, d_data, n, std::is_signed<T>::value ? 1 : 0, &ix));
-        const int rc = rtg::lookup_index_launch<T>(ix, ctx->stream, d_keys, q, d_out);
-        if (rc == RTG_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) {
-            rtg::lookup_index_free(ix);
-            return rtg_fail(RTG_EHIP, "rtg_lookup: indexed lookup failed");
-        }
-        rtg::lookup_index_free(ix);
-        RTG_TRY(rc);
+        rtg_lookup_index ix;
+        RTG_TRY(rtg::lookup_index_transient(ctx, d_data, n, std::is_signed<T>::value ? 1 : 0, &ix));
+        RTG_TRY(rtg::lookup_index_launch<T>(&ix, ctx->stream, d_keys, q, d_out));
     } else {
         RTG_TRY(rtg::lookup_launch<T>(ctx, ctx->stream, d_data, n, d_keys, q, d_out));
     }

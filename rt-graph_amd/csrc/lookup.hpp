@@ -38,5 +38,8 @@ template <class T>
 int lookup_index_launch(const rtg_lookup_index *ix, hipStream_t st, const T *d_keys, int64_t q, int32_t *d_out);
 // Builds the index (allocates, enqueues on the context stream and synchronises).
 int lookup_index_build(rtg_ctx *ctx, const void *d_data, int64_t n, int is_signed, rtg_lookup_index **out);
+// Enqueues the build of a one-call index whose levels live in the context's scratch (valid until
+// the next call that uses the scratch; caller holds ctx->mu).
+int lookup_index_transient(rtg_ctx *ctx, const void *d_data, int64_t n, int is_signed, rtg_lookup_index *ix);
 void lookup_index_free(rtg_lookup_index *ix);
 }  // namespace rtg

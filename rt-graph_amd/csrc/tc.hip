@@ -688,14 +688,14 @@ void launch_batched(BatchKernel k, int grid, int threads, size_t lds, hipStream_
 // ---- planning kernels ----
 
 // Per edge of the range [e_begin, e_end): the byte models, and a grouping key — the owner
-// (the endpoint with the longer out-list) in the low `bits`, the edge's index in the range
-// above. `bits` is a whole number of bytes: the radix sort orders by 8-bit digits, so the
+// (the endpoint with the longer out-list) in the low `bits`, above them the partner (the other
+// endpoint) or, for per-edge plans (with_edge), the edge's index in the range. `bits` is a whole number of bytes: the radix sort orders by 8-bit digits, so the
 // owner field must fill its last digit (edge bits inside it would split an owner's edges).
 // Edges with an empty list get owner n, so that a radix sort on the low bits (stable) lists
 // every owner's edges together, in edge order, and those edges last.
 __global__ __launch_bounds__(256) void plan_keys(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
                                                  const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
-                                                 int bits, int32_t n, uint64_t *__restrict__ keys,
+                                                 int bits, int32_t n, bool with_edge, uint64_t *__restrict__ keys,
                                                  unsigned long long *bytes /* [0] alg, [1] merge, [2] edges */) {
     uint64_t alg = 0, mrg = 0, ne = 0;
     for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
@@ -711,7 +711,8 @@ __global__ __launch_bounds__(256) void plan_keys(const int32_t *__restrict__ off
             owner = (du > dv || (du == dv && u < v)) ? u : v;
             ne += 1;
         }
-        keys[e - e_begin] = ((uint64_t)(e - e_begin) << bits) | (uint64_t)(uint32_t)owner;
+        const uint64_t payload = with_edge ? (uint64_t)(e - e_begin) : (uint64_t)(uint32_t)(owner == u ? v : u);
+        keys[e - e_begin] = (payload << bits) | (uint64_t)(uint32_t)owner;
     }
     // one atomic per block and counter
     __shared__ uint64_t part[4][3];
@@ -733,7 +734,9 @@ __global__ __launch_bounds__(256) void plan_keys(const int32_t *__restrict__ off
 
 // Position i of the owner-sorted keys (the first nw are work edges): the partner's out-list
 // span {start, length} (and the edge, for per-edge plans), its length for the work sums, and
-// each owner's first position.
+// each owner's first position. Keys of counting plans carry the partner itself, so only the
+// partner's row offsets are gathered; per-edge plans' keys carry the edge, whose endpoints are
+// read back.
 __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
                            const int32_t *__restrict__ row, int64_t e_begin, const uint64_t *__restrict__ sorted,
                            int64_t nw, int bits, int2 *__restrict__ wspan, uint32_t *__restrict__ wedge,
@@ -742,12 +745,17 @@ __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__res
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = sorted[i];
         const int32_t owner = (int32_t)(key & mask);
-        const int64_t e = e_begin + (int64_t)(key >> bits);
-        const int32_t u = row[e], v = adj[e];
-        const int32_t partner = owner == u ? v : u;
+        int32_t partner;
+        if (wedge) {
+            const int64_t e = e_begin + (int64_t)(key >> bits);
+            const int32_t u = row[e], v = adj[e];
+            partner = owner == u ? v : u;
+            wedge[i] = (uint32_t)e;
+        } else {
+            partner = (int32_t)(key >> bits);
+        }
         const int32_t ps = off[partner], len = off[partner + 1] - ps;
         wspan[i] = make_int2(ps, len);
-        if (wedge) wedge[i] = (uint32_t)e;
         lens[i] = (uint64_t)len;
         if (i == 0 || (int32_t)(sorted[i - 1] & mask) != owner) first[owner] = (uint32_t)i;
     }
@@ -892,7 +900,8 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     const int64_t ne = p->e_end - p->e_begin;
     // owner ids 0 .. n, in whole bytes (the radix sort's digits)
     const int bits = (std::max(1, 32 - __builtin_clz((uint32_t)n + 1u)) + 7) / 8 * 8;
-    if (ne >= ((int64_t)1 << (63 - bits))) return set_error(RTG_EINVAL, "batched TC: edge range too large for the plan keys");
+    if (ne >= ((int64_t)1 << (63 - bits)) || bits > 32)
+        return set_error(RTG_EINVAL, "batched TC: edge range too large for the plan keys");
     const size_t kn = (size_t)std::max<int64_t>(ne, 1);
     const size_t bytes = Arena::round(sizeof(uint32_t) * ((size_t)n + 1)) * 2 + Arena::round(64) +
                          Arena::round(sizeof(uint64_t) * kn) * 4 + Arena::round(radix_sort_work_bytes((int64_t)kn)) +
@@ -917,7 +926,7 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     int which = 0;
     if (ne > 0) {
         plan_keys<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end, bits,
-                                                             n, keys[0], d_bytes);
+                                                             n, p->opts.per_edge != 0, keys[0], d_bytes);
         HIP_TRY(hipGetLastError());
         int passes = 0;
         RTG_TRY(radix_sort_u64(ctx, keys, ne, bits, &which, &passes, sort_work));
@@ -982,6 +991,7 @@ struct QueueParams {
     int32_t wave_owner_max, fit_block, max_table, wave_table;
     int32_t only_bucket;             // -1: every owner; else only this bucket's (autotuning)
     int64_t item_work;
+    int64_t wave_item_work;          // a small owner is heavy when its work reaches this
     int32_t strat[kBuckets];
 };
 
@@ -1023,10 +1033,13 @@ __device__ __forceinline__ OwnerPlan owner_plan(const QueueParams &q, int32_t x)
 
 // Per owner: block items (nblk) or small-owner slot (nsmall), and the sums acc[0] staged bytes,
 // acc[1 + b] items of bucket b (owners for bucket 0), acc[5] search items, acc[6] items with a
-// bitmap or sorted-list table (including small owners whose wave mode is one of them).
+// bitmap or sorted-list table (including small owners whose wave mode is one of them), acc[7]
+// heavy small owners (order key at or above that of wave_item_work: the first acc[7] of the
+// sorted small owners).
 __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *__restrict__ nblk,
                                                       uint32_t *__restrict__ nsmall, unsigned long long *acc) {
-    uint64_t v[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint64_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t heavy_key = cost_key((uint64_t)q.wave_item_work);
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < q.n; x += (int64_t)gridDim.x * blockDim.x) {
         const OwnerPlan o = owner_plan(q, (int32_t)x);
         uint32_t nb = 0, ns = 0;
@@ -1037,6 +1050,7 @@ __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *_
                 v[0] += 4ull + 8ull + 4ull * (uint64_t)o.d;  // small-list entry, partner offsets, owner list
                 v[1] += 1;
                 v[6] += q.strat[0] == kStratBitmap || q.strat[0] == kStratMerge ? 1 : 0;
+                v[7] += cost_key(o.w + (uint64_t)o.d) <= heavy_key ? 1 : 0;
             } else {
                 nb = (uint32_t)o.chunks;
                 v[0] += (uint64_t)o.chunks * (16ull + 4ull * (uint64_t)o.d);  // queue entry, owner list per item
@@ -1048,14 +1062,14 @@ __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *_
         nblk[x] = nb;
         nsmall[x] = ns;
     }
-    __shared__ uint64_t part[4][7];
+    __shared__ uint64_t part[4][8];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
+    for (int k = 0; k < 8; ++k) {
         const uint64_t t = rtgd::wave_sum_u64(v[k]);
         if (__lane_id() == 0) part[threadIdx.x >> 6][k] = t;
     }
     __syncthreads();
-    if (threadIdx.x < 7) {
+    if (threadIdx.x < 8) {
         uint64_t t = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w][threadIdx.x];
         if (t) atomicAdd(&acc[threadIdx.x], (unsigned long long)t);
@@ -1107,17 +1121,22 @@ __global__ void queue_gather_small(const uint64_t *__restrict__ keys, int64_t n,
     }
 }
 
-// Wave items: consecutive small owners (heaviest first) grouped by
-// g(i) = min(P(i) / target, i / waves), P the exclusive prefix of their work — a group holds
-// `waves` owners when they are heavy and about `target` work when they are light. flag[i] = 1
-// where a group starts.
-__global__ void queue_group_flags(const uint64_t *__restrict__ pw, int64_t n, uint64_t target, int waves,
+// Wave items: consecutive small owners (heaviest first) in groups with at least `waves` owners
+// and about `target` = waves * wave_item_work of work, as a greedy pass would make them, but
+// in parallel: the first i0 owners (the heavy ones, each with at least wave_item_work) go
+// `waves` to a group; the light ones after them are cut where the prefix of their work,
+// counted from i0, crosses a multiple of target (so each light group also has more than `waves`
+// owners). P is the exclusive prefix of the work. flag[i] = 1 where a group starts.
+__device__ __forceinline__ uint64_t wave_group_of(const uint64_t *__restrict__ pw, int64_t i, int64_t i0, uint64_t target,
+                                                  int waves) {
+    if (i < i0) return (uint64_t)i / (uint64_t)waves;
+    return ((uint64_t)i0 + (uint64_t)waves - 1) / (uint64_t)waves + (pw[i] - pw[i0]) / target;
+}
+
+__global__ void queue_group_flags(const uint64_t *__restrict__ pw, int64_t n, int64_t i0, uint64_t target, int waves,
                                   uint32_t *__restrict__ flag) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t gi = min(pw[i] / target, (uint64_t)i / (uint64_t)waves);
-        const uint64_t gp = i ? min(pw[i - 1] / target, (uint64_t)(i - 1) / (uint64_t)waves) : ~0ull;
-        flag[i] = (i == 0 || gi != gp) ? 1u : 0u;
-    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flag[i] = (i == 0 || wave_group_of(pw, i, i0, target, waves) != wave_group_of(pw, i - 1, i0, target, waves)) ? 1u : 0u;
 }
 
 __global__ void queue_group_starts(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ gid, int64_t n,
@@ -1182,7 +1201,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     const int waves = threads / 64;
     const TableSizes ts = table_sizes(tune, g->max_row);
     QueueParams q{g->d_offsets, g->d_adjs, p->d_wofs, p->d_work, n, ts.wave_owner_max, ts.fit_block, ts.max_table,
-                  ts.wave_table, only_bucket, tune.item_work, {}};
+                  ts.wave_table, only_bucket, tune.item_work, tune.wave_item_work, {}};
     for (int b = 0; b < kBuckets; ++b) q.strat[b] = tune.strat[b];
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->d_small) (void)hipFree(p->d_small);
@@ -1200,7 +1219,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     Arena a1{buf1, 0, b1};
     uint32_t *nblk = a1.take<uint32_t>(nn), *nsmall = a1.take<uint32_t>(nn);
     uint32_t *blk_off = a1.take<uint32_t>(nn), *small_off = a1.take<uint32_t>(nn);
-    unsigned long long *acc = a1.take<unsigned long long>(16);  // [0..6] sums, [8] totals (two u32)
+    unsigned long long *acc = a1.take<unsigned long long>(16);  // [0..7] sums, [8] totals (two u32)
     void *scan1 = a1.take<char>(scan_work_bytes((int64_t)nn));
     HIP_TRY(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * 16, st));
     uint32_t *totals = reinterpret_cast<uint32_t *>(acc + 8);
@@ -1214,6 +1233,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     HIP_TRY(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     const int64_t nb = (int64_t)(uint32_t)(h[8] & 0xFFFFFFFFull), ns = (int64_t)(uint32_t)(h[8] >> 32);
+    const int64_t heavy_small = std::min<int64_t>((int64_t)h[7], ns);
     p->staged_bytes = h[0] + 16ull * (uint64_t)ns;  // a wave item per small owner at most
     for (int b = 0; b < kBuckets; ++b) p->bucket_items[b] = (int64_t)h[1 + b];
     p->n_search_items = (int64_t)h[5];
@@ -1263,7 +1283,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u64_w(ctx, sw, swork, ns, nullptr, scan2));  // exclusive work prefix, into swork
         const uint64_t target = (uint64_t)std::max<int64_t>(1, tune.wave_item_work * waves);
-        queue_group_flags<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(swork, ns, target, waves, flag);
+        queue_group_flags<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(swork, ns, heavy_small, target, waves, flag);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u32_w(ctx, flag, gid, ns, ngroups, scan2));
         queue_group_starts<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(flag, gid, ns, starts);
@@ -1335,7 +1355,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 
 // Revision of the batched kernel's code, part of the cache key: raised when a change to the
 // kernel can change which candidate is fastest, so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 3;
+constexpr int kTuneRevision = 4;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts.
