@@ -109,23 +109,35 @@ struct IndexArgs {
     int32_t top;
 };
 
-// Level L+1 entry i = last entry of the i-th run of 32 entries of level L (its subtree's
-// maximum); entries past the real ones hold the type's maximum.
+// Every level and the tail in one launch: entry i of level L >= 1 is the largest value of its
+// subtree, data[min((i + 1) * 32^L - 1, n - 1)] (the last element of the subtree's run of the
+// data), or the type's maximum past the real entries; the tail is the data's last, partial run
+// of 32, padded. One launch for every level (the levels are not built from each other).
 template <class T>
-__global__ __launch_bounds__(256) void index_build_level(const T *__restrict__ src, int64_t src_len,
-                                                         T *__restrict__ dst, int64_t dst_padded) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= dst_padded) return;
-    const int64_t nodes = (src_len + rtg::kIndexFan - 1) / rtg::kIndexFan;
-    const int64_t j = std::min<int64_t>(i * rtg::kIndexFan + rtg::kIndexFan - 1, src_len - 1);
-    dst[i] = i < nodes ? src[j] : type_max<T>();
-}
+struct LevelLayout {
+    int64_t off[rtg_lookup_index::kMaxLevels + 1];
+    int64_t len[rtg_lookup_index::kMaxLevels + 1];
+    int64_t padded[rtg_lookup_index::kMaxLevels + 1];
+    int64_t span[rtg_lookup_index::kMaxLevels + 1];  // 32^L
+    int64_t off_tail;
+    int32_t top;
+};
 
 template <class T>
-__global__ __launch_bounds__(64) void index_build_tail(const T *__restrict__ data, int64_t n, T *__restrict__ tail) {
-    const int64_t b = n / rtg::kIndexFan * rtg::kIndexFan;
-    const int k = threadIdx.x;
-    if (k < rtg::kIndexFan) tail[k] = b + k < n ? data[b + k] : type_max<T>();
+__global__ __launch_bounds__(256) void index_build_all(const T *__restrict__ data, int64_t n, LevelLayout<T> lay,
+                                                       T *__restrict__ levels) {
+    const int64_t total = lay.off_tail + rtg::kIndexFan;
+    for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < total; g += (int64_t)gridDim.x * 256) {
+        if (g >= lay.off_tail) {
+            const int64_t b = n / rtg::kIndexFan * rtg::kIndexFan, k = g - lay.off_tail;
+            levels[g] = b + k < n ? data[b + k] : type_max<T>();
+            continue;
+        }
+        int L = 1;
+        while (L < lay.top && g >= lay.off[L + 1]) ++L;
+        const int64_t i = g - lay.off[L];
+        levels[g] = i < lay.len[L] ? data[min((i + 1) * lay.span[L] - 1, n - 1)] : type_max<T>();
+    }
 }
 
 // One node: 32 consecutive entries from a 16-byte aligned address, as 8 independent loads.
@@ -294,18 +306,23 @@ template int lookup_index_launch<int32_t>(const rtg_lookup_index *, hipStream_t,
 namespace {
 template <class T>
 int build_levels(rtg_lookup_index *ix) {
+    if (ix->top == 0) return RTG_OK;
+    LevelLayout<T> lay{};
+    int64_t span = 1;
     for (int L = 1; L <= ix->top; ++L) {
-        const T *src = L == 1 ? static_cast<const T *>(ix->d_data) : static_cast<const T *>(ix->d_levels) + ix->off[L - 1];
-        T *dst = static_cast<T *>(ix->d_levels) + ix->off[L];
-        const int64_t padded = ceil_div(ix->len[L], kIndexFan) * kIndexFan;
-        index_build_level<T><<<(unsigned)ceil_div(padded, 256), 256, 0, ix->ctx->stream>>>(src, ix->len[L - 1], dst, padded);
-        HIP_TRY(hipGetLastError());
+        span *= kIndexFan;
+        lay.off[L] = ix->off[L];
+        lay.len[L] = ix->len[L];
+        lay.padded[L] = ceil_div(ix->len[L], kIndexFan) * kIndexFan;
+        lay.span[L] = span;
     }
-    if (ix->top > 0) {
-        index_build_tail<T><<<1, 64, 0, ix->ctx->stream>>>(static_cast<const T *>(ix->d_data), ix->n,
-                                                           static_cast<T *>(ix->d_levels) + ix->off_tail);
-        HIP_TRY(hipGetLastError());
-    }
+    lay.off_tail = ix->off_tail;
+    lay.top = ix->top;
+    const int64_t total = ix->off_tail + kIndexFan;
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, 256), (int64_t)ix->ctx->cu_count * 16));
+    index_build_all<T><<<grid, 256, 0, ix->ctx->stream>>>(static_cast<const T *>(ix->d_data), ix->n, lay,
+                                                            static_cast<T *>(ix->d_levels));
+    HIP_TRY(hipGetLastError());
     return RTG_OK;
 }
 }  // namespace
