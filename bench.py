@@ -555,11 +555,17 @@ def main() -> int:
     t0 = time.perf_counter()
     dev_pairs = rtg.gen_rmat_device(ctx, scale, nsamples, seed=1)
     log(f"generated {nsamples} R-MAT samples on device in {time.perf_counter() - t0:.2f}s")
+    # The first build of a process also pays the one-time loading of the build's kernels; the
+    # graph that is counted comes from a second build of the same pairs (the steady state).
+    g0 = rtg.DeviceGraph.from_pairs(ctx, dev_pairs)
+    t_build_first = g0.build_stats.t_total_ms
+    g0.close()
     g = rtg.DeviceGraph.from_pairs(ctx, dev_pairs)
     bst = g.build_stats
     dev_pairs.free()
     log(f"CSR build: n={g.n} m={g.m} max_out={g.max_row} device {bst.t_total_ms:.2f} ms "
-        f"(relabel {bst.t_relabel_ms:.2f}, orient {bst.t_orient_ms:.2f}, sort {bst.t_sort_ms:.2f}, csr {bst.t_csr_ms:.2f})")
+        f"(relabel {bst.t_relabel_ms:.2f}, orient {bst.t_orient_ms:.2f}, sort {bst.t_sort_ms:.2f} in "
+        f"{bst.sort_passes} passes, csr {bst.t_csr_ms:.2f}); first build of the process {t_build_first:.2f} ms")
 
     plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=True, autotune=not args.no_autotune,
                      tune_cache=args.tune_cache or None)
@@ -648,7 +654,10 @@ def main() -> int:
                      "measured_copy_gbs": copy_gbs},
         "cpu_baseline": cpu,
         "plain_ms": plain, "batched_over_plain": (plain_ms / t_ms) if plain_ms else None,
-        "t_build_ms": bst.t_total_ms, "build_model_bytes": build_bytes,
+        "t_build_ms": bst.t_total_ms, "t_build_first_ms": t_build_first,
+        "t_build_phases_ms": {"degree_renumber": bst.t_relabel_ms, "orient": bst.t_orient_ms,
+                              "sort": bst.t_sort_ms, "sort_passes": bst.sort_passes, "dedup_csr": bst.t_csr_ms},
+        "build_model_bytes": build_bytes,
         "build_gbs": build_bytes / (bst.t_total_ms * 1e-3) / 1e9, "t_plan_ms": t_plan_first,
         "t_plan_source": desc["source"], "t_plan_again_ms": t_plan_again,
         "t_end_to_end_ms": bst.t_total_ms + t_plan_again + t_ms,

@@ -88,7 +88,7 @@ typedef struct rtg_build_stats {
     double t_sort_ms;       /* radix sort of the packed (src,dst) keys                       */
     double t_csr_ms;        /* dedup, self-loop removal, offsets                             */
     int64_t raw_pairs;      /* input pairs                                                   */
-    int64_t valid_pairs;    /* after degree<=1 removal (before dedup)                        */
+    int64_t valid_pairs;    /* after degree<=1 and self-loop removal (before dedup)          */
     int32_t nodes_in;       /* after relabel                                                 */
     int32_t sort_passes;
 } rtg_build_stats;

@@ -10,11 +10,12 @@
 //             order-preserving renumberings, so the final id of a kept vertex is one exclusive
 //             scan of (degree > 1) over the input ids
 //   orient    src->dst iff deg[src] < deg[dst] or (equal and src < dst) (graph.cpp:116-124),
-//             on the input ids (same order as the relabelled ones), kept pairs packed with
-//             their final ids into 64-bit keys (src << b | dst), b = bits(n')
+//             on the input ids (same order as the relabelled ones), kept non-loop pairs packed
+//             with their final ids into 64-bit keys (src << b | dst), b = bits(n')
 //   sort      stable LSD radix sort on 2b bits (replaces ThrustSortEdgesInBatch)
-//   csr       drop self-loops and repeats of the previous key (graph.cpp:159-173), compact
-//             by exclusive scan, write adjs and the row offsets (graph.cpp:71-102)
+//   csr       drop repeats of the previous key (graph.cpp:159-173): kept keys per 4096-key
+//             tile, a scan of the tile counts, then one pass that ranks the kept keys inside
+//             each tile and writes adjs and the row offsets (graph.cpp:71-102)
 //
 // The output arrays are byte-identical to the reference semantics (checked against the
 // CPU oracle). Without RTG_PAIRS_ORIENT the same passes build the symmetric simple CSR
@@ -22,6 +23,7 @@
 #include "graph.hpp"
 
 #include <cstring>
+#include <memory>
 
 namespace {
 
@@ -33,8 +35,10 @@ struct MinMax {
 };
 
 // Smallest and largest id: four ids per 16-byte load (the pair buffer is 16-byte aligned and
-// holds an even count), the odd tail of n % 4 ids by the first threads.
-__global__ void minmax_kernel(const int32_t *__restrict__ v, int64_t n, MinMax *out) {
+// holds an even count), the odd tail of n % 4 ids by the first threads; one partial per block
+// (no same-address atomics: thousands of waves updating one word serialise), reduced by
+// minmax_final.
+__global__ __launch_bounds__(kT) void minmax_kernel(const int32_t *__restrict__ v, int64_t n, MinMax *__restrict__ part) {
     int32_t mn = INT32_MAX, mx = INT32_MIN;
     const int64_t n4 = n / 4;
     const int4 *v4 = reinterpret_cast<const int4 *>(v);
@@ -66,9 +70,43 @@ __global__ void minmax_kernel(const int32_t *__restrict__ v, int64_t n, MinMax *
         mn = min(mn, __shfl_xor(mn, off, 64));
         mx = max(mx, __shfl_xor(mx, off, 64));
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMin(&out->mn, mn);
-        atomicMax(&out->mx, mx);
+    __shared__ MinMax w[kT / 64];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = MinMax{mn, mx};
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        MinMax r = w[0];
+#pragma unroll
+        for (int k = 1; k < kT / 64; ++k) {
+            r.mn = min(r.mn, w[k].mn);
+            r.mx = max(r.mx, w[k].mx);
+        }
+        part[blockIdx.x] = r;
+    }
+}
+
+// One block: out = the min and max over the nb partials.
+__global__ __launch_bounds__(kT) void minmax_final(const MinMax *__restrict__ part, int nb, MinMax *__restrict__ out) {
+    int32_t mn = INT32_MAX, mx = INT32_MIN;
+    for (int b = threadIdx.x; b < nb; b += kT) {
+        mn = min(mn, part[b].mn);
+        mx = max(mx, part[b].mx);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, __shfl_xor(mn, off, 64));
+        mx = max(mx, __shfl_xor(mx, off, 64));
+    }
+    __shared__ MinMax w[kT / 64];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = MinMax{mn, mx};
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        MinMax r = w[0];
+#pragma unroll
+        for (int k = 1; k < kT / 64; ++k) {
+            r.mn = min(r.mn, w[k].mn);
+            r.mx = max(r.mx, w[k].mx);
+        }
+        *out = r;
     }
 }
 
@@ -98,10 +136,17 @@ __global__ __launch_bounds__(kT) void part_hist(const int32_t *__restrict__ v, i
     for (int p = threadIdx.x; p < nparts; p += kT) h[p] = 0u;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * kPartTile;
-#pragma unroll 8
-    for (int i = 0; i < kPartItems; ++i) {
-        const int64_t idx = base + (int64_t)i * kT + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(uint32_t)v[idx] >> kPartBits], 1u);
+#pragma unroll
+    for (int r = 0; r < kPartItems; r += 8) {  // eight loads in flight, then their LDS atomics
+        int32_t y[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int64_t idx = base + (int64_t)(r + u) * kT + threadIdx.x;
+            y[u] = idx < n ? v[idx] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (y[u] >= 0) atomicAdd(&h[(uint32_t)y[u] >> kPartBits], 1u);
     }
     __syncthreads();
     for (int p = threadIdx.x; p < nparts; p += kT) hist[(size_t)p * ntiles + blockIdx.x] = h[p];
@@ -125,11 +170,12 @@ __global__ __launch_bounds__(kT) void part_scatter(const int32_t *__restrict__ v
     int32_t x[kPartItems];
     uint32_t rank[kPartItems];
 #pragma unroll
-    for (int i = 0; i < kPartItems; ++i) {
+    for (int i = 0; i < kPartItems; ++i) {  // every load issued before the first LDS atomic
         const int64_t idx = base + (int64_t)i * kT + threadIdx.x;
         x[i] = idx < n ? v[idx] : -1;
-        rank[i] = x[i] >= 0 ? atomicAdd(&cnt[(uint32_t)x[i] >> kPartBits], 1u) : 0u;
     }
+#pragma unroll
+    for (int i = 0; i < kPartItems; ++i) rank[i] = x[i] >= 0 ? atomicAdd(&cnt[(uint32_t)x[i] >> kPartBits], 1u) : 0u;
     __syncthreads();
     // exclusive scan of the tile's counts: kBinsPerThread consecutive bins per thread
     uint32_t c[kBinsPerThread], sum = 0;
@@ -192,7 +238,18 @@ __global__ __launch_bounds__(kT) void part_count(const int32_t *__restrict__ gro
         if (r1 <= r0) continue;
         for (int j = threadIdx.x; j < (1 << kPartBits); j += kT) h[j] = 0u;
         __syncthreads();
-        for (int64_t i = r0 + threadIdx.x; i < r1; i += kT) atomicAdd(&h[(uint32_t)grouped[i] & ((1u << kPartBits) - 1u)], 1u);
+        // eight independent loads per round, then their LDS atomics (a load per trip followed
+        // by its atomic would keep one load in flight per thread)
+        constexpr int kU = 8;
+        int64_t i = r0 + threadIdx.x;
+        for (; i + (kU - 1) * kT < r1; i += kU * kT) {
+            int32_t y[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) y[u] = grouped[i + u * kT];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) atomicAdd(&h[(uint32_t)y[u] & ((1u << kPartBits) - 1u)], 1u);
+        }
+        for (; i < r1; i += kT) atomicAdd(&h[(uint32_t)grouped[i] & ((1u << kPartBits) - 1u)], 1u);
         __syncthreads();
         uint32_t *dst = deg + ((size_t)p << kPartBits);
         for (int j = threadIdx.x; j < (1 << kPartBits); j += kT) {
@@ -207,7 +264,7 @@ __global__ __launch_bounds__(kT) void part_count(const int32_t *__restrict__ gro
 // scan of keep; keep = degree > 1 when orienting (the relabel and the degree<=1 removal
 // of graph.cpp:40-62, 126-152 in one renumbering: both are order-preserving), else degree > 0
 // (relabel only). present counts ids of degree > 0 (the relabelled vertex count).
-__global__ void keep_flags(const uint32_t *__restrict__ deg, int64_t n, uint32_t min_deg, uint32_t *__restrict__ keep,
+__global__ __launch_bounds__(kT) void keep_flags(const uint32_t *__restrict__ deg, int64_t n, uint32_t min_deg, uint32_t *__restrict__ keep,
                            unsigned long long *present) {
     uint64_t cnt = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -216,7 +273,15 @@ __global__ void keep_flags(const uint32_t *__restrict__ deg, int64_t n, uint32_t
         cnt += d > 0 ? 1u : 0u;
     }
     cnt = rtgd::wave_sum_u64(cnt);
-    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(present, (unsigned long long)cnt);
+    __shared__ unsigned long long w[kT / 64];  // one device atomic per block, not per wave
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = 0;
+#pragma unroll
+        for (int k = 0; k < kT / 64; ++k) c += w[k];
+        if (c) atomicAdd(present, c);
+    }
 }
 
 __global__ void vertex_info(const uint32_t *__restrict__ deg, const uint32_t *__restrict__ newid, int64_t n,
@@ -260,32 +325,44 @@ __device__ __forceinline__ void block_append(const uint64_t (&keys)[kMax], uint3
 // Orientation, removal and key packing (TCPreprocessing) on the input's own ids: vinfo[x] =
 // {raw degree, final id}. The orientation compares (degree, id) — the relabel preserves order,
 // so the input ids order ties as the relabelled ones do (graph.cpp:116-124) — and the kept
-// endpoints (degree > 1) are renumbered by their final ids (graph.cpp:126-152).
+// endpoints (degree > 1) are renumbered by their final ids (graph.cpp:126-152). Self-loops,
+// which the reference drops with the duplicates (graph.cpp:159-173), are dropped here, so the
+// sorted keys hold none.
 __global__ __launch_bounds__(kPackThreads) void orient_pack(const int32_t *__restrict__ pairs, int64_t np,
                                                             const uint2 *__restrict__ vinfo, int bits,
                                                             uint64_t *__restrict__ keys, unsigned long long *counter) {
     const int64_t tile = (int64_t)kPackThreads * kPer;
     for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < np; t0 += (int64_t)gridDim.x * tile) {
+        // Three phases, so that the loads of all kPer items are in flight together: the pairs,
+        // then the 2 * kPer vertex gathers, then the orientation (an item past the end reads
+        // pair 0 and is masked off).
+        int2 pr[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int64_t idx = t0 + (int64_t)i * kPackThreads + threadIdx.x;  // coalesced across the block
+            pr[i] = reinterpret_cast<const int2 *>(pairs)[idx < np ? idx : 0];
+        }
+        uint2 vs[kPer], vd[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            vs[i] = vinfo[pr[i].x];
+            vd[i] = vinfo[pr[i].y];
+        }
         uint64_t k[kPer];
         uint32_t mask = 0;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            const int64_t idx = t0 + (int64_t)i * kPackThreads + threadIdx.x;  // coalesced across the block
-            k[i] = 0;
-            if (idx < np) {
-                const int2 pr = reinterpret_cast<const int2 *>(pairs)[idx];
-                int32_t s = pr.x, d = pr.y;
-                uint2 is = vinfo[s], id = vinfo[d];
-                if (is.x > id.x || (is.x == id.x && s > d)) {
-                    const uint2 t = is;
-                    is = id;
-                    id = t;
-                }
-                if (is.x > 1u && id.x > 1u) {
-                    k[i] = ((uint64_t)is.y << bits) | (uint64_t)id.y;
-                    mask |= 1u << i;
-                }
+            const int64_t idx = t0 + (int64_t)i * kPackThreads + threadIdx.x;
+            const int32_t s = pr[i].x, d = pr[i].y;
+            uint2 is = vs[i], id = vd[i];
+            if (is.x > id.x || (is.x == id.x && s > d)) {
+                const uint2 t = is;
+                is = id;
+                id = t;
             }
+            k[i] = ((uint64_t)is.y << bits) | (uint64_t)id.y;
+            if (idx < np && s != d && is.x > 1u && id.x > 1u)  // a self-loop counts in the degree only
+                mask |= 1u << i;
         }
         block_append<kPer>(k, mask, keys, counter);
     }
@@ -298,63 +375,113 @@ __global__ __launch_bounds__(kPackThreads) void symmetric_pack(const int32_t *__
                                                                uint64_t *__restrict__ keys, unsigned long long *counter) {
     const int64_t tile = (int64_t)kPackThreads * kPer;
     for (int64_t t0 = (int64_t)blockIdx.x * tile; t0 < np; t0 += (int64_t)gridDim.x * tile) {
+        // pairs first, then the id gathers of all items (an item past the end reads pair 0 and is
+        // masked off), so that each phase's loads are in flight together
+        int2 pr[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int64_t idx = t0 + (int64_t)i * kPackThreads + threadIdx.x;  // coalesced across the block
+            pr[i] = reinterpret_cast<const int2 *>(pairs)[idx < np ? idx : 0];
+        }
         uint64_t k[2 * kPer];
         uint32_t mask = 0;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            const int64_t idx = t0 + (int64_t)i * kPackThreads + threadIdx.x;  // coalesced across the block
-            k[2 * i] = k[2 * i + 1] = 0;
-            if (idx < np) {
-                const int2 pr = reinterpret_cast<const int2 *>(pairs)[idx];
-                if (pr.x != pr.y) {
-                    const uint64_t s = newid ? newid[pr.x] : (uint32_t)pr.x, d = newid ? newid[pr.y] : (uint32_t)pr.y;
-                    k[2 * i] = (s << bits) | d;
-                    k[2 * i + 1] = (d << bits) | s;
-                    mask |= 3u << (2 * i);
-                }
-            }
+            const int64_t idx = t0 + (int64_t)i * kPackThreads + threadIdx.x;
+            const uint64_t s = newid ? newid[pr[i].x] : (uint32_t)pr[i].x, d = newid ? newid[pr[i].y] : (uint32_t)pr[i].y;
+            k[2 * i] = (s << bits) | d;
+            k[2 * i + 1] = (d << bits) | s;
+            if (idx < np && pr[i].x != pr[i].y) mask |= 3u << (2 * i);
         }
         block_append<2 * kPer>(k, mask, keys, counter);
     }
 }
 
-__global__ void dedup_flags(const uint64_t *__restrict__ keys, int64_t n, int bits,
-                            uint32_t *__restrict__ flag) {
-    const uint64_t lo = (1ull << bits) - 1;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = keys[i];
-        const bool loop = (k >> bits) == (k & lo);
-        const bool repeat = i > 0 && keys[i - 1] == k;
-        flag[i] = (!loop && !repeat) ? 1u : 0u;
+// ---- dedup + CSR (graph.cpp:159-173, 71-102): the keys are sorted and hold no self-loops, so
+// key i is kept iff it differs from key i - 1 ----
+constexpr int kDedupItems = 16;
+constexpr int64_t kDedupTile = (int64_t)kT * kDedupItems;  // 4096 keys: rounds of kT consecutive keys
+static_assert(kDedupItems * (kT / 64) == 64, "one wave scans the tile's (round, wave) counts");
+
+// Kept keys per tile.
+__global__ __launch_bounds__(kT) void dedup_count(const uint64_t *__restrict__ keys, int64_t n,
+                                                  uint32_t *__restrict__ tile_cnt) {
+    const int64_t base = (int64_t)blockIdx.x * kDedupTile;
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < kDedupItems; ++j) {
+        const int64_t i = base + (int64_t)j * kT + threadIdx.x;
+        if (i < n) c += (i == 0 || keys[i - 1] != keys[i]) ? 1u : 0u;
+    }
+    c = rtgd::wave_sum_u32(c);
+    __shared__ uint32_t w[kT / 64];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < kT / 64; ++k) t += w[k];
+        tile_cnt[blockIdx.x] = t;
     }
 }
 
-__global__ void dedup_scatter(const uint64_t *__restrict__ keys, int64_t n, int bits,
-                              const uint32_t *__restrict__ flag, const uint32_t *__restrict__ pos,
-                              int32_t *__restrict__ src_c, int32_t *__restrict__ adjs) {
-    const uint64_t lo = (1ull << bits) - 1;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        if (flag[i]) {
-            const uint64_t k = keys[i];
-            const uint32_t p = pos[i];
-            src_c[p] = (int32_t)(k >> bits);
-            adjs[p] = (int32_t)(k & lo);
+// Writes each kept key's column to adjs[p], p = tile_off[tile] + its rank among the tile's kept
+// keys (a ballot per wave and round, one wave scan over the tile's 64 (round, wave) counts), and
+// the row offsets: a kept key at p in row s, whose predecessor key is in row ps < s, sets
+// offsets[ps + 1 .. s] = p (the first key: offsets[0 .. s] = 0); the last key, in row s, sets
+// offsets[s + 1 .. nrows] = m, the kept total.
+__global__ __launch_bounds__(kT) void dedup_write(const uint64_t *__restrict__ keys, int64_t n, int bits,
+                                                  const uint32_t *__restrict__ tile_off,
+                                                  const uint32_t *__restrict__ d_m, int32_t nrows,
+                                                  int32_t *__restrict__ adjs, int32_t *__restrict__ offsets) {
+    __shared__ uint32_t cell[64];  // kept keys per (round, wave), then their exclusive prefix
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kDedupTile;
+    uint64_t k[kDedupItems];
+    int32_t prow[kDedupItems];  // row of the preceding key (-1 before the first)
+    uint64_t bal[kDedupItems];
+#pragma unroll
+    for (int j = 0; j < kDedupItems; ++j) {
+        const int64_t i = base + (int64_t)j * kT + threadIdx.x;
+        bool keep = false;
+        k[j] = 0;
+        prow[j] = -1;
+        if (i < n) {
+            k[j] = keys[i];
+            if (i > 0) {
+                const uint64_t q = keys[i - 1];
+                keep = q != k[j];
+                prow[j] = (int32_t)(q >> bits);
+            } else {
+                keep = true;
+            }
         }
+        bal[j] = __ballot(keep);
+        if (lane == 0) cell[j * (kT / 64) + wave] = (uint32_t)__popcll(bal[j]);
     }
-}
-
-// offsets[u] = first compacted index whose source is >= u (rows are sorted by source).
-__global__ void fill_offsets(const int32_t *__restrict__ src_c, int64_t m, int32_t n,
-                             int32_t *__restrict__ offsets) {
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < m;
-         j += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t s = src_c[j];
-        const int32_t p = j == 0 ? -1 : src_c[j - 1];
-        for (int32_t u = p + 1; u <= s; ++u) offsets[u] = (int32_t)j;
-        if (j == m - 1)
-            for (int32_t u = s + 1; u <= n; ++u) offsets[u] = (int32_t)m;
+    __syncthreads();
+    if (wave == 0) {  // cells in key order: round-major, then wave
+        const uint32_t v = cell[lane];
+        const uint32_t inc = rtgd::wave_inclusive_scan(v);
+        cell[lane] = inc - v;
+    }
+    __syncthreads();
+    const uint32_t t0 = tile_off[blockIdx.x];
+    const uint64_t lt = rtgd::lanemask_lt();
+    const uint64_t lo = (1ull << bits) - 1;
+#pragma unroll
+    for (int j = 0; j < kDedupItems; ++j) {
+        const int64_t i = base + (int64_t)j * kT + threadIdx.x;
+        const int32_t row = (int32_t)(k[j] >> bits);
+        if ((bal[j] >> lane) & 1ull) {
+            const uint32_t p = t0 + cell[j * (kT / 64) + wave] + (uint32_t)__popcll(bal[j] & lt);
+            adjs[p] = (int32_t)(k[j] & lo);
+            for (int32_t u = prow[j] + 1; u <= row; ++u) offsets[u] = (int32_t)p;
+        }
+        if (i == n - 1) {
+            const int32_t m = (int32_t)*d_m;
+            for (int32_t u = row + 1; u <= nrows; ++u) offsets[u] = m;
+        }
     }
 }
 
@@ -364,14 +491,21 @@ __global__ void fill_i32(int32_t *p, int64_t n, int32_t v) {
         p[i] = v;
 }
 
-__global__ void max_row_kernel(const int32_t *__restrict__ offsets, int32_t n, int32_t *out) {
+__global__ __launch_bounds__(kT) void max_row_kernel(const int32_t *__restrict__ offsets, int32_t n, int32_t *out) {
     int32_t mx = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         mx = max(mx, offsets[i + 1] - offsets[i]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, mx);
+    __shared__ int32_t w[kT / 64];  // one device atomic per block, not per wave
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 1; k < kT / 64; ++k) mx = max(mx, w[k]);
+        atomicMax(out, max(mx, w[0]));
+    }
 }
 
 // CSR validation for rtg_graph_from_csr: err bit 1 = offsets not monotone / bad bounds,
@@ -398,9 +532,10 @@ __global__ void validate_csr(const int32_t *__restrict__ offsets, const int32_t 
     }
 }
 
-unsigned grid_for(rtg_ctx *ctx, int64_t n) {
+// Blocks of kT threads for n items, at most per_cu per CU (grid-stride loops take the rest).
+unsigned grid_for(rtg_ctx *ctx, int64_t n, int per_cu = 16) {
     int64_t want = rtg::ceil_div(n, kT);
-    int64_t cap = (int64_t)ctx->cu_count * 16;
+    int64_t cap = (int64_t)ctx->cu_count * per_cu;
     if (want > cap) want = cap;
     if (want < 1) want = 1;
     return (unsigned)want;
@@ -430,7 +565,7 @@ int csr_max_row(rtg_ctx *ctx, const int32_t *d_offsets, int32_t n, int32_t *out)
     RTG_TRY(ctx_scratch(ctx, 256, &s));
     int32_t *d = (int32_t *)s;
     HIP_TRY(hipMemsetAsync(d, 0, sizeof(int32_t), ctx->stream));
-    max_row_kernel<<<grid_for(ctx, n), kT, 0, ctx->stream>>>(d_offsets, n, d);
+    max_row_kernel<<<grid_for(ctx, n, 4), kT, 0, ctx->stream>>>(d_offsets, n, d);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, d, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -463,10 +598,27 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
     hipEvent_t *ev = ctx->ev;
     const int64_t nv = 2 * npairs;  // endpoint count
 
-    // Working copy of the pairs.
-    DevBuf pbuf;
+    // Working copy of the pairs, and the key-sized buffers (sized by the pair count, allocated
+    // before the first synchronisation so that no allocation waits on the device mid-build).
+    DevBuf pbuf, kbuf;
     HIP_TRY(hipMalloc(&pbuf.p, sizeof(int32_t) * (size_t)std::max<int64_t>(nv, 1)));
     int32_t *d_pairs = (int32_t *)pbuf.p;
+    const int64_t kcap = std::max<int64_t>(max_keys, 1);
+    const int64_t ktiles = ceil_div(kcap, kDedupTile);
+    const int mm_blocks = (int)grid_for(ctx, std::max<int64_t>(nv, 1));
+    const size_t kbytes = Arena::round(8 * (size_t)kcap) * 2 + Arena::round(radix_sort_work_bytes(kcap)) +
+                          Arena::round(4 * (size_t)ktiles) + Arena::round(scan_work_bytes(ktiles)) +
+                          Arena::round(sizeof(MinMax) * (size_t)(mm_blocks + 1)) + 4096;
+    HIP_TRY(hipMalloc(&kbuf.p, kbytes));
+    Arena ka{(char *)kbuf.p, 0, kbytes};
+    uint64_t *keys0 = ka.take<uint64_t>(kcap);
+    uint64_t *keys1 = ka.take<uint64_t>(kcap);
+    void *sort_work = ka.take<char>(radix_sort_work_bytes(kcap));
+    uint32_t *tile_off = ka.take<uint32_t>(ktiles);
+    void *kscan = ka.take<char>(scan_work_bytes(ktiles));
+    MinMax *mm_part = ka.take<MinMax>((size_t)mm_blocks + 1);
+    unsigned long long *counter = ka.take<unsigned long long>(4);
+    uint32_t *d_m = ka.take<uint32_t>(4);
     if (npairs > 0)
         HIP_TRY(hipMemcpyAsync(d_pairs, pairs, sizeof(int32_t) * (size_t)nv,
                                (flags & RTG_PAIRS_DEVICE) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
@@ -475,13 +627,11 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
     // ---- domain, raw degree, renumbering (graph.cpp:40-62, 109-114, 126-152) ----
     MinMax mm{0, -1};
     if (npairs > 0) {
-        void *s = nullptr;
-        RTG_TRY(ctx_scratch(ctx, 256, &s));
-        MinMax init{INT32_MAX, INT32_MIN};
-        HIP_TRY(hipMemcpyAsync(s, &init, sizeof(init), hipMemcpyHostToDevice, st));
-        minmax_kernel<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, nv, (MinMax *)s);
+        minmax_kernel<<<mm_blocks, kT, 0, st>>>(d_pairs, nv, mm_part);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(&mm, s, sizeof(mm), hipMemcpyDeviceToHost, st));
+        minmax_final<<<1, kT, 0, st>>>(mm_part, mm_blocks, mm_part + mm_blocks);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&mm, mm_part + mm_blocks, sizeof(mm), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (mm.mn < 0) return rtg_fail(RTG_EINVAL, "rtg_graph_from_pairs: negative vertex id");
     }
@@ -531,7 +681,7 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
             raw_degree<<<grid_for(ctx, nv), kT, 0, st>>>(d_pairs, nv, deg);
             HIP_TRY(hipGetLastError());
         }
-        keep_flags<<<grid_for(ctx, domain), kT, 0, st>>>(deg, domain, orient ? 2u : 1u, keep, d_small);
+        keep_flags<<<grid_for(ctx, domain, 4), kT, 0, st>>>(deg, domain, orient ? 2u : 1u, keep, d_small);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u32_w(ctx, keep, newid, domain, reinterpret_cast<uint32_t *>(d_small + 1), vscan));
         if (orient) {
@@ -546,25 +696,19 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
         n_out = orient ? kept : present;
     }
     HIP_TRY(hipEventRecord(ev[1], st));
+    auto *g = new rtg_graph();
+    g->ctx = ctx;
+    g->n = (int32_t)n_out;
+    g->oriented = orient;
+    if (hipMalloc(&g->d_offsets, sizeof(int32_t) * (size_t)(n_out + 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        rtg_graph_destroy(g);
+        return rtg_fail(RTG_ENOMEM, "rtg_graph_from_pairs: CSR allocation failed");
+    }
+    std::unique_ptr<rtg_graph, int (*)(rtg_graph *)> gguard(g, rtg_graph_destroy);
 
     // ---- orientation and key packing (graph.cpp:116-124, 126-152), or both directions ----
     int bits = bits_for(std::max<int64_t>(n_out, 2));
-    DevBuf kbuf;
-    const int64_t kcap = std::max<int64_t>(max_keys, 1);
-    const size_t kbytes = Arena::round(8 * (size_t)kcap) * 2 + radix_sort_work_bytes(kcap) +
-                          Arena::round(4 * (size_t)kcap) * 4 + scan_work_bytes(kcap) + 4096;
-    HIP_TRY(hipMalloc(&kbuf.p, kbytes));
-    Arena ka{(char *)kbuf.p, 0, kbytes};
-    uint64_t *keys0 = ka.take<uint64_t>(kcap);
-    uint64_t *keys1 = ka.take<uint64_t>(kcap);
-    void *sort_work = ka.take<char>(radix_sort_work_bytes(kcap));
-    uint32_t *flag = ka.take<uint32_t>(kcap);
-    uint32_t *pos = ka.take<uint32_t>(kcap);
-    int32_t *src_c = ka.take<int32_t>(kcap);
-    unsigned long long *counter = ka.take<unsigned long long>(4);
-    uint32_t *d_m = ka.take<uint32_t>(4);
-    void *kscan = ka.take<char>(scan_work_bytes(kcap));
-
     HIP_TRY(hipMemsetAsync(counter, 0, sizeof(unsigned long long), st));
     if (npairs > 0) {
         const unsigned pgrid = (unsigned)std::min<int64_t>(ceil_div(npairs, (int64_t)kPackThreads * kPer), (int64_t)ctx->cu_count * 8);
@@ -578,6 +722,13 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
     HIP_TRY(hipMemcpyAsync(&nkeys, counter, sizeof(nkeys), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipEventRecord(ev[2], st));
+    // The columns, sized by the keys before deduplication (an upper bound of m: the count of
+    // distinct keys is only known after the sort, and sizing to it would stall the build on a
+    // synchronisation and an allocation).
+    if (hipMalloc(&g->d_adjs, sizeof(int32_t) * (size_t)std::max<unsigned long long>(nkeys, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        return rtg_fail(RTG_ENOMEM, "rtg_graph_from_pairs: CSR allocation failed");
+    }
 
     // ---- sort (thrust_helper.cu:16-42) ----
     uint64_t *kk[2] = {keys0, keys1};
@@ -587,38 +738,25 @@ RTG_API int rtg_graph_from_pairs(rtg_ctx *ctx, const int32_t *pairs, int64_t npa
     HIP_TRY(hipEventRecord(ev[3], st));
 
     // ---- dedup + CSR (graph.cpp:159-173, 71-102) ----
-    auto *g = new rtg_graph();
-    g->ctx = ctx;
-    g->n = (int32_t)n_out;
-    g->oriented = orient;
     uint32_t m = 0;
     if (nkeys > 0) {
-        dedup_flags<<<grid_for(ctx, nkeys), kT, 0, st>>>(sorted, nkeys, bits, flag);
+        const int64_t tiles = ceil_div((int64_t)nkeys, kDedupTile);
+        dedup_count<<<(unsigned)tiles, kT, 0, st>>>(sorted, (int64_t)nkeys, tile_off);
         HIP_TRY(hipGetLastError());
-        RTG_TRY(scan_exclusive_u32_w(ctx, flag, pos, nkeys, d_m, kscan));
+        RTG_TRY(scan_exclusive_u32_w(ctx, tile_off, tile_off, tiles, d_m, kscan));
+        dedup_write<<<(unsigned)tiles, kT, 0, st>>>(sorted, (int64_t)nkeys, bits, tile_off, d_m, (int32_t)n_out,
+                                                    g->d_adjs, g->d_offsets);
+        HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(&m, d_m, 4, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-    }
-    g->m = m;
-    if (hipMalloc(&g->d_offsets, sizeof(int32_t) * (size_t)(n_out + 1)) != hipSuccess ||
-        hipMalloc(&g->d_adjs, sizeof(int32_t) * (size_t)std::max<uint32_t>(m, 1)) != hipSuccess) {
-        rtg_graph_destroy(g);
-        return rtg_fail(RTG_ENOMEM, "rtg_graph_from_pairs: CSR allocation failed");
-    }
-    if (m > 0) {
-        dedup_scatter<<<grid_for(ctx, nkeys), kT, 0, st>>>(sorted, nkeys, bits, flag, pos, src_c, g->d_adjs);
-        fill_offsets<<<grid_for(ctx, m), kT, 0, st>>>(src_c, m, (int32_t)n_out, g->d_offsets);
     } else {
         fill_i32<<<grid_for(ctx, n_out + 1), kT, 0, st>>>(g->d_offsets, n_out + 1, 0);
+        HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[4], st));
     HIP_TRY(hipStreamSynchronize(st));
-    int st2 = csr_max_row(ctx, g->d_offsets, g->n, &g->max_row);
-    if (st2 != RTG_OK) {
-        rtg_graph_destroy(g);
-        return st2;
-    }
+    g->m = m;
+    RTG_TRY(csr_max_row(ctx, g->d_offsets, g->n, &g->max_row));
+    gguard.release();
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->t_relabel_ms = elapsed_ms(ev[0], ev[1]);

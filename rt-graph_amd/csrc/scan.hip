@@ -75,13 +75,24 @@ __global__ __launch_bounds__(kThreads) void tile_scan(const T *__restrict__ in, 
         }
         __syncthreads();  // lds is reused by the tile's own scan
     }
-    const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+    // Coalesced (striped) loads into LDS, each thread scans kItems consecutive values read back
+    // from LDS, writes the results there, and the tile leaves with coalesced stores. One pad
+    // element per 16 keeps the consecutive-item reads on distinct banks.
+    __shared__ T tile[kTile + kTile / 16];
+    const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const int j = i * kThreads + threadIdx.x;
+        const int64_t idx = base + j;
+        tile[j + (j >> 4)] = idx < n ? in[idx] : (T)0;
+    }
+    __syncthreads();
     T v[kItems];
     T local = 0;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        int64_t idx = base + i;
-        v[i] = idx < n ? in[idx] : (T)0;
+        const int j = threadIdx.x * kItems + i;
+        v[i] = tile[j + (j >> 4)];
         local += v[i];
     }
     T tile_total;
@@ -92,9 +103,16 @@ __global__ __launch_bounds__(kThreads) void tile_scan(const T *__restrict__ in, 
     run += off;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
-        int64_t idx = base + i;
-        if (idx < n) out[idx] = run;
+        const int j = threadIdx.x * kItems + i;
+        tile[j + (j >> 4)] = run;
         run += v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const int j = i * kThreads + threadIdx.x;
+        const int64_t idx = base + j;
+        if (idx < n) out[idx] = tile[j + (j >> 4)];
     }
     if (d_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *d_total = off + tile_total;
 }

@@ -34,6 +34,9 @@ def _graphs():
                           dtype=np.int32).reshape(-1)
     e = rng.integers(0, 3000, size=(60000, 2)).astype(np.int32)  # duplicates and self-loops
     g["gnm_dups"] = e.reshape(-1)
+    # self-loops count in the degree but leave no key; every pair here is a loop or touches a
+    # degree-1 vertex, so the build has vertices (0 and 3) and no edge
+    g["loops_only"] = np.array([0, 0, 0, 0, 1, 2, 3, 3, 3, 3, 3, 4], dtype=np.int32)
     g["ba_100000"] = rtg.gen_ba(100_000, 5, 1)
     g["rmat_s12"] = rtg.gen_rmat(12, 16 << 12, seed=1)
     g["rmat_s16"] = rtg.gen_rmat(16, 44 << 16, seed=1)
