@@ -39,10 +39,24 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int kBufferWord3 = 0x00020000;
 constexpr int kPlainThreads = 256;
 
-__global__ void edge_rows_kernel(const int32_t *__restrict__ off, int32_t n, int32_t *__restrict__ row) {
-    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
-         u += (int64_t)gridDim.x * blockDim.x)
-        for (int32_t e = off[u]; e < off[u + 1]; ++e) row[e] = (int32_t)u;
+// row[e] = source vertex of edge e. A block takes 256 consecutive vertices, whose edges form one
+// contiguous range, and writes that range with coalesced stores: each thread walks its strided
+// edges in order and advances a cursor over the block's row offsets (in LDS).
+__global__ __launch_bounds__(256) void edge_rows_kernel(const int32_t *__restrict__ off, int32_t n,
+                                                        int32_t *__restrict__ row) {
+    __shared__ int32_t so[257];
+    for (int64_t v0 = (int64_t)blockIdx.x * 256; v0 < n; v0 += (int64_t)gridDim.x * 256) {
+        const int32_t nv = (int32_t)min<int64_t>(256, n - v0);
+        for (int i = threadIdx.x; i <= nv; i += 256) so[i] = off[v0 + i];
+        __syncthreads();
+        const int32_t e1 = so[nv];
+        int32_t v = 0;  // so[v] <= e < so[v + 1]
+        for (int32_t e = so[0] + threadIdx.x; e < e1; e += 256) {
+            while (so[v + 1] <= e) ++v;
+            row[e] = (int32_t)v0 + v;
+        }
+        __syncthreads();  // so is refilled for the next vertex range
+    }
 }
 
 __global__ void reset_counters(unsigned long long *c) {
@@ -693,26 +707,51 @@ void launch_batched(BatchKernel k, int grid, int threads, size_t lds, hipStream_
 // owner field must fill its last digit (edge bits inside it would split an owner's edges).
 // Edges with an empty list get owner n, so that a radix sort on the low bits (stable) lists
 // every owner's edges together, in edge order, and those edges last.
+constexpr int kPlanU = 4;  // independent items per thread and round in the plan kernels
+
 __global__ __launch_bounds__(256) void plan_keys(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
                                                  const int32_t *__restrict__ row, int64_t e_begin, int64_t e_end,
                                                  int bits, int32_t n, bool with_edge, uint64_t *__restrict__ keys,
                                                  unsigned long long *bytes /* [0] alg, [1] merge, [2] edges */) {
     uint64_t alg = 0, mrg = 0, ne = 0;
-    for (int64_t e = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < e_end;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t u = row[e], v = adj[e];
-        const uint32_t du = (uint32_t)(off[u + 1] - off[u]), dv = (uint32_t)(off[v + 1] - off[v]);
-        const uint32_t s = min(du, dv), l = max(du, dv);
-        const uint32_t lg = l ? 32u - __clz(l) : 0u;  // ceil(log2(l+1))
-        alg += 24ull + 4ull * min((uint64_t)s + l, (uint64_t)s * (1u + lg));
-        mrg += 24ull + 4ull * ((uint64_t)du + dv);
-        int32_t owner = n;
-        if (s > 0) {
-            owner = (du > dv || (du == dv && u < v)) ? u : v;
-            ne += 1;
+    // kPlanU edges per thread and round, loads in phases (endpoints, then their row offsets), so
+    // that a round keeps kPlanU independent chains in flight; a slot past the end reads edge
+    // e_begin and is dropped.
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e0 = e_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e0 < e_end; e0 += stride * kPlanU) {
+        int32_t u[kPlanU], v[kPlanU];
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            const int64_t e = e0 + k * stride;
+            const int64_t ec = e < e_end ? e : e_begin;
+            u[k] = row[ec];
+            v[k] = adj[ec];
         }
-        const uint64_t payload = with_edge ? (uint64_t)(e - e_begin) : (uint64_t)(uint32_t)(owner == u ? v : u);
-        keys[e - e_begin] = (payload << bits) | (uint64_t)(uint32_t)owner;
+        int32_t u0[kPlanU], u1[kPlanU], v0[kPlanU], v1[kPlanU];
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            u0[k] = off[u[k]];
+            u1[k] = off[u[k] + 1];
+            v0[k] = off[v[k]];
+            v1[k] = off[v[k] + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            const int64_t e = e0 + k * stride;
+            if (e >= e_end) break;
+            const uint32_t du = (uint32_t)(u1[k] - u0[k]), dv = (uint32_t)(v1[k] - v0[k]);
+            const uint32_t s = min(du, dv), l = max(du, dv);
+            const uint32_t lg = l ? 32u - __clz(l) : 0u;  // ceil(log2(l+1))
+            alg += 24ull + 4ull * min((uint64_t)s + l, (uint64_t)s * (1u + lg));
+            mrg += 24ull + 4ull * ((uint64_t)du + dv);
+            int32_t owner = n;
+            if (s > 0) {
+                owner = (du > dv || (du == dv && u[k] < v[k])) ? u[k] : v[k];
+                ne += 1;
+            }
+            const uint64_t payload = with_edge ? (uint64_t)(e - e_begin) : (uint64_t)(uint32_t)(owner == u[k] ? v[k] : u[k]);
+            keys[e - e_begin] = (payload << bits) | (uint64_t)(uint32_t)owner;
+        }
     }
     // one atomic per block and counter
     __shared__ uint64_t part[4][3];
@@ -742,22 +781,54 @@ __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__res
                            int64_t nw, int bits, int2 *__restrict__ wspan, uint32_t *__restrict__ wedge,
                            uint64_t *__restrict__ lens, uint32_t *__restrict__ first) {
     const uint64_t mask = (1ull << bits) - 1;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t key = sorted[i];
-        const int32_t owner = (int32_t)(key & mask);
-        int32_t partner;
-        if (wedge) {
-            const int64_t e = e_begin + (int64_t)(key >> bits);
-            const int32_t u = row[e], v = adj[e];
-            partner = owner == u ? v : u;
-            wedge[i] = (uint32_t)e;
-        } else {
-            partner = (int32_t)(key >> bits);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // kPlanU positions per thread and round, loads in phases (keys, then the edges' endpoints
+    // for per-edge plans, then the partners' row offsets); a slot past the end reads position 0.
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < nw; i0 += stride * kPlanU) {
+        uint64_t key[kPlanU], prev[kPlanU];
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            const int64_t i = i0 + k * stride;
+            const int64_t ic = i < nw ? i : 0;
+            key[k] = sorted[ic];
+            prev[k] = ic > 0 ? sorted[ic - 1] : ~0ull;
         }
-        const int32_t ps = off[partner], len = off[partner + 1] - ps;
-        wspan[i] = make_int2(ps, len);
-        lens[i] = (uint64_t)len;
-        if (i == 0 || (int32_t)(sorted[i - 1] & mask) != owner) first[owner] = (uint32_t)i;
+        int32_t partner[kPlanU];
+        int64_t edge[kPlanU];
+        if (wedge) {
+            int32_t eu[kPlanU], ev[kPlanU];
+#pragma unroll
+            for (int k = 0; k < kPlanU; ++k) {
+                edge[k] = e_begin + (int64_t)(key[k] >> bits);
+                eu[k] = row[edge[k]];
+                ev[k] = adj[edge[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < kPlanU; ++k) partner[k] = (int32_t)(key[k] & mask) == eu[k] ? ev[k] : eu[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < kPlanU; ++k) {
+                partner[k] = (int32_t)(key[k] >> bits);
+                edge[k] = 0;
+            }
+        }
+        int32_t p0[kPlanU], p1[kPlanU];
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            p0[k] = off[partner[k]];
+            p1[k] = off[partner[k] + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            const int64_t i = i0 + k * stride;
+            if (i >= nw) break;
+            const int32_t owner = (int32_t)(key[k] & mask);
+            const int32_t len = p1[k] - p0[k];
+            if (wedge) wedge[i] = (uint32_t)edge[k];
+            wspan[i] = make_int2(p0[k], len);
+            lens[i] = (uint64_t)len;
+            if (i == 0 || (int32_t)(prev[k] & mask) != owner) first[owner] = (uint32_t)i;
+        }
     }
 }
 
@@ -767,12 +838,25 @@ __global__ void plan_runs(const uint64_t *__restrict__ sorted, int64_t nw, int b
                           const uint64_t *__restrict__ lens, const uint64_t *__restrict__ lens_ex, uint32_t *__restrict__ cnt,
                           unsigned long long *__restrict__ work) {
     const uint64_t mask = (1ull << bits) - 1;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t owner = (int32_t)(sorted[i] & mask);
-        if (i + 1 < nw && (int32_t)(sorted[i + 1] & mask) == owner) continue;
-        const uint32_t f = first[owner];
-        cnt[owner] = (uint32_t)(i + 1 - f);
-        work[owner] = (unsigned long long)(lens_ex[i] + lens[i] - lens_ex[f]);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < nw; i0 += stride * kPlanU) {
+        uint64_t key[kPlanU], next[kPlanU];  // kPlanU positions' loads in flight together
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            const int64_t i = i0 + k * stride;
+            const int64_t ic = i < nw ? i : 0;
+            key[k] = sorted[ic];
+            next[k] = ic + 1 < nw ? sorted[ic + 1] : ~0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            const int64_t i = i0 + k * stride;
+            const int32_t owner = (int32_t)(key[k] & mask);
+            if (i >= nw || (i + 1 < nw && (int32_t)(next[k] & mask) == owner)) continue;
+            const uint32_t f = first[owner];
+            cnt[owner] = (uint32_t)(i + 1 - f);
+            work[owner] = (unsigned long long)(lens_ex[i] + lens[i] - lens_ex[f]);
+        }
     }
 }
 
