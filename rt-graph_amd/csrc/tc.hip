@@ -1427,6 +1427,11 @@ const char *kCandidates[] = {
     "threads=256,unroll=2,item=131072,wave_item=4096,wave_max=-1,long=32,grab=16",
     "threads=512,unroll=4,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
     "threads=512,unroll=8,item=2097152,wave_item=4096,wave_max=-1,long=128,grab=64",
+    // smaller wave items (more, lighter groups of small owners) with finer queue grabs
+    "threads=256,unroll=4,item=2097152,wave_item=2048,wave_max=64,long=128,grab=32",
+    "threads=256,unroll=4,item=2097152,wave_item=2048,wave_max=128,long=128,grab=32",
+    "threads=256,unroll=4,item=2097152,wave_item=1024,wave_max=64,long=128,grab=32",
+    "threads=256,unroll=8,item=2097152,wave_item=2048,wave_max=64,long=128,grab=32",
 };
 
 // Strategies the autotuner times per bucket (the first is the default).
@@ -1437,9 +1442,10 @@ const int32_t kBucketCandidates[kBuckets][4] = {
     {kStratSearch, kStratMerge, kStratBitmap, -1},
 };
 
-// Revision of the batched kernel's code, part of the cache key: raised when a change to the
-// kernel can change which candidate is fastest, so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 4;
+// Revision of the batched kernel's code and candidates, part of the cache key: raised when a
+// change to the kernel or to the candidate list can change which candidate the autotuner keeps,
+// so that entries tuned earlier are re-tuned.
+constexpr int kTuneRevision = 5;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts.
