@@ -648,7 +648,9 @@ def main() -> int:
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": args.workload, "generator": f"R-MAT scale {scale}, edge factor {ef}, "
                    "(0.57,0.19,0.19,0.05), seed 1", "n": g.n, "m_oriented": g.m, "triangles": cnt,
-                   "method": "batched: owner-grouped two-choice cuckoo LDS tables, persistent work queue, hipGraph",
+                   "method": "batched on the degree-ordered copy: owners' lists as LDS bitmaps (block owners) or "
+                             "two-choice cuckoo tables (small owners), 16-bit partner lists where they fit, "
+                             "persistent work queue, hipGraph",
                    "plan": desc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
@@ -665,7 +667,10 @@ def main() -> int:
         "build_model_bytes": build_bytes,
         "build_gbs": build_bytes / (bst.t_total_ms * 1e-3) / 1e9, "t_plan_ms": t_plan_first,
         "t_plan_source": desc["source"], "t_plan_again_ms": t_plan_again,
-        "t_end_to_end_ms": bst.t_total_ms + t_plan_again + t_ms,
+        # the degree-ordered copy is built once per graph (by the first plan) and shared by later
+        # plans: a fresh graph pays build + copy + plan + count
+        "t_renumber_ms": float(desc.get("renumber_ms", 0.0)),
+        "t_end_to_end_ms": bst.t_total_ms + float(desc.get("renumber_ms", 0.0)) + t_plan_again + t_ms,
         "per_edge": per_edge, "ba_100k": ba_rows, "config_model_3m": config_model,
         "set_intersection": si,
         "wall_s_timed": wall, "alg_bytes": alg, "merge_bytes": st["merge_bytes"],

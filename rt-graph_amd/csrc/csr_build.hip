@@ -575,6 +575,40 @@ int csr_max_row(rtg_ctx *ctx, const int32_t *d_offsets, int32_t n, int32_t *out)
 }  // namespace rtg
 
 namespace rtg {
+
+int id_histogram(rtg_ctx *ctx, const int32_t *ids, int64_t n, int64_t domain, uint32_t *deg) {
+    if (n <= 0) return RTG_OK;
+    hipStream_t st = ctx->stream;
+    const int64_t nparts = ceil_div(std::max<int64_t>(domain, 1), (int64_t)1 << kPartBits);
+    if (nparts > kMaxPartitions) {
+        raw_degree<<<grid_for(ctx, n), kT, 0, st>>>(ids, n, deg);
+        HIP_TRY(hipGetLastError());
+        return RTG_OK;
+    }
+    const uint32_t ntiles = (uint32_t)std::max<int64_t>(1, ceil_div(n, kPartTile));
+    const int64_t nhist = nparts * (int64_t)ntiles;
+    const size_t bytes = Arena::round(4 * (size_t)n) + Arena::round(4 * (size_t)nhist) +
+                         Arena::round(4 * (size_t)(nparts + 1)) + Arena::round(scan_work_bytes(nhist)) + 1024;
+    DevBuf buf;
+    HIP_TRY(hipMalloc(&buf.p, bytes));
+    Arena a{(char *)buf.p, 0, bytes};
+    int32_t *grouped = a.take<int32_t>((size_t)n);
+    uint32_t *hist = a.take<uint32_t>((size_t)nhist);
+    uint32_t *pstart = a.take<uint32_t>((size_t)nparts + 1);
+    void *scan_work = a.take<char>(scan_work_bytes(nhist));
+    part_hist<<<ntiles, kT, 0, st>>>(ids, n, (int32_t)nparts, hist, ntiles);
+    HIP_TRY(hipGetLastError());
+    RTG_TRY(scan_exclusive_u32_w(ctx, hist, hist, nhist, nullptr, scan_work));
+    part_scatter<<<ntiles, kT, 0, st>>>(ids, n, (int32_t)nparts, hist, ntiles, grouped);
+    HIP_TRY(hipGetLastError());
+    part_starts<<<(unsigned)ceil_div(nparts + 1, 256), 256, 0, st>>>(hist, ntiles, (int32_t)nparts, n, pstart);
+    HIP_TRY(hipGetLastError());
+    part_count<<<(unsigned)ceil_div(n, kCountChunk), kT, 0, st>>>(grouped, n, pstart, (int32_t)nparts, deg);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));  // the temporaries are freed on return
+    return RTG_OK;
+}
+
 bool graph_oriented(const rtg_graph *g) { return g && g->oriented; }
 void graph_set_oriented(rtg_graph *g, bool oriented) {
     if (g) g->oriented = oriented;
@@ -855,6 +889,7 @@ RTG_API int rtg_graph_destroy(rtg_graph *g) {
     if (g->ctx) (void)hipSetDevice(g->ctx->device);
     if (g->d_offsets) (void)hipFree(g->d_offsets);
     if (g->d_adjs) (void)hipFree(g->d_adjs);
+    rtg::renum_free(g->renum);
     delete g;
     return RTG_OK;
 }

@@ -3,6 +3,8 @@
 
 #include "device.hpp"
 
+struct rtg_renum;
+
 struct rtg_graph {
     rtg_ctx *ctx = nullptr;
     int32_t n = 0;
@@ -11,9 +13,38 @@ struct rtg_graph {
     bool oriented = true;
     int32_t *d_offsets = nullptr;  // n + 1
     int32_t *d_adjs = nullptr;     // m
+    // Degree-ordered copy for batched counting plans, built by the first plan that asks for it
+    // (under the context lock) and kept until the graph is destroyed.
+    rtg_renum *renum = nullptr;
+};
+
+// An oriented CSR with its vertices renumbered in descending order of total degree (out + in,
+// ties by id): a triangle count does not depend on vertex ids, and in this order every
+// out-neighbour (an endpoint of higher degree) gets a small id, so an out-list holds only ids
+// below its own vertex's id in most rows. Rows are sorted. adj16 holds the low 16 bits of every
+// entry; fit16[v] = 1 when row v is non-empty and its largest entry is below 2^16, so that
+// adj16 holds it exactly.
+struct rtg_renum {
+    int32_t n = 0;
+    int64_t m = 0;
+    int32_t *d_off = nullptr;     // n + 1
+    int32_t *d_adj = nullptr;     // m
+    uint16_t *d_adj16 = nullptr;  // m
+    uint8_t *d_fit16 = nullptr;   // n
+    int64_t rows16 = 0;           // rows that fit 16 bits
+    int64_t entries16 = 0;        // their entries
+    float t_build_ms = 0.f;
 };
 
 namespace rtg {
 // Largest row length of a device CSR (device reduction, synchronous).
 int csr_max_row(rtg_ctx *ctx, const int32_t *d_offsets, int32_t n, int32_t *out);
+// Occurrences of each id in ids[0 .. n) (0 <= id < domain) into deg[0 .. domain), which the
+// caller has zeroed: a partitioned LDS histogram (ids grouped by id >> 14, each group counted
+// in LDS), or device-scope atomics for domains above 2^24 ids. Allocates its own temporaries;
+// enqueued on the context stream.
+int id_histogram(rtg_ctx *ctx, const int32_t *ids, int64_t n, int64_t domain, uint32_t *deg);
+// The graph's degree-ordered copy, built on first use (caller holds ctx->mu).
+int graph_renumbered(rtg_ctx *ctx, const rtg_graph *g, const rtg_renum **out);
+void renum_free(rtg_renum *r);
 }

@@ -219,9 +219,16 @@ __device__ __forceinline__ uint32_t cuckoo_has(const Table &c, uint32_t x) {
     return (uint32_t)(a == x) | (uint32_t)(b == x);
 }
 
+// Words of a bitmap over [lo, hi]: bits 0 .. t = hi - lo + 1, bit t (one past the range) always
+// clear, so that a probe can clamp instead of branching.
+__host__ __device__ __forceinline__ uint32_t bitmap_words(uint32_t lo, uint32_t hi) { return ((hi - lo + 1u) >> 5) + 1u; }
+
+// Values outside [lo, hi] (below lo they wrap to large offsets) are clamped to bit t, which is
+// clear: a branch-free probe whose LDS reads all issue back to back.
 __device__ __forceinline__ uint32_t bitmap_has(const Table &c, uint32_t x) {
-    const uint32_t o = x - c.k1;
-    return o < c.t ? (c.tab[o >> 5] >> (o & 31u)) & 1u : 0u;
+    const uint32_t o = min(x - c.k1, c.t);
+    // bit o of the table: word o >> 5, then the bit field extract's offset is o's low 5 bits
+    return __builtin_amdgcn_ubfe(c.tab[__builtin_amdgcn_ubfe(o, 5u, 27u)], o, 1u);
 }
 
 // First index of tab[lo, lo + len) whose value is not below x (lo + len if none): binary lifting
@@ -288,7 +295,7 @@ template <int kThreads>
 __device__ Table build_block_bitmap(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d) {
     const uint32_t lo = (uint32_t)adj[xb], hi = (uint32_t)adj[xb + d - 1];
     const Table c{tab, hi - lo + 1u, lo, 0u, kBitmap};
-    const uint32_t words = ((hi - lo) >> 5) + 1u;
+    const uint32_t words = bitmap_words(lo, hi);
     for (uint32_t i = threadIdx.x; i < words; i += kThreads) tab[i] = 0u;
     __syncthreads();
     for (int32_t i = threadIdx.x; i < d; i += kThreads) {
@@ -335,7 +342,7 @@ __device__ Table build_wave_other(uint32_t *tab, const int32_t *__restrict__ adj
     const int lane = __lane_id();
     if (mode == kBitmap) {
         const uint32_t lo = (uint32_t)adj[xb], hi = (uint32_t)adj[xb + d - 1];
-        const uint32_t words = ((hi - lo) >> 5) + 1u;
+        const uint32_t words = bitmap_words(lo, hi);
         if (words <= cap) {
             for (uint32_t i = lane; i < words; i += 64) tab[i] = 0u;
             rtgd::wave_sync_lds();
@@ -376,7 +383,7 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
         const int2 sp = wspan[pi];
         uint32_t sh = 0;
         for (int32_t j = __lane_id(); j < sp.y; j += 64) {
-            const int32_t x = adj[sp.x + j];
+            const int32_t x = adj[(sp.x & 0x7FFFFFFF) + j];
             int32_t lo = 0, hi = d;
             while (lo < hi) {
                 const int32_t mid = (lo + hi) >> 1;
@@ -406,11 +413,68 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
 // ascend, so a round of 64*U consecutive elements can only match the owner-list window between
 // the lower bound of its first element and the upper bound of its last (two searches the wave
 // shares), each lane searches only that window, and the next round starts where it ended.
-template <uint32_t kMode, int U, bool kPerEdge>
+// The rounds of one long span base[0 .. l0): U loads of kW-byte elements per lane through a raw
+// buffer resource rebased on the round's first element (scalar work per round; a lane's offsets
+// are loop-invariant plus an immediate) whose range ends with the span, so a lane past the end
+// reads 0 without a memory access and the rounds need no per-element bounds test or branch.
+// Returns the span's hits, including the probes of 0 of the last round's loads past the end;
+// *jb_end is where the rounds stopped.
+template <uint32_t kMode, int U, typename T>
+__device__ __forceinline__ uint32_t long_rounds(const T *base, int32_t l0, const Table &c, int32_t *jb_end) {
+    constexpr int kW = (int)sizeof(T);
+    const int lane = __lane_id();
+    int vo = lane * kW;
+    uint32_t h = 0;
+    int32_t jb = 0;
+    uint32_t w0 = 0;  // kSorted: owner-list position the span has reached
+    for (; jb < l0; jb += 64 * U) {
+        // An empty asm that "redefines" vo each round: without it the U offsets vo + u * 64 * kW
+        // are hoisted out of every loop into U registers kept live across the kernel (spilled
+        // at its register budget); with it they fold into the loads' immediate offsets.
+        asm volatile("" : "+v"(vo));
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base + jb), (short)0,
+                                                                            (l0 - jb) * kW, kBufferWord3);
+        uint32_t x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if constexpr (kW == 4)
+                x[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, vo + u * 64 * kW, 0, 0);
+            else
+                x[u] = (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rs, vo + u * 64 * kW, 0, 0);
+        }
+        if constexpr (kMode == kSorted) {
+            const int32_t last = min(64 * U, l0 - jb) - 1;  // the round's last element
+            uint32_t xl = x[0];
+#pragma unroll
+            for (int u = 1; u < U; ++u)
+                if ((last >> 6) == u) xl = x[u];
+            const uint32_t first_v = __builtin_amdgcn_readfirstlane(x[0]);
+            const uint32_t last_v = __builtin_amdgcn_readlane(xl, last & 63);
+            const uint32_t wa = rtgd::uniform_u32(lds_lower_bound(c.tab, w0, c.t - w0, first_v));
+            const uint32_t wb = rtgd::uniform_u32(lds_lower_bound(c.tab, wa, c.t - wa, last_v + 1u));
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t q = lds_lower_bound(c.tab, wa, wb - wa, x[u]);
+                h += (jb + u * 64 + lane < l0 && q < wb && c.tab[min(q, c.t - 1)] == x[u]) ? 1u : 0u;
+            }
+            w0 = wb;
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) h += probe<kMode>(c, x[u]);
+        }
+    }
+    *jb_end = jb;
+    return h;
+}
+
+// kP16: a span whose start has bit 31 set is a row whose entries all fit 16 bits; it is streamed
+// from adj16 (the same positions, 2 bytes each), half the bytes of the 32-bit rows.
+template <uint32_t kMode, int U, bool kPerEdge, bool kP16 = false>
 __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                  int64_t p0, int64_t p1, const Table &c, int32_t long_min,
                                                  int32_t *ex, int32_t *st, const uint32_t *__restrict__ wedge,
-                                                 uint32_t *__restrict__ eout, uint32_t *pe) {
+                                                 uint32_t *__restrict__ eout, uint32_t *pe,
+                                                 const uint16_t *__restrict__ adj16 = nullptr) {
     const int lane = __lane_id();
     const uint64_t lt = rtgd::lanemask_lt();
     uint32_t hits = 0;
@@ -424,47 +488,16 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
         while (lm) {
             const int p = __builtin_ctzll(lm);
             lm &= lm - 1;
-            const int32_t s0 = __builtin_amdgcn_readlane(sp.x, p);
+            const int32_t s0r = __builtin_amdgcn_readlane(sp.x, p);
+            const int32_t s0 = s0r & 0x7FFFFFFF;
             const int32_t l0 = __builtin_amdgcn_readlane(sp.y, p);
             uint32_t sh = 0;  // this span's hits (per-edge counts only)
-            // Raw buffer loads whose range is the span: a lane past its end reads 0 without a
-            // memory access, so the rounds need no per-element bounds test or branch, and the
-            // 2U probes of a round issue back to back.
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                const_cast<int32_t *>(adj + s0), (short)0, l0 * 4, kBufferWord3);
             int32_t jb = 0;
-            uint32_t w0 = 0;  // kSorted: owner-list position the span has reached
-            for (; jb < l0; jb += 64 * U) {
-                const int vo = (jb + lane) * 4;
-                uint32_t x[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) x[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, vo + u * 256, 0, 0);
-                if constexpr (kMode == kSorted) {
-                    const int32_t last = min(64 * U, l0 - jb) - 1;  // the round's last element
-                    uint32_t xl = x[0];
-#pragma unroll
-                    for (int u = 1; u < U; ++u)
-                        if ((last >> 6) == u) xl = x[u];
-                    const uint32_t first_v = __builtin_amdgcn_readfirstlane(x[0]);
-                    const uint32_t last_v = __builtin_amdgcn_readlane(xl, last & 63);
-                    const uint32_t wa = rtgd::uniform_u32(lds_lower_bound(c.tab, w0, c.t - w0, first_v));
-                    const uint32_t wb = rtgd::uniform_u32(lds_lower_bound(c.tab, wa, c.t - wa, last_v + 1u));
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const uint32_t q = lds_lower_bound(c.tab, wa, wb - wa, x[u]);
-                        const uint32_t h = (jb + u * 64 + lane < l0 && q < wb && c.tab[min(q, c.t - 1)] == x[u]) ? 1u : 0u;
-                        if constexpr (kPerEdge) sh += h;
-                        else hits += h;
-                    }
-                    w0 = wb;
-                } else {
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        if constexpr (kPerEdge) sh += probe<kMode>(c, x[u]);
-                        else hits += probe<kMode>(c, x[u]);
-                    }
-                }
-            }
+            uint32_t h;
+            if (kP16 && s0r < 0) h = long_rounds<kMode, U>(adj16 + s0, l0, c, &jb);
+            else h = long_rounds<kMode, U>(adj + s0, l0, c, &jb);
+            if constexpr (kPerEdge) sh += h;
+            else hits += h;
             if constexpr (kMode != kSorted) {
                 // This lane's loads past the end in the last round each probed 0 and hit iff
                 // the owner's list holds 0 (has0): take them back.
@@ -490,7 +523,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             if (is_short) {
                 const int ci = __popcll(sm & lt);
                 ex[ci] = incl - sl;
-                st[ci] = sp.x;
+                st[ci] = sp.x & 0x7FFFFFFF;
                 if constexpr (kPerEdge) pe[ci] = 0u;
             }
             if (lane == 0) ex[ns] = total;
@@ -539,23 +572,47 @@ __device__ __noinline__ uint32_t stream_spans_ool(const int32_t *__restrict__ ad
     return stream_spans<kMode, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
 }
 
+// With 16-bit spans: linear probing (the hash's fallback) and the sorted list, out of line.
+template <bool kP16, int U>
+__device__ __noinline__ uint32_t stream_spans_ool(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
+                                                  int64_t p0, int64_t p1, Table c, int32_t long_min, int32_t *ex,
+                                                  int32_t *st, const uint32_t *__restrict__ wedge,
+                                                  uint32_t *__restrict__ eout, uint32_t *pe,
+                                                  const uint16_t *__restrict__ adj16, uint32_t mode) {
+    if (mode == kLinearProbe)
+        return stream_spans<kLinearProbe, U, false, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+    return stream_spans<kSorted, U, false, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+}
+
 // Partners [p0, p1) against the owner table c, in the probe mode the table was built in. kAll:
-// the bitmap and sorted-list modes are compiled in (else only the hash modes exist).
-template <int U, bool kPerEdge, bool kAll>
+// the bitmap and sorted-list modes are compiled in (else only the hash modes exist). kP16:
+// plans on a degree-ordered copy of the graph (16-bit spans), where the bitmap is the common
+// block strategy: it is compiled inline beside the hash, the others stay out of line.
+template <int U, bool kPerEdge, bool kAll, bool kP16 = false>
 __device__ __forceinline__ uint32_t stream_table(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                  int64_t p0, int64_t p1, const Table &c, int32_t long_min, int32_t *ex,
                                                  int32_t *st, const uint32_t *__restrict__ wedge,
-                                                 uint32_t *__restrict__ eout, uint32_t *pe) {
+                                                 uint32_t *__restrict__ eout, uint32_t *pe,
+                                                 const uint16_t *__restrict__ adj16 = nullptr) {
     const uint32_t mode = rtgd::uniform_u32(c.mode);
-    if (mode == kCuckoo) return stream_spans<kCuckoo, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
-    if (!kAll || mode == kLinearProbe)
-        return stream_spans<kLinearProbe, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
-    if constexpr (kAll) {
+    if constexpr (kP16) {
+        static_assert(kAll && !kPerEdge, "16-bit spans come with every strategy, counting only");
         if (mode == kBitmap)
-            return stream_spans_ool<kBitmap, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
-        return stream_spans_ool<kSorted, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+            return stream_spans<kBitmap, U, false, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+        if (mode == kCuckoo)
+            return stream_spans<kCuckoo, U, false, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+        return stream_spans_ool<kP16, U>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16, mode);
+    } else {
+        if (mode == kCuckoo) return stream_spans<kCuckoo, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+        if (!kAll || mode == kLinearProbe)
+            return stream_spans<kLinearProbe, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+        if constexpr (kAll) {
+            if (mode == kBitmap)
+                return stream_spans_ool<kBitmap, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+            return stream_spans_ool<kSorted, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+        }
+        return 0u;
     }
-    return 0u;
 }
 
 struct BatchArgs {
@@ -573,8 +630,11 @@ struct BatchArgs {
     int32_t grab;               // partners a wave takes per grab in a block item (<= 64)
     const uint32_t *wedge;      // per-edge counts: oriented edge of each span (else null)
     uint32_t *eout;             // per-edge counts: output by oriented edge (else null)
+    const uint16_t *adj16;      // 16-bit copy of adj (degree-ordered plans, else null)
 };
 
+// The 16-bit instantiation (degree-ordered copies) is held to 6 waves per SIMD: at 8 it spills
+// about 60 VGPRs and takes 14.5 ms on R-MAT scale 21, at 6 13.7 ms, at 7 13.9, at 5 14.9.
 // The per-edge instantiation is held to 8 waves per SIMD (at most 64 VGPRs, a few spilled to
 // scratch outside the streaming loops): left alone it takes 74 and runs 6 waves, which costs
 // more than the spills (R-MAT scale 21: 23.7 against 25.3 ms).
@@ -582,8 +642,8 @@ struct BatchArgs {
 // run the instantiation without the bitmap and sorted-list code (the out-of-line calls would
 // hold more registers across the hash path and cost it occupancy); there, bitmap and sorted
 // items would be searched and wave items hashed, which gives the same counts.
-template <int kThreads, int U, bool kPerEdge, bool kAll>
-__global__ __launch_bounds__(kThreads, (kPerEdge || kAll) ? 8 : 1) void tc_batched(BatchArgs a) {
+template <int kThreads, int U, bool kPerEdge, bool kAll, bool kP16 = false>
+__global__ __launch_bounds__(kThreads, kP16 ? 6 : (kPerEdge || kAll) ? 8 : 1) void tc_batched(BatchArgs a) {
     constexpr int kWaves = kThreads / 64;
     extern __shared__ uint32_t table[];
     __shared__ int32_t ex_all[kWaves][65];
@@ -616,7 +676,7 @@ __global__ __launch_bounds__(kThreads, (kPerEdge || kAll) ? 8 : 1) void tc_batch
             // The planner only gives these strategies to owners whose bitmap or list fits the
             // table; anything else is searched in HBM rather than written past it.
             const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
-            const uint32_t need = kind == kItemSorted ? (uint32_t)d : (((uint32_t)(a.adj[xb + d - 1] - a.adj[xb])) >> 5) + 1u;
+            const uint32_t need = kind == kItemSorted ? (uint32_t)d : bitmap_words((uint32_t)a.adj[xb], (uint32_t)a.adj[xb + d - 1]);
             if (!kAll || rtgd::uniform_u32(need) > (uint32_t)a.max_table) kind = kItemSearch;
         }
         if (kind == kItemHash || kind == kItemBitmap || kind == kItemSorted) {
@@ -633,7 +693,8 @@ __global__ __launch_bounds__(kThreads, (kPerEdge || kAll) ? 8 : 1) void tc_batch
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
-                hits += stream_table<U, kPerEdge, kAll>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge, a.eout, pe);
+                hits += stream_table<U, kPerEdge, kAll, kP16>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
+                                                              a.eout, pe, a.adj16);
             }
         } else if (kind == kItemSearch) {
             // Search item: partners [it.b, it.c) are binary-searched in the owner's sorted
@@ -658,8 +719,8 @@ __global__ __launch_bounds__(kThreads, (kPerEdge || kAll) ? 8 : 1) void tc_batch
                 const int32_t xb = a.off[xs], d = a.off[xs + 1] - xb;
                 const Table c = mode == kCuckoo ? build_wave_table(tab, a.adj, xb, d)
                                                 : build_wave_other(tab, a.adj, xb, d, mode, (uint32_t)a.wave_table);
-                hits += stream_table<U, kPerEdge, kAll>(a.adj, a.wspan, a.wofs[xs], a.wofs[xs + 1], c, a.long_min, ex,
-                                                        st, a.wedge, a.eout, pe);
+                hits += stream_table<U, kPerEdge, kAll, kP16>(a.adj, a.wspan, a.wofs[xs], a.wofs[xs + 1], c, a.long_min,
+                                                              ex, st, a.wedge, a.eout, pe, a.adj16);
                 rtgd::wave_sync_lds();
             }
         }
@@ -690,7 +751,19 @@ BatchKernel batched_kernel_t(int threads, int unroll) {
     return nullptr;
 }
 
-BatchKernel batched_kernel(int threads, int unroll, bool per_edge, bool all_strategies) {
+// Counting on a degree-ordered copy (16-bit spans, every strategy).
+BatchKernel batched_kernel_p16(int threads, int unroll) {
+    if (threads == 256 && unroll == 2) return tc_batched<256, 2, false, true, true>;
+    if (threads == 256 && unroll == 4) return tc_batched<256, 4, false, true, true>;
+    if (threads == 256 && unroll == 8) return tc_batched<256, 8, false, true, true>;
+    if (threads == 512 && unroll == 2) return tc_batched<512, 2, false, true, true>;
+    if (threads == 512 && unroll == 4) return tc_batched<512, 4, false, true, true>;
+    if (threads == 512 && unroll == 8) return tc_batched<512, 8, false, true, true>;
+    return nullptr;
+}
+
+BatchKernel batched_kernel(int threads, int unroll, bool per_edge, bool all_strategies, bool p16 = false) {
+    if (p16 && !per_edge) return batched_kernel_p16(threads, unroll);
     if (per_edge) return all_strategies ? batched_kernel_t<true, true>(threads, unroll) : batched_kernel_t<true, false>(threads, unroll);
     return all_strategies ? batched_kernel_t<false, true>(threads, unroll) : batched_kernel_t<false, false>(threads, unroll);
 }
@@ -776,10 +849,14 @@ __global__ __launch_bounds__(256) void plan_keys(const int32_t *__restrict__ off
 // each owner's first position. Keys of counting plans carry the partner itself, so only the
 // partner's row offsets are gathered; per-edge plans' keys carry the edge, whose endpoints are
 // read back.
+// fit16 (degree-ordered copies, else null): a partner whose row fits 16 bits gets bit 31 in its
+// span's start, and its elements are summed into *streamed16.
 __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
                            const int32_t *__restrict__ row, int64_t e_begin, const uint64_t *__restrict__ sorted,
                            int64_t nw, int bits, int2 *__restrict__ wspan, uint32_t *__restrict__ wedge,
-                           uint64_t *__restrict__ lens, uint32_t *__restrict__ first) {
+                           uint64_t *__restrict__ lens, uint32_t *__restrict__ first, const uint8_t *__restrict__ fit16,
+                           unsigned long long *__restrict__ streamed16) {
+    uint64_t s16 = 0;
     const uint64_t mask = (1ull << bits) - 1;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     // kPlanU positions per thread and round, loads in phases (keys, then the edges' endpoints
@@ -813,10 +890,12 @@ __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__res
             }
         }
         int32_t p0[kPlanU], p1[kPlanU];
+        uint32_t f16[kPlanU];
 #pragma unroll
         for (int k = 0; k < kPlanU; ++k) {
             p0[k] = off[partner[k]];
             p1[k] = off[partner[k] + 1];
+            f16[k] = fit16 ? (uint32_t)fit16[partner[k]] : 0u;
         }
 #pragma unroll
         for (int k = 0; k < kPlanU; ++k) {
@@ -825,10 +904,15 @@ __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__res
             const int32_t owner = (int32_t)(key[k] & mask);
             const int32_t len = p1[k] - p0[k];
             if (wedge) wedge[i] = (uint32_t)edge[k];
-            wspan[i] = make_int2(p0[k], len);
+            wspan[i] = make_int2((int32_t)((uint32_t)p0[k] | (f16[k] << 31)), len);
             lens[i] = (uint64_t)len;
+            s16 += f16[k] ? (uint64_t)len : 0ull;
             if (i == 0 || (int32_t)(prev[k] & mask) != owner) first[owner] = (uint32_t)i;
         }
+    }
+    if (fit16) {  // one device atomic per wave
+        s16 = rtgd::wave_sum_u64(s16);
+        if (__lane_id() == 0 && s16) atomicAdd(streamed16, (unsigned long long)s16);
     }
 }
 
@@ -1009,7 +1093,7 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     HIP_TRY(hipMemsetAsync(d_bytes, 0, sizeof(unsigned long long) * 8, st));
     int which = 0;
     if (ne > 0) {
-        plan_keys<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, p->e_end, bits,
+        plan_keys<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(p->v_off, p->v_adj, p->d_row, p->e_begin, p->e_end, bits,
                                                              n, p->opts.per_edge != 0, keys[0], d_bytes);
         HIP_TRY(hipGetLastError());
         int passes = 0;
@@ -1028,8 +1112,9 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     RTG_TRY(dev_alloc(&p->d_wspan, (size_t)std::max<int64_t>(nw, 1)));
     if (p->opts.per_edge) RTG_TRY(dev_alloc(&p->d_wedge, (size_t)std::max<int64_t>(nw, 1)));
     if (nw > 0) {
-        plan_spans<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(g->d_offsets, g->d_adjs, p->d_row, p->e_begin, keys[which],
-                                                              nw, bits, p->d_wspan, p->d_wedge, lens, d_first);
+        plan_spans<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(p->v_off, p->v_adj, p->d_row, p->e_begin, keys[which],
+                                                              nw, bits, p->d_wspan, p->d_wedge, lens, d_first, p->v_fit16,
+                                                              d_bytes + 5);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u64_w(ctx, lens, lens_ex, nw, nullptr, scan_work));
         plan_runs<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(keys[which], nw, bits, d_first, lens, lens_ex, d_cnt,
@@ -1044,9 +1129,12 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
         HIP_TRY(hipGetLastError());
     }
     uint32_t flag = 0;
+    unsigned long long s16 = 0;
     HIP_TRY(hipMemcpyAsync(&flag, d_flag, sizeof(flag), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&s16, d_bytes + 5, sizeof(s16), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (flag) return set_error(RTG_EHIP, "batched TC: inconsistent partner offsets in the plan");
+    p->streamed16 = (int64_t)s16;
     return RTG_OK;
 }
 
@@ -1105,7 +1193,7 @@ __device__ __forceinline__ OwnerPlan owner_plan(const QueueParams &q, int32_t x)
         return o;
     }
     const int32_t s = q.strat[o.bucket];
-    const uint32_t words = (((uint32_t)(q.adj[q.off[x] + o.d - 1] - q.adj[q.off[x]])) >> 5) + 1u;
+    const uint32_t words = bitmap_words((uint32_t)q.adj[q.off[x]], (uint32_t)q.adj[q.off[x] + o.d - 1]);
     if (s == kStratBitmap && words <= (uint32_t)q.max_table) o.kind = kItemBitmap;
     else if (s == kStratMerge && o.d <= q.max_table) o.kind = kItemSorted;
     else if (s == kStratSearch || o.d > q.fit_block) o.kind = kItemSearch;
@@ -1284,7 +1372,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     if (threads != 256 && threads != 512) return set_error(RTG_EINVAL, "batched TC: threads must be 256 or 512");
     const int waves = threads / 64;
     const TableSizes ts = table_sizes(tune, g->max_row);
-    QueueParams q{g->d_offsets, g->d_adjs, p->d_wofs, p->d_work, n, ts.wave_owner_max, ts.fit_block, ts.max_table,
+    QueueParams q{p->v_off, p->v_adj, p->d_wofs, p->d_work, n, ts.wave_owner_max, ts.fit_block, ts.max_table,
                   ts.wave_table, only_bucket, tune.item_work, tune.wave_item_work, {}};
     for (int b = 0; b < kBuckets; ++b) q.strat[b] = tune.strat[b];
     if (p->d_items) (void)hipFree(p->d_items);
@@ -1318,7 +1406,8 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     HIP_TRY(hipStreamSynchronize(st));
     const int64_t nb = (int64_t)(uint32_t)(h[8] & 0xFFFFFFFFull), ns = (int64_t)(uint32_t)(h[8] >> 32);
     const int64_t heavy_small = std::min<int64_t>((int64_t)h[7], ns);
-    p->staged_bytes = h[0] + 16ull * (uint64_t)ns;  // a wave item per small owner at most
+    // a wave item per small owner at most; 16-bit spans read 2 of the 4 bytes per element
+    p->staged_bytes = h[0] + 16ull * (uint64_t)ns - 2ull * (uint64_t)p->streamed16;
     for (int b = 0; b < kBuckets; ++b) p->bucket_items[b] = (int64_t)h[1 + b];
     p->n_search_items = (int64_t)h[5];
     p->all_strategies = h[6] > 0;
@@ -1385,7 +1474,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
 
     // Persistent grid: as many blocks as fit on the device at once.
     const size_t lds = sizeof(uint32_t) * (size_t)ts.max_table;
-    BatchKernel k = batched_kernel(threads, tune.unroll, false, p->all_strategies);
+    BatchKernel k = batched_kernel(threads, tune.unroll, false, p->all_strategies, p->renumbered);
     if (!k) return set_error(RTG_EINVAL, "batched TC: unsupported (threads, unroll) = (" + std::to_string(threads) + ", " +
                                              std::to_string(tune.unroll) + ")");
     int per_cu = 0;
@@ -1445,15 +1534,16 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 5;
+constexpr int kTuneRevision = 7;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
-// fingerprint, the edge range and whether the plan is for per-edge counts.
+// fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
+// degree-ordered copy.
 std::string tune_key(rtg_ctx *ctx, const rtg_tc_plan *p) {
     char buf[256];
     std::snprintf(buf, sizeof(buf), "%s/cu%d/r%d/n%d/m%lld/%016llx/e%lld-%lld%s", ctx->prop.gcnArchName, ctx->cu_count,
                   kTuneRevision, p->g->n, (long long)p->g->m, (unsigned long long)p->offsets_fingerprint,
-                  (long long)p->e_begin, (long long)p->e_end, p->opts.per_edge ? "/pe" : "");
+                  (long long)p->e_begin, (long long)p->e_end, p->opts.per_edge ? "/pe" : (p->renumbered ? "/rn" : ""));
     return buf;
 }
 
@@ -1516,6 +1606,15 @@ int time_plan(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *pe_out, float *median_ms) 
 // then, for the fastest shape, each bucket's strategy candidates timed on a queue of that bucket
 // alone (a strategy no owner of the bucket can use is skipped). The combination is kept if it
 // is at least as fast as the default strategies on the whole plan.
+// The starting configuration: hash tables for every bucket but the last, except on a
+// degree-ordered copy, where block owners' lists lie in a short id range and their bitmaps (one
+// LDS read per element, no hashing) are the better default.
+BatchTune default_tune(const rtg_tc_plan *p) {
+    BatchTune t;
+    if (p->renumbered) t.strat[1] = t.strat[2] = kStratBitmap;
+    return t;
+}
+
 int autotune(rtg_ctx *ctx, rtg_tc_plan *p, BatchTune *best) {
     uint32_t *pe_out = nullptr;
     DevFree fpe;
@@ -1526,7 +1625,7 @@ int autotune(rtg_ctx *ctx, rtg_tc_plan *p, BatchTune *best) {
     float best_ms = 1e30f;
     std::string best_s;
     for (const char *cand : kCandidates) {
-        BatchTune t;
+        BatchTune t = default_tune(p);
         tune_from_string(t, cand);
         float ms = 0.f;
         if (plan_items(ctx, p, t) != RTG_OK || time_plan(ctx, p, pe_out, &ms) != RTG_OK) continue;
@@ -1536,7 +1635,7 @@ int autotune(rtg_ctx *ctx, rtg_tc_plan *p, BatchTune *best) {
         }
     }
     if (best_s.empty()) return rtg::set_error(RTG_EHIP, "autotune: no candidate ran");
-    BatchTune shape;
+    BatchTune shape = default_tune(p);
     tune_from_string(shape, best_s);
     BatchTune mix = shape;
     for (int b = 0; b < kBuckets; ++b) {
@@ -1580,7 +1679,7 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p) {
     RTG_TRY(plan_graph(ctx, p));
     p->t_plan_graph_ms = ms_since(t0);
     t0 = Clk::now();
-    BatchTune tune;
+    BatchTune tune = default_tune(p);
     const char *env = std::getenv("RTG_TC_TUNE");
     const std::string key = tune_key(ctx, p);
     auto cache = read_tune_cache(p->opts.tune_cache);
@@ -1630,9 +1729,9 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
             break;
         case RTG_TC_BATCHED: {
             if (p->n_items == 0) break;
-            BatchArgs a{g->d_offsets, g->d_adjs, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
+            BatchArgs a{p->v_off, p->v_adj, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
                         p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
-                        p->long_min, p->grab, nullptr, nullptr};
+                        p->long_min, p->grab, nullptr, nullptr, p->v_adj16};
             const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
             launch_batched((BatchKernel)p->kernel, p->grid, p->block_threads, lds, st, a);
             break;
@@ -1648,20 +1747,31 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
 // instantiation of the batched kernel writes every span's count at its edge (edges whose
 // shorter list is empty keep 0). The total is accumulated as in a count.
 int launch_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *eout) {
-    const rtg_graph *g = p->g;
     hipStream_t st = ctx->stream;
     const int64_t ne = p->e_end - p->e_begin;
     if (ne > 0) HIP_TRY(hipMemsetAsync(eout + p->e_begin, 0, sizeof(uint32_t) * (size_t)ne, st));
     reset_counters<<<1, 64, 0, st>>>(p->d_count);
     HIP_TRY(hipGetLastError());
     if (p->n_items == 0) return RTG_OK;
-    BatchArgs a{g->d_offsets, g->d_adjs, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
+    BatchArgs a{p->v_off, p->v_adj, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
                 p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
-                p->long_min, p->grab, p->d_wedge, eout};
+                p->long_min, p->grab, p->d_wedge, eout, nullptr};
     const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
     launch_batched((BatchKernel)p->kernel_pe, p->grid_pe, p->block_threads, lds, st, a);
     HIP_TRY(hipGetLastError());
     return RTG_OK;
+}
+
+// Whether a plan counts on the graph's degree-ordered copy: batched counting plans over the whole
+// edge range (per-edge counts and edge ranges are indexed by the graph's own edges). By default
+// graphs of at least 2^20 edges, which repay the copy's one-time build (a few ms on 83M edges);
+// RTG_TC_RENUMBER=0 turns it off, =1 also takes smaller graphs.
+bool want_renumber(const rtg_tc_plan *p) {
+    if (p->opts.method != RTG_TC_BATCHED || p->opts.per_edge) return false;
+    if (p->e_begin != 0 || p->e_end != p->g->m || p->g->m <= 0 || p->g->n <= 0) return false;
+    const char *e = std::getenv("RTG_TC_RENUMBER");
+    if (e && *e) return e[0] != '0';
+    return p->g->m >= ((int64_t)1 << 20);
 }
 
 }  // namespace
@@ -1675,14 +1785,27 @@ int rtg_tc_plan_create_impl(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts 
     p->e_begin = std::max<int64_t>(0, opts->e_begin);
     p->e_end = (opts->e_end <= 0 || opts->e_end > g->m) ? g->m : opts->e_end;
     if (p->e_begin > p->e_end) p->e_begin = p->e_end;
+    p->v_off = g->d_offsets;
+    p->v_adj = g->d_adjs;
     hipStream_t st = ctx->stream;
     int rc = RTG_OK;
     HIP_TRY(hipEventRecord(ctx->ev[5], st));
     do {
+        if (want_renumber(p)) {
+            const bool built = g->renum == nullptr;
+            const rtg_renum *r = nullptr;
+            if ((rc = graph_renumbered(ctx, g, &r)) != RTG_OK) break;
+            p->v_off = r->d_off;
+            p->v_adj = r->d_adj;
+            p->v_adj16 = r->d_adj16;
+            p->v_fit16 = r->d_fit16;
+            p->renumbered = true;
+            p->t_renum_ms = built ? r->t_build_ms : 0.0;
+        }
         if ((rc = dev_alloc(&p->d_row, (size_t)g->m)) != RTG_OK) break;
         if ((rc = dev_alloc(&p->d_count, 4)) != RTG_OK) break;
         if (g->n > 0) {
-            edge_rows_kernel<<<grid_cap(ctx, g->n, 256, 16), 256, 0, st>>>(g->d_offsets, g->n, p->d_row);
+            edge_rows_kernel<<<grid_cap(ctx, g->n, 256, 16), 256, 0, st>>>(p->v_off, g->n, p->d_row);
             if (hipGetLastError() != hipSuccess) { rc = set_error(RTG_EHIP, "edge_rows launch failed"); break; }
         }
         if (opts->method == RTG_TC_BATCHED) {
@@ -1806,7 +1929,8 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
              kStratName[p->strat[0]] + "/" + kStratName[p->strat[1]] + "/" + kStratName[p->strat[2]] + "/" +
              kStratName[p->strat[3]] + ",all_strategies=" + (p->all_strategies ? "1" : "0") +
              ",plan_graph_ms=" + std::to_string(p->t_plan_graph_ms) + ",plan_tune_ms=" + std::to_string(p->t_plan_tune_ms) +
-             ",plan_items_ms=" + std::to_string(p->t_plan_items_ms);
+             ",plan_items_ms=" + std::to_string(p->t_plan_items_ms) + ",renumbered=" + (p->renumbered ? "1" : "0") +
+             ",renumber_ms=" + std::to_string(p->t_renum_ms) + ",streamed16=" + std::to_string(p->streamed16);
     std::snprintf(buf, len, "%s", d.c_str());
     return RTG_OK;
 }

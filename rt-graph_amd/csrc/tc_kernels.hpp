@@ -66,6 +66,16 @@ struct rtg_tc_plan {
     int64_t bucket_items[4] = {};      // per owner bucket: block items (small owners for bucket 0)
     int32_t strat[4] = {};             // per owner bucket: strategy (0 hash, 1 search, 2 bitmap, 3 merge)
     bool all_strategies = false;       // the queue holds bitmap or sorted-list items
+    // The CSR the batched plan counts on: the graph's own, or its degree-ordered copy (full-range
+    // counting plans; same count). With the copy, partner spans whose row fits 16 bits carry
+    // bit 31 in their start and are streamed from v_adj16.
+    const int32_t *v_off = nullptr;
+    const int32_t *v_adj = nullptr;
+    const uint16_t *v_adj16 = nullptr;
+    const uint8_t *v_fit16 = nullptr;
+    bool renumbered = false;
+    int64_t streamed16 = 0;            // streamed partner elements in 16-bit spans
+    double t_renum_ms = 0.0;           // building the degree-ordered copy (first plan of a graph)
     std::string tune;                       // chosen configuration
     bool autotuned = false, tune_from_cache = false;
 };

@@ -483,3 +483,52 @@ def test_autotuned_bucket_strategies(ctx, tmp_path):
     assert p2.run()[0] == t
     st = p2.plan_stats
     assert st["nbuckets"] == 4 and len(st["strategy"]) == 4
+
+
+RENUMBER_TUNES = ["strat=hhhs", "strat=bbbs", "strat=hbbs", "strat=mmms", "strat=ssss", "strat=bhbb",
+                  "strat=hbbs,unroll=8", "strat=bbbs,threads=512,unroll=2"]
+
+
+@pytest.mark.parametrize("tune", RENUMBER_TUNES)
+def test_renumbered_counts(ctx, monkeypatch, tune):
+    """Batched counting plans on the graph's degree-ordered copy (RTG_TC_RENUMBER=1 forces it
+    below the default size): every fixture graph, every strategy mix, long spans of 1 and 128
+    elements (16-bit and 32-bit spans, loads past the end), against the oracle's count."""
+    monkeypatch.setenv("RTG_TC_RENUMBER", "1")
+    for name in GRAPHS:
+        _, _, t = oracle_csr(name)
+        g = rtg.DeviceGraph.from_pairs(ctx, GRAPHS[name])
+        if g.m == 0:
+            continue
+        for long_min in (1, 128):
+            monkeypatch.setenv("RTG_TC_TUNE", f"threads=256,unroll=4,item=65536,wave_item=1024,long={long_min},grab=16,{tune}")
+            p = g.tc_plan(method=rtg.TC_BATCHED)
+            d = p.describe()
+            assert d["renumbered"] == "1", (name, d)
+            assert p.run()[0] == t, (name, tune, long_min)
+            p.close()
+        g.close()
+
+
+def test_renumbering_default_and_opt_out(ctx, monkeypatch):
+    """Graphs below 2^20 edges keep their own CSR by default; RTG_TC_RENUMBER=0 keeps it for any
+    graph; per-edge plans and edge-range plans never renumber; the copy is built once per graph
+    and shared by its later plans."""
+    off, adj, t = oracle_csr("ba_100000")  # about 500k oriented edges
+    assert adj.size < (1 << 20)
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    assert g.tc_plan(method=rtg.TC_BATCHED).describe()["renumbered"] == "0"
+    monkeypatch.setenv("RTG_TC_RENUMBER", "1")
+    p1 = g.tc_plan(method=rtg.TC_BATCHED)
+    p2 = g.tc_plan(method=rtg.TC_BATCHED)
+    d1, d2 = p1.describe(), p2.describe()
+    assert d1["renumbered"] == "1" and d2["renumbered"] == "1"
+    assert float(d1["renumber_ms"]) > 0 and float(d2["renumber_ms"]) == 0  # built once
+    assert int(d1["streamed16"]) > 0  # rows of the 65,536 highest-degree vertices' neighbours fit 16 bits
+    assert p1.run()[0] == t and p2.run()[0] == t
+    pe = g.tc_plan(method=rtg.TC_BATCHED, per_edge=True)
+    assert pe.describe()["renumbered"] == "0"
+    half = g.tc_plan(method=rtg.TC_BATCHED, e_begin=0, e_end=g.m // 2)
+    assert half.describe()["renumbered"] == "0"
+    monkeypatch.setenv("RTG_TC_RENUMBER", "0")
+    assert g.tc_plan(method=rtg.TC_BATCHED).describe()["renumbered"] == "0"
