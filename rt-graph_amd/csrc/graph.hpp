@@ -31,9 +31,12 @@ struct rtg_renum {
     int32_t *d_adj = nullptr;     // m
     uint16_t *d_adj16 = nullptr;  // m
     uint8_t *d_fit16 = nullptr;   // n
+    int32_t *d_inv = nullptr;     // n: new id -> the graph's id
+    int32_t *d_emap = nullptr;    // m: edge of the copy -> the graph's edge (built for per-edge plans)
     int64_t rows16 = 0;           // rows that fit 16 bits
     int64_t entries16 = 0;        // their entries
     float t_build_ms = 0.f;
+    float t_emap_ms = 0.f;
 };
 
 namespace rtg {
@@ -44,7 +47,8 @@ int csr_max_row(rtg_ctx *ctx, const int32_t *d_offsets, int32_t n, int32_t *out)
 // in LDS), or device-scope atomics for domains above 2^24 ids. Allocates its own temporaries;
 // enqueued on the context stream.
 int id_histogram(rtg_ctx *ctx, const int32_t *ids, int64_t n, int64_t domain, uint32_t *deg);
-// The graph's degree-ordered copy, built on first use (caller holds ctx->mu).
-int graph_renumbered(rtg_ctx *ctx, const rtg_graph *g, const rtg_renum **out);
+// The graph's degree-ordered copy, built on first use (caller holds ctx->mu); with_emap also
+// builds its edge map (once).
+int graph_renumbered(rtg_ctx *ctx, const rtg_graph *g, bool with_emap, const rtg_renum **out);
 void renum_free(rtg_renum *r);
 }

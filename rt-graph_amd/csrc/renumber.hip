@@ -16,6 +16,8 @@
 //             sorted keys' low bits are the new rows, in order and sorted
 //   16 bits   the low 16 bits of every entry, and a flag per row whose largest (last) entry
 //             is below 2^16
+//   edge map  for per-edge plans (built on their first use): each edge of the copy's index in
+//             the graph's own CSR, by a binary search in the graph's row
 #include "graph.hpp"
 
 #include <chrono>
@@ -45,13 +47,50 @@ __global__ void order_keys(const int32_t *__restrict__ off, const uint32_t *__re
     }
 }
 
-// newid[v] = position of v in the sorted order; cnt[newid[v]] = out-degree of v.
+// newid[v] = position of v in the sorted order, inv[i] = v; cnt[newid[v]] = out-degree of v.
 __global__ void order_scatter(const uint64_t *__restrict__ sorted, const int32_t *__restrict__ off, int32_t n,
-                              uint32_t *__restrict__ newid, uint32_t *__restrict__ cnt) {
+                              uint32_t *__restrict__ newid, int32_t *__restrict__ inv, uint32_t *__restrict__ cnt) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t v = (uint32_t)sorted[i];
         newid[v] = (uint32_t)i;
+        inv[i] = (int32_t)v;
         cnt[i] = (uint32_t)(off[v + 1] - off[v]);
+    }
+}
+
+// emap[i] = the graph's index of the copy's edge i = (u' -> v'): the position of inv[v'] in the
+// graph's sorted row inv[u'] (a binary search; the edge is there, the copy being the same DAG
+// renamed). Walks each block's range of 256 copy rows like edge_keys.
+__global__ __launch_bounds__(kT) void edge_map(const int32_t *__restrict__ coff, const int32_t *__restrict__ cadj,
+                                               int32_t n, const int32_t *__restrict__ inv,
+                                               const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                               int32_t *__restrict__ emap) {
+    __shared__ int32_t so[kT + 1];
+    __shared__ int32_t rb[kT];  // the graph's row start of each copy row
+    __shared__ int32_t rl[kT];  // and its length
+    for (int64_t v0 = (int64_t)blockIdx.x * kT; v0 < n; v0 += (int64_t)gridDim.x * kT) {
+        const int32_t nv = (int32_t)min<int64_t>(kT, n - v0);
+        for (int i = threadIdx.x; i <= nv; i += kT) so[i] = coff[v0 + i];
+        if ((int)threadIdx.x < nv) {
+            const int32_t u = inv[v0 + threadIdx.x];
+            rb[threadIdx.x] = off[u];
+            rl[threadIdx.x] = off[u + 1] - off[u];
+        }
+        __syncthreads();
+        const int32_t e1 = so[nv];
+        int32_t v = 0;  // so[v] <= e < so[v + 1]
+        for (int32_t e = so[0] + threadIdx.x; e < e1; e += kT) {
+            while (so[v + 1] <= e) ++v;
+            const int32_t x = inv[cadj[e]], b = rb[v];
+            int32_t lo = 0, hi = rl[v];  // first position of the row holding a value >= x
+            while (lo < hi) {
+                const int32_t mid = (lo + hi) >> 1;
+                if (adj[b + mid] < x) lo = mid + 1;
+                else hi = mid;
+            }
+            emap[e] = b + lo;
+        }
+        __syncthreads();  // so, rb and rl are refilled for the next rows
     }
 }
 
@@ -135,6 +174,7 @@ int build(rtg_ctx *ctx, const rtg_graph *g, rtg_renum *r) {
     HIP_TRY(hipMalloc(&r->d_adj, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)));
     HIP_TRY(hipMalloc(&r->d_adj16, sizeof(uint16_t) * (size_t)std::max<int64_t>(m, 1)));
     HIP_TRY(hipMalloc(&r->d_fit16, (size_t)std::max<int32_t>(n, 1)));
+    HIP_TRY(hipMalloc(&r->d_inv, sizeof(int32_t) * (size_t)std::max<int32_t>(n, 1)));
     const int bits = bits_of(std::max<int64_t>(n, 2));
     const int64_t kn = std::max<int64_t>(m, (int64_t)n + 1);
     const size_t bytes = Arena::round(8 * (size_t)kn) * 2 + Arena::round(radix_sort_work_bytes(kn)) +
@@ -160,7 +200,7 @@ int build(rtg_ctx *ctx, const rtg_graph *g, rtg_renum *r) {
     int which = 0, passes = 0;
     RTG_TRY(radix_sort_u64(ctx, kk, n, 64, &which, &passes, sort_work));
     HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * ((size_t)n + 1), st));
-    order_scatter<<<grid_of(ctx, n), kT, 0, st>>>(kk[which], g->d_offsets, n, newid, cnt);
+    order_scatter<<<grid_of(ctx, n), kT, 0, st>>>(kk[which], g->d_offsets, n, newid, r->d_inv, cnt);
     HIP_TRY(hipGetLastError());
     RTG_TRY(scan_exclusive_u32_w(ctx, cnt, reinterpret_cast<uint32_t *>(r->d_off), (int64_t)n + 1, nullptr, scan_work));
 
@@ -190,28 +230,40 @@ namespace rtg {
 
 void renum_free(rtg_renum *r) {
     if (!r) return;
-    void *p[] = {r->d_off, r->d_adj, r->d_adj16, r->d_fit16};
+    void *p[] = {r->d_off, r->d_adj, r->d_adj16, r->d_fit16, r->d_inv, r->d_emap};
     for (void *x : p)
         if (x) (void)hipFree(x);
     delete r;
 }
 
-int graph_renumbered(rtg_ctx *ctx, const rtg_graph *g, const rtg_renum **out) {
+int graph_renumbered(rtg_ctx *ctx, const rtg_graph *g, bool with_emap, const rtg_renum **out) {
     *out = nullptr;
-    if (g->renum) {
-        *out = g->renum;
-        return RTG_OK;
+    rtg_renum *r = g->renum;
+    if (!r) {
+        if (g->n <= 0 || g->m <= 0) return set_error(RTG_EINVAL, "renumbered CSR of an empty graph");
+        if (g->m >= ((int64_t)1 << 31)) return set_error(RTG_EINVAL, "renumbered CSR: too many edges");
+        r = new rtg_renum();
+        const int rc = build(ctx, g, r);
+        if (rc != RTG_OK) {
+            (void)hipGetLastError();
+            renum_free(r);
+            return rc;
+        }
+        const_cast<rtg_graph *>(g)->renum = r;  // a cache on the graph; plan creation holds ctx->mu
     }
-    if (g->n <= 0 || g->m <= 0) return set_error(RTG_EINVAL, "renumbered CSR of an empty graph");
-    if (g->m >= ((int64_t)1 << 32)) return set_error(RTG_EINVAL, "renumbered CSR: too many edges");
-    auto *r = new rtg_renum();
-    const int rc = build(ctx, g, r);
-    if (rc != RTG_OK) {
-        (void)hipGetLastError();
-        renum_free(r);
-        return rc;
+    if (with_emap && !r->d_emap) {
+        const auto t0 = std::chrono::steady_clock::now();
+        int32_t *emap = nullptr;
+        HIP_TRY(hipMalloc(&emap, sizeof(int32_t) * (size_t)r->m));
+        edge_map<<<grid_of(ctx, r->n), kT, 0, ctx->stream>>>(r->d_off, r->d_adj, r->n, r->d_inv, g->d_offsets, g->d_adjs,
+                                                             emap);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            (void)hipFree(emap);
+            return set_error(RTG_EHIP, "renumbered CSR: edge map failed");
+        }
+        r->d_emap = emap;
+        r->t_emap_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    const_cast<rtg_graph *>(g)->renum = r;  // a cache on the graph; plan creation holds ctx->mu
     *out = r;
     return RTG_OK;
 }

@@ -573,15 +573,16 @@ __device__ __noinline__ uint32_t stream_spans_ool(const int32_t *__restrict__ ad
 }
 
 // With 16-bit spans: linear probing (the hash's fallback) and the sorted list, out of line.
-template <bool kP16, int U>
+template <bool kP16, int U, bool kPerEdge>
 __device__ __noinline__ uint32_t stream_spans_ool(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                   int64_t p0, int64_t p1, Table c, int32_t long_min, int32_t *ex,
                                                   int32_t *st, const uint32_t *__restrict__ wedge,
                                                   uint32_t *__restrict__ eout, uint32_t *pe,
                                                   const uint16_t *__restrict__ adj16, uint32_t mode) {
     if (mode == kLinearProbe)
-        return stream_spans<kLinearProbe, U, false, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
-    return stream_spans<kSorted, U, false, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+        return stream_spans<kLinearProbe, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe,
+                                                              adj16);
+    return stream_spans<kSorted, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
 }
 
 // Partners [p0, p1) against the owner table c, in the probe mode the table was built in. kAll:
@@ -596,12 +597,12 @@ __device__ __forceinline__ uint32_t stream_table(const int32_t *__restrict__ adj
                                                  const uint16_t *__restrict__ adj16 = nullptr) {
     const uint32_t mode = rtgd::uniform_u32(c.mode);
     if constexpr (kP16) {
-        static_assert(kAll && !kPerEdge, "16-bit spans come with every strategy, counting only");
+        static_assert(kAll, "16-bit spans come with every strategy");
         if (mode == kBitmap)
-            return stream_spans<kBitmap, U, false, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+            return stream_spans<kBitmap, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
         if (mode == kCuckoo)
-            return stream_spans<kCuckoo, U, false, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
-        return stream_spans_ool<kP16, U>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16, mode);
+            return stream_spans<kCuckoo, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+        return stream_spans_ool<kP16, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16, mode);
     } else {
         if (mode == kCuckoo) return stream_spans<kCuckoo, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
         if (!kAll || mode == kLinearProbe)
@@ -751,19 +752,20 @@ BatchKernel batched_kernel_t(int threads, int unroll) {
     return nullptr;
 }
 
-// Counting on a degree-ordered copy (16-bit spans, every strategy).
+// On a degree-ordered copy (16-bit spans, every strategy), counting or per-edge.
+template <bool kPerEdge>
 BatchKernel batched_kernel_p16(int threads, int unroll) {
-    if (threads == 256 && unroll == 2) return tc_batched<256, 2, false, true, true>;
-    if (threads == 256 && unroll == 4) return tc_batched<256, 4, false, true, true>;
-    if (threads == 256 && unroll == 8) return tc_batched<256, 8, false, true, true>;
-    if (threads == 512 && unroll == 2) return tc_batched<512, 2, false, true, true>;
-    if (threads == 512 && unroll == 4) return tc_batched<512, 4, false, true, true>;
-    if (threads == 512 && unroll == 8) return tc_batched<512, 8, false, true, true>;
+    if (threads == 256 && unroll == 2) return tc_batched<256, 2, kPerEdge, true, true>;
+    if (threads == 256 && unroll == 4) return tc_batched<256, 4, kPerEdge, true, true>;
+    if (threads == 256 && unroll == 8) return tc_batched<256, 8, kPerEdge, true, true>;
+    if (threads == 512 && unroll == 2) return tc_batched<512, 2, kPerEdge, true, true>;
+    if (threads == 512 && unroll == 4) return tc_batched<512, 4, kPerEdge, true, true>;
+    if (threads == 512 && unroll == 8) return tc_batched<512, 8, kPerEdge, true, true>;
     return nullptr;
 }
 
 BatchKernel batched_kernel(int threads, int unroll, bool per_edge, bool all_strategies, bool p16 = false) {
-    if (p16 && !per_edge) return batched_kernel_p16(threads, unroll);
+    if (p16) return per_edge ? batched_kernel_p16<true>(threads, unroll) : batched_kernel_p16<false>(threads, unroll);
     if (per_edge) return all_strategies ? batched_kernel_t<true, true>(threads, unroll) : batched_kernel_t<true, false>(threads, unroll);
     return all_strategies ? batched_kernel_t<false, true>(threads, unroll) : batched_kernel_t<false, false>(threads, unroll);
 }
@@ -855,7 +857,7 @@ __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__res
                            const int32_t *__restrict__ row, int64_t e_begin, const uint64_t *__restrict__ sorted,
                            int64_t nw, int bits, int2 *__restrict__ wspan, uint32_t *__restrict__ wedge,
                            uint64_t *__restrict__ lens, uint32_t *__restrict__ first, const uint8_t *__restrict__ fit16,
-                           unsigned long long *__restrict__ streamed16) {
+                           unsigned long long *__restrict__ streamed16, const int32_t *__restrict__ emap) {
     uint64_t s16 = 0;
     const uint64_t mask = (1ull << bits) - 1;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -903,7 +905,7 @@ __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__res
             if (i >= nw) break;
             const int32_t owner = (int32_t)(key[k] & mask);
             const int32_t len = p1[k] - p0[k];
-            if (wedge) wedge[i] = (uint32_t)edge[k];
+            if (wedge) wedge[i] = emap ? (uint32_t)emap[edge[k]] : (uint32_t)edge[k];  // the graph's edge
             wspan[i] = make_int2((int32_t)((uint32_t)p0[k] | (f16[k] << 31)), len);
             lens[i] = (uint64_t)len;
             s16 += f16[k] ? (uint64_t)len : 0ull;
@@ -1114,7 +1116,7 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     if (nw > 0) {
         plan_spans<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(p->v_off, p->v_adj, p->d_row, p->e_begin, keys[which],
                                                               nw, bits, p->d_wspan, p->d_wedge, lens, d_first, p->v_fit16,
-                                                              d_bytes + 5);
+                                                              d_bytes + 5, p->v_emap);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u64_w(ctx, lens, lens_ex, nw, nullptr, scan_work));
         plan_runs<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(keys[which], nw, bits, d_first, lens, lens_ex, d_cnt,
@@ -1487,7 +1489,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     p->fit_block = ts.fit_block;
     p->kernel = (void *)k;
     if (p->opts.per_edge) {
-        BatchKernel ke = batched_kernel(threads, tune.unroll, true, p->all_strategies);
+        BatchKernel ke = batched_kernel(threads, tune.unroll, true, p->all_strategies, p->renumbered);
         int per_cu_e = 0;
         HIP_TRY(hipFuncSetAttribute((const void *)ke, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_e, ke, threads, lds));
@@ -1534,7 +1536,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 7;
+constexpr int kTuneRevision = 8;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
@@ -1543,7 +1545,8 @@ std::string tune_key(rtg_ctx *ctx, const rtg_tc_plan *p) {
     char buf[256];
     std::snprintf(buf, sizeof(buf), "%s/cu%d/r%d/n%d/m%lld/%016llx/e%lld-%lld%s", ctx->prop.gcnArchName, ctx->cu_count,
                   kTuneRevision, p->g->n, (long long)p->g->m, (unsigned long long)p->offsets_fingerprint,
-                  (long long)p->e_begin, (long long)p->e_end, p->opts.per_edge ? "/pe" : (p->renumbered ? "/rn" : ""));
+                  (long long)p->e_begin, (long long)p->e_end, p->opts.per_edge ? (p->renumbered ? "/pe/rn" : "/pe")
+                                                                              : (p->renumbered ? "/rn" : ""));
     return buf;
 }
 
@@ -1755,19 +1758,19 @@ int launch_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *eout) {
     if (p->n_items == 0) return RTG_OK;
     BatchArgs a{p->v_off, p->v_adj, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
                 p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
-                p->long_min, p->grab, p->d_wedge, eout, nullptr};
+                p->long_min, p->grab, p->d_wedge, eout, p->v_adj16};
     const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
     launch_batched((BatchKernel)p->kernel_pe, p->grid_pe, p->block_threads, lds, st, a);
     HIP_TRY(hipGetLastError());
     return RTG_OK;
 }
 
-// Whether a plan counts on the graph's degree-ordered copy: batched counting plans over the whole
-// edge range (per-edge counts and edge ranges are indexed by the graph's own edges). By default
-// graphs of at least 2^20 edges, which repay the copy's one-time build (a few ms on 83M edges);
-// RTG_TC_RENUMBER=0 turns it off, =1 also takes smaller graphs.
+// Whether a plan counts on the graph's degree-ordered copy: batched plans over the whole edge
+// range (an edge range is a range of the graph's own edges); per-edge plans write through the
+// copy's edge map. By default graphs of at least 2^20 edges, which repay the copy's one-time
+// build (a few ms on 83M edges); RTG_TC_RENUMBER=0 turns it off, =1 also takes smaller graphs.
 bool want_renumber(const rtg_tc_plan *p) {
-    if (p->opts.method != RTG_TC_BATCHED || p->opts.per_edge) return false;
+    if (p->opts.method != RTG_TC_BATCHED) return false;
     if (p->e_begin != 0 || p->e_end != p->g->m || p->g->m <= 0 || p->g->n <= 0) return false;
     const char *e = std::getenv("RTG_TC_RENUMBER");
     if (e && *e) return e[0] != '0';
@@ -1792,15 +1795,17 @@ int rtg_tc_plan_create_impl(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts 
     HIP_TRY(hipEventRecord(ctx->ev[5], st));
     do {
         if (want_renumber(p)) {
-            const bool built = g->renum == nullptr;
+            const bool built_copy = g->renum == nullptr;
+            const bool built_map = p->opts.per_edge && (built_copy || g->renum->d_emap == nullptr);
             const rtg_renum *r = nullptr;
-            if ((rc = graph_renumbered(ctx, g, &r)) != RTG_OK) break;
+            if ((rc = graph_renumbered(ctx, g, p->opts.per_edge != 0, &r)) != RTG_OK) break;
+            p->v_emap = r->d_emap;
             p->v_off = r->d_off;
             p->v_adj = r->d_adj;
             p->v_adj16 = r->d_adj16;
             p->v_fit16 = r->d_fit16;
             p->renumbered = true;
-            p->t_renum_ms = built ? r->t_build_ms : 0.0;
+            p->t_renum_ms = (built_copy ? r->t_build_ms : 0.0) + (built_map ? r->t_emap_ms : 0.0);
         }
         if ((rc = dev_alloc(&p->d_row, (size_t)g->m)) != RTG_OK) break;
         if ((rc = dev_alloc(&p->d_count, 4)) != RTG_OK) break;

@@ -73,6 +73,7 @@ struct rtg_tc_plan {
     const int32_t *v_adj = nullptr;
     const uint16_t *v_adj16 = nullptr;
     const uint8_t *v_fit16 = nullptr;
+    const int32_t *v_emap = nullptr;   // per-edge plans on the copy: its edge -> the graph's edge
     bool renumbered = false;
     int64_t streamed16 = 0;            // streamed partner elements in 16-bit spans
     double t_renum_ms = 0.0;           // building the degree-ordered copy (first plan of a graph)

@@ -512,8 +512,8 @@ def test_renumbered_counts(ctx, monkeypatch, tune):
 
 def test_renumbering_default_and_opt_out(ctx, monkeypatch):
     """Graphs below 2^20 edges keep their own CSR by default; RTG_TC_RENUMBER=0 keeps it for any
-    graph; per-edge plans and edge-range plans never renumber; the copy is built once per graph
-    and shared by its later plans."""
+    graph; edge-range plans never renumber; the copy is built once per graph and shared by its
+    later plans (a per-edge plan adds the copy's edge map)."""
     off, adj, t = oracle_csr("ba_100000")  # about 500k oriented edges
     assert adj.size < (1 << 20)
     g = rtg.DeviceGraph.from_csr(ctx, off, adj)
@@ -527,8 +527,32 @@ def test_renumbering_default_and_opt_out(ctx, monkeypatch):
     assert int(d1["streamed16"]) > 0  # rows of the 65,536 highest-degree vertices' neighbours fit 16 bits
     assert p1.run()[0] == t and p2.run()[0] == t
     pe = g.tc_plan(method=rtg.TC_BATCHED, per_edge=True)
-    assert pe.describe()["renumbered"] == "0"
+    dpe = pe.describe()
+    assert dpe["renumbered"] == "1" and float(dpe["renumber_ms"]) > 0  # the copy's edge map, built now
     half = g.tc_plan(method=rtg.TC_BATCHED, e_begin=0, e_end=g.m // 2)
     assert half.describe()["renumbered"] == "0"
     monkeypatch.setenv("RTG_TC_RENUMBER", "0")
     assert g.tc_plan(method=rtg.TC_BATCHED).describe()["renumbered"] == "0"
+
+
+@pytest.mark.parametrize("tune", ["strat=hbbs", "strat=hhhs", "strat=bmbs", "strat=hbbs,unroll=4,long=1"])
+def test_renumbered_per_edge(ctx, monkeypatch, tune):
+    """Per-edge plans on the degree-ordered copy: every span's count is written at the graph's
+    own edge through the copy's edge map; every fixture graph against the oracle's per-edge
+    counts, twice (the output is cleared between runs)."""
+    monkeypatch.setenv("RTG_TC_RENUMBER", "1")
+    monkeypatch.setenv("RTG_TC_TUNE", f"threads=256,unroll=8,item=65536,wave_item=1024,long=128,grab=16,{tune}")
+    for name in GRAPHS:
+        off, adj, t = oracle_csr(name)
+        if adj.size == 0:
+            continue
+        g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+        exp = orc.tc_per_edge(off, adj, 0)
+        plan = g.tc_plan(method=rtg.TC_BATCHED, use_hip_graph=False, per_edge=True)
+        assert plan.describe()["renumbered"] == "1"
+        for _ in range(2):
+            out, cnt, _ = plan.run_per_edge()
+            assert cnt == t and np.array_equal(out.numpy()[:adj.size], exp), (name, tune)
+        assert plan.run()[0] == t
+        plan.close()
+        g.close()
