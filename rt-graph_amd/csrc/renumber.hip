@@ -15,7 +15,7 @@
 //             edge range with a cursor over its 256 row offsets; radix sort on 2b bits; the
 //             sorted keys' low bits are the new rows, in order and sorted
 //   16 bits   the low 16 bits of every entry, and a flag per row whose largest (last) entry
-//             is below 2^16
+//             is below 2^16; every 32nd entry sampled (two-level searches in a row)
 //   edge map  for per-edge plans (built on their first use): each edge of the copy's index in
 //             the graph's own CSR, by a binary search in the graph's row
 #include "graph.hpp"
@@ -127,16 +127,29 @@ __global__ void split_entries(const uint64_t *__restrict__ sorted, int64_t m, in
     }
 }
 
-// fit16[v] = 1 when row v is non-empty and its last (largest) entry is below 2^16; counts[0] =
-// such rows, counts[1] = their entries (one device atomic per block).
-__global__ __launch_bounds__(kT) void fit16_rows(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                                 int32_t n, uint8_t *__restrict__ fit16,
-                                                 unsigned long long *__restrict__ counts) {
+// s16[i] = adj[16 i] for i < 16 * ceil(m / 256) + 16 (INT32_MAX past the entries), and
+// s256[j] = adj[256 j].
+__global__ void sample_rows(const int32_t *__restrict__ adj, int64_t m, int32_t *__restrict__ s16,
+                            int32_t *__restrict__ s256) {
+    const int64_t n16 = 16 * ((m + 255) / 256) + 16;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = 16 * i < m ? adj[16 * i] : INT32_MAX;
+        s16[i] = x;
+        if ((i & 15) == 0 && 16 * i < m) s256[i >> 4] = x;
+    }
+}
+
+// vmax[v] = row v's largest entry (-1 when empty); counts[0] = non-empty rows whose entries all
+// lie below 2^16, counts[1] = their entries (one device atomic per block).
+__global__ __launch_bounds__(kT) void row_max(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                              int32_t n, int32_t *__restrict__ vmax,
+                                              unsigned long long *__restrict__ counts) {
     unsigned long long r = 0, e = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         const int32_t b = off[v], d = off[v + 1] - b;
-        const bool f = d > 0 && adj[b + d - 1] < 65536;
-        fit16[v] = f ? 1 : 0;
+        const int32_t x = d > 0 ? adj[b + d - 1] : -1;
+        vmax[v] = x;
+        const bool f = d > 0 && x < 65536;
         r += f ? 1 : 0;
         e += f ? (unsigned long long)d : 0ull;
     }
@@ -173,8 +186,11 @@ int build(rtg_ctx *ctx, const rtg_graph *g, rtg_renum *r) {
     HIP_TRY(hipMalloc(&r->d_off, sizeof(int32_t) * ((size_t)n + 1)));
     HIP_TRY(hipMalloc(&r->d_adj, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)));
     HIP_TRY(hipMalloc(&r->d_adj16, sizeof(uint16_t) * (size_t)std::max<int64_t>(m, 1)));
-    HIP_TRY(hipMalloc(&r->d_fit16, (size_t)std::max<int32_t>(n, 1)));
+    HIP_TRY(hipMalloc(&r->d_vmax, sizeof(int32_t) * (size_t)std::max<int32_t>(n, 1)));
     HIP_TRY(hipMalloc(&r->d_inv, sizeof(int32_t) * (size_t)std::max<int32_t>(n, 1)));
+    const int64_t n256 = (m + 255) / 256;
+    HIP_TRY(hipMalloc(&r->d_s16, sizeof(int32_t) * (size_t)(16 * n256 + 16)));
+    HIP_TRY(hipMalloc(&r->d_s256, sizeof(int32_t) * (size_t)std::max<int64_t>(n256, 1)));
     const int bits = bits_of(std::max<int64_t>(n, 2));
     const int64_t kn = std::max<int64_t>(m, (int64_t)n + 1);
     const size_t bytes = Arena::round(8 * (size_t)kn) * 2 + Arena::round(radix_sort_work_bytes(kn)) +
@@ -212,8 +228,10 @@ int build(rtg_ctx *ctx, const rtg_graph *g, rtg_renum *r) {
     RTG_TRY(radix_sort_u64(ctx, kk, m, 2 * bits, &which, &passes, sort_work));
     split_entries<<<grid_of(ctx, m), kT, 0, st>>>(kk[which], m, bits, r->d_adj, r->d_adj16);
     HIP_TRY(hipGetLastError());
+    sample_rows<<<grid_of(ctx, 16 * n256 + 16), kT, 0, st>>>(r->d_adj, m, r->d_s16, r->d_s256);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemsetAsync(counts, 0, sizeof(unsigned long long) * 2, st));
-    fit16_rows<<<grid_of(ctx, n, 4), kT, 0, st>>>(r->d_off, r->d_adj, n, r->d_fit16, counts);
+    row_max<<<grid_of(ctx, n, 4), kT, 0, st>>>(r->d_off, r->d_adj, n, r->d_vmax, counts);
     HIP_TRY(hipGetLastError());
     unsigned long long hc[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -230,7 +248,7 @@ namespace rtg {
 
 void renum_free(rtg_renum *r) {
     if (!r) return;
-    void *p[] = {r->d_off, r->d_adj, r->d_adj16, r->d_fit16, r->d_inv, r->d_emap};
+    void *p[] = {r->d_off, r->d_adj, r->d_adj16, r->d_vmax, r->d_inv, r->d_emap, r->d_s16, r->d_s256};
     for (void *x : p)
         if (x) (void)hipFree(x);
     delete r;

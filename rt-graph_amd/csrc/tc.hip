@@ -851,14 +851,17 @@ __global__ __launch_bounds__(256) void plan_keys(const int32_t *__restrict__ off
 // each owner's first position. Keys of counting plans carry the partner itself, so only the
 // partner's row offsets are gathered; per-edge plans' keys carry the edge, whose endpoints are
 // read back.
-// fit16 (degree-ordered copies, else null): a partner whose row fits 16 bits gets bit 31 in its
-// span's start, and its elements are summed into *streamed16.
+// vmax, s16 and s256 (degree-ordered copies, else null: each row's largest entry and the samples
+// of every 16th and 256th entry): a span is cut at a multiple of 16 past which every element
+// exceeds the owner's largest entry, a span whose elements fit 16 bits gets bit 31 in its start,
+// and their elements are summed into *streamed16.
 __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
                            const int32_t *__restrict__ row, int64_t e_begin, const uint64_t *__restrict__ sorted,
                            int64_t nw, int bits, int2 *__restrict__ wspan, uint32_t *__restrict__ wedge,
-                           uint64_t *__restrict__ lens, uint32_t *__restrict__ first, const uint8_t *__restrict__ fit16,
+                           uint64_t *__restrict__ lens, uint32_t *__restrict__ first, const int32_t *__restrict__ vmax,
+                           const int32_t *__restrict__ s16, const int32_t *__restrict__ s256,
                            unsigned long long *__restrict__ streamed16, const int32_t *__restrict__ emap) {
-    uint64_t s16 = 0;
+    uint64_t s16sum = 0;
     const uint64_t mask = (1ull << bits) - 1;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     // kPlanU positions per thread and round, loads in phases (keys, then the edges' endpoints
@@ -872,49 +875,126 @@ __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__res
             key[k] = sorted[ic];
             prev[k] = ic > 0 ? sorted[ic - 1] : ~0ull;
         }
-        int32_t partner[kPlanU];
-        int64_t edge[kPlanU];
+        int32_t owner[kPlanU], partner[kPlanU];
+        uint32_t firsts = 0;  // bit k: position k starts its owner's run
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            owner[k] = (int32_t)(key[k] & mask);
+            firsts |= (i0 + k * stride == 0 || (int32_t)(prev[k] & mask) != owner[k]) ? 1u << k : 0u;
+        }
+        uint32_t wed[kPlanU];  // per-edge plans: the graph's edge of each position
         if (wedge) {
+            int64_t edge[kPlanU];
             int32_t eu[kPlanU], ev[kPlanU];
 #pragma unroll
             for (int k = 0; k < kPlanU; ++k) {
                 edge[k] = e_begin + (int64_t)(key[k] >> bits);
                 eu[k] = row[edge[k]];
                 ev[k] = adj[edge[k]];
+                wed[k] = emap ? (uint32_t)emap[edge[k]] : (uint32_t)edge[k];
             }
 #pragma unroll
-            for (int k = 0; k < kPlanU; ++k) partner[k] = (int32_t)(key[k] & mask) == eu[k] ? ev[k] : eu[k];
+            for (int k = 0; k < kPlanU; ++k) partner[k] = owner[k] == eu[k] ? ev[k] : eu[k];
         } else {
 #pragma unroll
             for (int k = 0; k < kPlanU; ++k) {
                 partner[k] = (int32_t)(key[k] >> bits);
-                edge[k] = 0;
+                wed[k] = 0;
             }
         }
-        int32_t p0[kPlanU], p1[kPlanU];
-        uint32_t f16[kPlanU];
+        int32_t p0[kPlanU], p1[kPlanU], pmax[kPlanU], hi[kPlanU];
 #pragma unroll
         for (int k = 0; k < kPlanU; ++k) {
             p0[k] = off[partner[k]];
             p1[k] = off[partner[k] + 1];
-            f16[k] = fit16 ? (uint32_t)fit16[partner[k]] : 0u;
+            pmax[k] = vmax ? vmax[partner[k]] : INT32_MAX;
+            hi[k] = vmax ? vmax[owner[k]] : INT32_MAX;
+        }
+        uint32_t f16[kPlanU];
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) f16[k] = pmax[k] < 65536 ? 1u : 0u;
+        if (vmax) {
+            // Degree-ordered copies: an element above the owner's largest entry hi matches
+            // nothing, and rows are sorted, so a span whose last element exceeds hi (and that
+            // holds a sample) is cut before its first sample of s16 above hi. The owner's
+            // smallest entry is near 0 in this order, so the head is kept. Level 1 finds by
+            // binary lifting (all kPlanU searches together) the first s256 sample of the row
+            // above hi; the 16 s16 samples of the chunk before it (one aligned 64-byte line,
+            // four independent loads) give the first s16 sample above hi. A span becomes 16-bit
+            // when that sample is at most 2^16: every kept element lies below it.
+            bool need[kPlanU];
+            int32_t jf[kPlanU], jl[kPlanU], nmax = 0;
+#pragma unroll
+            for (int k = 0; k < kPlanU; ++k) {
+                need[k] = p1[k] - p0[k] > 16 && pmax[k] > hi[k];
+                jf[k] = (p0[k] + 255) >> 8;  // s256 samples inside the row: jf .. jl
+                jl[k] = (p1[k] - 1) >> 8;
+                nmax = max(nmax, need[k] ? jl[k] - jf[k] + 1 : 0);
+            }
+            for (int32_t step = nmax > 0 ? (int32_t)(1u << (31 - __clz(nmax))) : 0; step; step >>= 1) {
+                int32_t v[kPlanU];
+#pragma unroll
+                for (int k = 0; k < kPlanU; ++k) v[k] = (need[k] && jf[k] + step <= jl[k] + 1) ? s256[jf[k] + step - 1] : INT32_MAX;
+#pragma unroll
+                for (int k = 0; k < kPlanU; ++k)
+                    if (v[k] <= hi[k]) jf[k] += step;
+            }
+            // jf: the row's first s256 sample above hi (jl + 1 if none); the chunk c before it
+            // (the last one when none) holds the first s16 sample above hi, unless it is sample
+            // 16 jf itself. Two positions' lines are in flight at a time (all four would double
+            // the registers).
+#pragma unroll
+            for (int h = 0; h < kPlanU; h += 2) {
+                int4 q[2][4];
+                int32_t c[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int k = h + u;
+                    c[u] = jf[k] <= jl[k] ? jf[k] - 1 : jl[k];
+                    const int4 *line = reinterpret_cast<const int4 *>(s16 + 16 * (int64_t)max(c[u], 0));
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        q[u][t] = (need[k] && c[u] >= 0) ? line[t] : make_int4(INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX);
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int k = h + u;
+                    if (!need[k]) continue;
+                    const int32_t i0s = (p0[k] + 15) >> 4, i1s = (p1[k] - 1) >> 4;  // s16 samples inside the row
+                    const int32_t base = 16 * c[u];
+                    const int32_t lo = max(i0s, base);
+                    int32_t cnt = 0, nextv = INT32_MAX;
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) {
+                        const int32_t x = (&q[u][t >> 2].x)[t & 3];
+                        const int32_t idx = base + t;
+                        const bool in = idx >= lo && idx <= i1s;
+                        cnt += (in && x <= hi[k]) ? 1 : 0;
+                        nextv = (in && x > hi[k] && nextv == INT32_MAX) ? x : nextv;
+                    }
+                    const int32_t is = lo + cnt;  // first s16 sample of the row above hi
+                    if (is > i1s) continue;       // none: the span is kept whole
+                    if (is == base + 16) nextv = s256[jf[k]];
+                    p1[k] = 16 * is;
+                    f16[k] |= nextv <= 65536 ? 1u : 0u;
+                }
+            }
         }
 #pragma unroll
         for (int k = 0; k < kPlanU; ++k) {
             const int64_t i = i0 + k * stride;
             if (i >= nw) break;
-            const int32_t owner = (int32_t)(key[k] & mask);
             const int32_t len = p1[k] - p0[k];
-            if (wedge) wedge[i] = emap ? (uint32_t)emap[edge[k]] : (uint32_t)edge[k];  // the graph's edge
+            if (wedge) wedge[i] = wed[k];
             wspan[i] = make_int2((int32_t)((uint32_t)p0[k] | (f16[k] << 31)), len);
             lens[i] = (uint64_t)len;
-            s16 += f16[k] ? (uint64_t)len : 0ull;
-            if (i == 0 || (int32_t)(prev[k] & mask) != owner) first[owner] = (uint32_t)i;
+            s16sum += f16[k] ? (uint64_t)len : 0ull;
+            if ((firsts >> k) & 1u) first[owner[k]] = (uint32_t)i;
         }
     }
-    if (fit16) {  // one device atomic per wave
-        s16 = rtgd::wave_sum_u64(s16);
-        if (__lane_id() == 0 && s16) atomicAdd(streamed16, (unsigned long long)s16);
+    if (vmax) {  // one device atomic per wave
+        s16sum = rtgd::wave_sum_u64(s16sum);
+        if (__lane_id() == 0 && s16sum) atomicAdd(streamed16, (unsigned long long)s16sum);
     }
 }
 
@@ -1115,8 +1195,8 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     if (p->opts.per_edge) RTG_TRY(dev_alloc(&p->d_wedge, (size_t)std::max<int64_t>(nw, 1)));
     if (nw > 0) {
         plan_spans<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(p->v_off, p->v_adj, p->d_row, p->e_begin, keys[which],
-                                                              nw, bits, p->d_wspan, p->d_wedge, lens, d_first, p->v_fit16,
-                                                              d_bytes + 5, p->v_emap);
+                                                              nw, bits, p->d_wspan, p->d_wedge, lens, d_first, p->v_vmax,
+                                                              p->v_s16, p->v_s256, d_bytes + 5, p->v_emap);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u64_w(ctx, lens, lens_ex, nw, nullptr, scan_work));
         plan_runs<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(keys[which], nw, bits, d_first, lens, lens_ex, d_cnt,
@@ -1132,11 +1212,17 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     }
     uint32_t flag = 0;
     unsigned long long s16 = 0;
+    uint64_t last[2] = {0, 0};  // the last span's length and exclusive prefix: all streamed elements
     HIP_TRY(hipMemcpyAsync(&flag, d_flag, sizeof(flag), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(&s16, d_bytes + 5, sizeof(s16), hipMemcpyDeviceToHost, st));
+    if (nw > 0) {
+        HIP_TRY(hipMemcpyAsync(&last[0], lens + (nw - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(&last[1], lens_ex + (nw - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    }
     HIP_TRY(hipStreamSynchronize(st));
     if (flag) return set_error(RTG_EHIP, "batched TC: inconsistent partner offsets in the plan");
     p->streamed16 = (int64_t)s16;
+    p->streamed = (int64_t)(last[0] + last[1]);
     return RTG_OK;
 }
 
@@ -1536,7 +1622,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 8;
+constexpr int kTuneRevision = 9;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
@@ -1800,10 +1886,12 @@ int rtg_tc_plan_create_impl(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts 
             const rtg_renum *r = nullptr;
             if ((rc = graph_renumbered(ctx, g, p->opts.per_edge != 0, &r)) != RTG_OK) break;
             p->v_emap = r->d_emap;
+            p->v_s16 = r->d_s16;
+            p->v_s256 = r->d_s256;
             p->v_off = r->d_off;
             p->v_adj = r->d_adj;
             p->v_adj16 = r->d_adj16;
-            p->v_fit16 = r->d_fit16;
+            p->v_vmax = r->d_vmax;
             p->renumbered = true;
             p->t_renum_ms = (built_copy ? r->t_build_ms : 0.0) + (built_map ? r->t_emap_ms : 0.0);
         }
@@ -1935,7 +2023,7 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
              kStratName[p->strat[3]] + ",all_strategies=" + (p->all_strategies ? "1" : "0") +
              ",plan_graph_ms=" + std::to_string(p->t_plan_graph_ms) + ",plan_tune_ms=" + std::to_string(p->t_plan_tune_ms) +
              ",plan_items_ms=" + std::to_string(p->t_plan_items_ms) + ",renumbered=" + (p->renumbered ? "1" : "0") +
-             ",renumber_ms=" + std::to_string(p->t_renum_ms) + ",streamed16=" + std::to_string(p->streamed16);
+             ",renumber_ms=" + std::to_string(p->t_renum_ms) + ",streamed=" + std::to_string(p->streamed) + ",streamed16=" + std::to_string(p->streamed16);
     std::snprintf(buf, len, "%s", d.c_str());
     return RTG_OK;
 }

@@ -72,10 +72,13 @@ struct rtg_tc_plan {
     const int32_t *v_off = nullptr;
     const int32_t *v_adj = nullptr;
     const uint16_t *v_adj16 = nullptr;
-    const uint8_t *v_fit16 = nullptr;
+    const int32_t *v_vmax = nullptr;   // the copy's row maxima
+    const int32_t *v_s16 = nullptr;    // the copy's every 16th entry
+    const int32_t *v_s256 = nullptr;   // the copy's every 256th entry
     const int32_t *v_emap = nullptr;   // per-edge plans on the copy: its edge -> the graph's edge
     bool renumbered = false;
-    int64_t streamed16 = 0;            // streamed partner elements in 16-bit spans
+    int64_t streamed = 0;              // streamed partner elements (after the cuts)
+    int64_t streamed16 = 0;            // of them, in 16-bit spans
     double t_renum_ms = 0.0;           // building the degree-ordered copy (first plan of a graph)
     std::string tune;                       // chosen configuration
     bool autotuned = false, tune_from_cache = false;

@@ -38,6 +38,9 @@ def _graphs():
     # degree-1 vertex, so the build has vertices (0 and 3) and no edge
     g["loops_only"] = np.array([0, 0, 0, 0, 1, 2, 3, 3, 3, 3, 3, 4], dtype=np.int32)
     g["ba_100000"] = rtg.gen_ba(100_000, 5, 1)
+    # dense and nearly regular (degrees about 440): out-lists of 150-300 entries whose largest
+    # entries fall anywhere in a partner's row, so span cuts land at every offset
+    g["gnp_dense"] = rng.integers(0, 2000, size=(500_000, 2)).astype(np.int32).reshape(-1)
     g["rmat_s12"] = rtg.gen_rmat(12, 16 << 12, seed=1)
     g["rmat_s16"] = rtg.gen_rmat(16, 44 << 16, seed=1)
     deg = rtg.gen_powerlaw_degrees(20_000, 2.1, 2_000.0, 20.0, seed=1)
@@ -517,7 +520,8 @@ def test_renumbering_default_and_opt_out(ctx, monkeypatch):
     off, adj, t = oracle_csr("ba_100000")  # about 500k oriented edges
     assert adj.size < (1 << 20)
     g = rtg.DeviceGraph.from_csr(ctx, off, adj)
-    assert g.tc_plan(method=rtg.TC_BATCHED).describe()["renumbered"] == "0"
+    d0 = g.tc_plan(method=rtg.TC_BATCHED).describe()
+    assert d0["renumbered"] == "0" and int(d0["streamed16"]) == 0
     monkeypatch.setenv("RTG_TC_RENUMBER", "1")
     p1 = g.tc_plan(method=rtg.TC_BATCHED)
     p2 = g.tc_plan(method=rtg.TC_BATCHED)
@@ -525,6 +529,11 @@ def test_renumbering_default_and_opt_out(ctx, monkeypatch):
     assert d1["renumbered"] == "1" and d2["renumbered"] == "1"
     assert float(d1["renumber_ms"]) > 0 and float(d2["renumber_ms"]) == 0  # built once
     assert int(d1["streamed16"]) > 0  # rows of the 65,536 highest-degree vertices' neighbours fit 16 bits
+    assert int(d1["streamed16"]) <= int(d1["streamed"])
+    # without cuts both CSRs stream the same elements (the shorter list of each edge); no partner
+    # list here is longer than 16 elements, the shortest the planner cuts
+    uncut = int(np.minimum(np.diff(off)[np.repeat(np.arange(off.size - 1), np.diff(off))], np.diff(off)[adj]).sum())
+    assert int(d0["streamed"]) == uncut and int(d1["streamed"]) == uncut
     assert p1.run()[0] == t and p2.run()[0] == t
     pe = g.tc_plan(method=rtg.TC_BATCHED, per_edge=True)
     dpe = pe.describe()
@@ -533,6 +542,65 @@ def test_renumbering_default_and_opt_out(ctx, monkeypatch):
     assert half.describe()["renumbered"] == "0"
     monkeypatch.setenv("RTG_TC_RENUMBER", "0")
     assert g.tc_plan(method=rtg.TC_BATCHED).describe()["renumbered"] == "0"
+
+
+def _degree_ordered(off, adj):
+    """The degree-ordered copy by its definition: vertices renamed by descending out + in degree
+    (ties by id), each row's entries renamed and sorted."""
+    n = off.size - 1
+    dplus = np.diff(off).astype(np.int64)
+    row = np.repeat(np.arange(n), dplus)
+    tot = dplus + np.bincount(adj, minlength=n)
+    rank = np.empty(n, np.int64)
+    rank[np.lexsort((np.arange(n), -tot))] = np.arange(n)
+    nrow, nadj = rank[row], rank[adj]
+    o = np.lexsort((nadj, nrow))
+    noff = np.zeros(n + 1, np.int64)
+    np.add.at(noff, nrow + 1, 1)
+    return np.cumsum(noff), nadj[o]
+
+
+def test_span_cuts_streamed(ctx, monkeypatch):
+    """The planner's span cuts on the degree-ordered copy, by a plain scan: every edge with both
+    out-lists non-empty streams its partner's row (the shorter list; ties to the smaller id's
+    row being the owner's), and a row longer than 16 entries whose last entry exceeds the
+    owner's largest entry hi ends before its first entry at a multiple-of-16 position above hi.
+    On a dense graph the cuts land at every offset of a row; the streamed total must match
+    exactly, and the count the oracle's."""
+    monkeypatch.setenv("RTG_TC_RENUMBER", "1")
+    off, adj, t = oracle_csr("gnp_dense")
+    noff, nadj = _degree_ordered(off, adj)
+    n = noff.size - 1
+    nd = np.diff(noff)
+    vmax = np.full(n, -1, np.int64)
+    vmax[nd > 0] = nadj[noff[1:][nd > 0] - 1]
+    nrow = np.repeat(np.arange(n), nd)
+    du, dv = nd[nrow], nd[nadj]
+    work = np.minimum(du, dv) > 0
+    own_u = (du > dv) | ((du == dv) & (nrow < nadj))
+    owner = np.where(own_u, nrow, nadj)[work]
+    partner = np.where(own_u, nadj, nrow)[work]
+    p0, p1, hi = noff[partner], noff[partner + 1], vmax[owner]
+    cut = p1.copy()
+    need = np.nonzero((p1 - p0 > 16) & (vmax[partner] > hi))[0]
+    assert need.size > 1000
+    for k in need:
+        pos = (p0[k] + 15) // 16 * 16
+        while pos < p1[k] and nadj[pos] <= hi[k]:
+            pos += 16
+        if pos < p1[k]:
+            cut[k] = pos
+    expect = int((cut - p0).sum())
+    assert expect < int((p1 - p0).sum())
+    g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+    p = g.tc_plan(method=rtg.TC_BATCHED)
+    d = p.describe()
+    assert d["renumbered"] == "1"
+    assert int(d["streamed"]) == expect
+    assert int(d["streamed16"]) == expect  # 2,000 vertices: every row fits 16 bits
+    assert p.run()[0] == t
+    p.close()
+    g.close()
 
 
 @pytest.mark.parametrize("tune", ["strat=hbbs", "strat=hhhs", "strat=bmbs", "strat=hbbs,unroll=4,long=1"])
