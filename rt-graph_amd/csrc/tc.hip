@@ -179,8 +179,9 @@ __host__ __device__ __forceinline__ uint32_t cuckoo_half(int32_t d) {
 __device__ __forceinline__ uint32_t hmul(uint32_t x, uint32_t k, uint32_t t) { return __umulhi(x * k, t); }
 
 // Probe modes of an owner table: two-choice cuckoo hash, linear probing (the hash's fallback),
-// a bitmap over the owner's value range, the owner's sorted list.
-constexpr uint32_t kCuckoo = 0, kLinearProbe = 1, kBitmap = 2, kSorted = 3;
+// a bitmap over the owner's value range, the owner's sorted list, and (degree-ordered copies
+// only, in place of kBitmap) a bitmap from 0 that covers at least every 16-bit value.
+constexpr uint32_t kCuckoo = 0, kLinearProbe = 1, kBitmap = 2, kSorted = 3, kBitmap16 = 4;
 
 struct Table {
     uint32_t *tab;
@@ -230,6 +231,28 @@ __device__ __forceinline__ uint32_t bitmap_has(const Table &c, uint32_t x) {
     // bit o of the table: word o >> 5, then the bit field extract's offset is o's low 5 bits
     return __builtin_amdgcn_ubfe(c.tab[__builtin_amdgcn_ubfe(o, 5u, 27u)], o, 1u);
 }
+
+// A bitmap from 0 over [0, t) with bit t clear, t = max(hi + 1, 2^16) for the owner's largest
+// value hi: a 16-bit value needs neither an offset nor a clamp.
+__host__ __device__ __forceinline__ uint32_t bitmap16_range(uint32_t hi) { return hi + 1u > 65536u ? hi + 1u : 65536u; }
+__host__ __device__ __forceinline__ uint32_t bitmap16_words(uint32_t hi) { return (bitmap16_range(hi) >> 5) + 1u; }
+
+// Bit o of a bitmap at LDS address `base`: the word's address in one shift-add (the compiler
+// otherwise spends a shift, a mask and an add), one LDS read, one bit field extract.
+__device__ __forceinline__ uint32_t bitmap_bit(uint32_t base, uint32_t o) {
+    uint32_t addr;
+    asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(addr) : "v"(o >> 5), "v"(base));
+    const uint32_t w = *reinterpret_cast<const __attribute__((address_space(3))) uint32_t *>((uintptr_t)addr);
+    return __builtin_amdgcn_ubfe(w, o, 1u);
+}
+
+__device__ __forceinline__ uint32_t lds_address(const uint32_t *p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) uint32_t *)p;
+}
+
+// kBitmap16 probes: any value (clamped to the clear bit t), and a 16-bit value (no clamp).
+__device__ __forceinline__ uint32_t bitmap16_has(const Table &c, uint32_t x) { return bitmap_bit(lds_address(c.tab), min(x, c.t)); }
+__device__ __forceinline__ uint32_t bitmap16_has16(const Table &c, uint32_t x) { return bitmap_bit(lds_address(c.tab), x); }
 
 // First index of tab[lo, lo + len) whose value is not below x (lo + len if none): binary lifting
 // whose trip count depends on len only, so the lanes of a wave stay together.
@@ -306,6 +329,22 @@ __device__ Table build_block_bitmap(uint32_t *tab, const int32_t *__restrict__ a
     return c;
 }
 
+// Block-wide bitmap from 0 of adj[xb .. xb+d) (kBitmap16; the caller has checked that its
+// bitmap16_words fit the table).
+template <int kThreads>
+__device__ Table build_block_bitmap16(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d) {
+    const uint32_t hi = (uint32_t)adj[xb + d - 1];
+    const uint32_t words = bitmap16_words(hi);
+    for (uint32_t i = threadIdx.x; i < words; i += kThreads) tab[i] = 0u;
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < d; i += kThreads) {
+        const uint32_t o = (uint32_t)adj[xb + i];
+        atomicOr(&tab[o >> 5], 1u << (o & 31u));
+    }
+    __syncthreads();
+    return Table{tab, bitmap16_range(hi), 0u, 0u, kBitmap16};
+}
+
 // Block-wide copy of the sorted list adj[xb .. xb+d) (d fits the table).
 template <int kThreads>
 __device__ Table build_block_sorted(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d) {
@@ -336,11 +375,25 @@ __device__ Table build_wave_table(uint32_t *tab, const int32_t *__restrict__ adj
 }
 
 // Wave-private bitmap or sorted copy (the caller has checked that it fits `cap` words; else it
-// builds the hash table).
+// builds the hash table). kB16: bitmaps from 0 (kBitmap16).
+template <bool kB16 = false>
 __device__ Table build_wave_other(uint32_t *tab, const int32_t *__restrict__ adj, int32_t xb, int32_t d, uint32_t mode,
                                   uint32_t cap) {
     const int lane = __lane_id();
-    if (mode == kBitmap) {
+    if (kB16 && mode == kBitmap) {
+        const uint32_t hi = (uint32_t)adj[xb + d - 1];
+        const uint32_t words = bitmap16_words(hi);
+        if (words <= cap) {
+            for (uint32_t i = lane; i < words; i += 64) tab[i] = 0u;
+            rtgd::wave_sync_lds();
+            for (int32_t i = lane; i < d; i += 64) {
+                const uint32_t o = (uint32_t)adj[xb + i];
+                atomicOr(&tab[o >> 5], 1u << (o & 31u));
+            }
+            rtgd::wave_sync_lds();
+            return Table{tab, bitmap16_range(hi), 0u, 0u, kBitmap16};
+        }
+    } else if (mode == kBitmap) {
         const uint32_t lo = (uint32_t)adj[xb], hi = (uint32_t)adj[xb + d - 1];
         const uint32_t words = bitmap_words(lo, hi);
         if (words <= cap) {
@@ -365,6 +418,7 @@ template <uint32_t kMode>
 __device__ __forceinline__ uint32_t probe(const Table &c, uint32_t x) {
     if constexpr (kMode == kLinearProbe) return linear_has(c, x);
     else if constexpr (kMode == kBitmap) return bitmap_has(c, x);
+    else if constexpr (kMode == kBitmap16) return bitmap16_has(c, x);
     else if constexpr (kMode == kSorted) return sorted_has(c, x);
     else return cuckoo_has(c, x);
 }
@@ -413,24 +467,85 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
 // ascend, so a round of 64*U consecutive elements can only match the owner-list window between
 // the lower bound of its first element and the upper bound of its last (two searches the wave
 // shares), each lane searches only that window, and the next round starts where it ended.
+// One group of W loads of kW-byte elements per lane from buffer resource rs (lane offsets vo + u
+// * 64 * kW), then their W probes: the loads all issue before the first probe waits.
+template <uint32_t kMode, int W, typename T>
+__device__ __forceinline__ uint32_t probe_group(const __amdgpu_buffer_rsrc_t &rs, int vo, const Table &c) {
+    constexpr int kW = (int)sizeof(T);
+    uint32_t x[W];
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+        if constexpr (kW == 4) x[u] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, vo + u * 64 * kW, 0, 0);
+        else x[u] = (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rs, vo + u * 64 * kW, 0, 0);
+    }
+    uint32_t h = 0;
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+        if constexpr (kMode == kBitmap16 && kW == 2) h += bitmap16_has16(c, x[u]);
+        else h += probe<kMode>(c, x[u]);
+    }
+    return h;
+}
+
+// The last, partial round of a span: nu (1 .. W) groups of 64 elements are left; the group of
+// the smallest power-of-two width >= nu runs. Returns its width in *w.
+template <uint32_t kMode, int W, typename T>
+__device__ __forceinline__ uint32_t tail_group(const __amdgpu_buffer_rsrc_t &rs, int vo, const Table &c, int32_t nu,
+                                               int32_t *w) {
+    if constexpr (W > 1) {
+        if (nu <= W / 2) return tail_group<kMode, W / 2, T>(rs, vo, c, nu, w);
+    }
+    *w = W;
+    return probe_group<kMode, W, T>(rs, vo, c);
+}
+
 // The rounds of one long span base[0 .. l0): U loads of kW-byte elements per lane through a raw
 // buffer resource rebased on the round's first element (scalar work per round; a lane's offsets
 // are loop-invariant plus an immediate) whose range ends with the span, so a lane past the end
 // reads 0 without a memory access and the rounds need no per-element bounds test or branch.
-// Returns the span's hits, including the probes of 0 of the last round's loads past the end;
-// *jb_end is where the rounds stopped.
+// Full rounds of 64*U elements come first; the last, partial round runs a group of 1, 2, 4 ..
+// U loads, the fewest that cover it (a span of 150 elements takes 256 probes per wave instead
+// of 512 at U = 8). Returns the span's hits, including the probes of 0 of the loads past the
+// end; *slots is the number of probes each lane made.
 template <uint32_t kMode, int U, typename T>
-__device__ __forceinline__ uint32_t long_rounds(const T *base, int32_t l0, const Table &c, int32_t *jb_end) {
+__device__ __forceinline__ uint32_t long_rounds(const T *base, int32_t l0, const Table &c, int32_t *slots) {
+    constexpr int kW = (int)sizeof(T);
+    int vo = __lane_id() * kW;
+    uint32_t h = 0;
+    int32_t jb = 0;
+    for (; l0 - jb >= 64 * U; jb += 64 * U) {
+        // An empty asm that "redefines" vo each round: without it the U offsets vo + u * 64 * kW
+        // are hoisted out of every loop into U registers kept live across the kernel (spilled
+        // at its register budget); with it they fold into the loads' immediate offsets.
+        asm volatile("" : "+v"(vo));
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base + jb), (short)0,
+                                                                            (l0 - jb) * kW, kBufferWord3);
+        h += probe_group<kMode, U, T>(rs, vo, c);
+    }
+    int32_t w = 0;
+    if (jb < l0) {
+        asm volatile("" : "+v"(vo));
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base + jb), (short)0,
+                                                                            (l0 - jb) * kW, kBufferWord3);
+        h += tail_group<kMode, U, T>(rs, vo, c, (l0 - jb + 63) >> 6, &w);
+    }
+    *slots = (jb >> 6) + w;
+    return h;
+}
+
+// The owner's sorted list in LDS merged with one long span base[0 .. l0) (kSorted), in rounds
+// of 64*U elements loaded as in long_rounds: both ascend, so a round can only match the
+// owner-list window between the lower bound of its first element and the upper bound of its
+// last (two searches the wave shares); each lane searches only that window (loads past the end
+// are not counted), and the next round starts where it ended.
+template <int U, typename T>
+__device__ __forceinline__ uint32_t merge_rounds(const T *base, int32_t l0, const Table &c) {
     constexpr int kW = (int)sizeof(T);
     const int lane = __lane_id();
     int vo = lane * kW;
     uint32_t h = 0;
-    int32_t jb = 0;
-    uint32_t w0 = 0;  // kSorted: owner-list position the span has reached
-    for (; jb < l0; jb += 64 * U) {
-        // An empty asm that "redefines" vo each round: without it the U offsets vo + u * 64 * kW
-        // are hoisted out of every loop into U registers kept live across the kernel (spilled
-        // at its register budget); with it they fold into the loads' immediate offsets.
+    uint32_t w0 = 0;  // owner-list position the span has reached
+    for (int32_t jb = 0; jb < l0; jb += 64 * U) {
         asm volatile("" : "+v"(vo));
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base + jb), (short)0,
                                                                             (l0 - jb) * kW, kBufferWord3);
@@ -442,28 +557,22 @@ __device__ __forceinline__ uint32_t long_rounds(const T *base, int32_t l0, const
             else
                 x[u] = (uint32_t)(uint16_t)__builtin_amdgcn_raw_buffer_load_b16(rs, vo + u * 64 * kW, 0, 0);
         }
-        if constexpr (kMode == kSorted) {
-            const int32_t last = min(64 * U, l0 - jb) - 1;  // the round's last element
-            uint32_t xl = x[0];
+        const int32_t last = min(64 * U, l0 - jb) - 1;  // the round's last element
+        uint32_t xl = x[0];
 #pragma unroll
-            for (int u = 1; u < U; ++u)
-                if ((last >> 6) == u) xl = x[u];
-            const uint32_t first_v = __builtin_amdgcn_readfirstlane(x[0]);
-            const uint32_t last_v = __builtin_amdgcn_readlane(xl, last & 63);
-            const uint32_t wa = rtgd::uniform_u32(lds_lower_bound(c.tab, w0, c.t - w0, first_v));
-            const uint32_t wb = rtgd::uniform_u32(lds_lower_bound(c.tab, wa, c.t - wa, last_v + 1u));
+        for (int u = 1; u < U; ++u)
+            if ((last >> 6) == u) xl = x[u];
+        const uint32_t first_v = __builtin_amdgcn_readfirstlane(x[0]);
+        const uint32_t last_v = __builtin_amdgcn_readlane(xl, last & 63);
+        const uint32_t wa = rtgd::uniform_u32(lds_lower_bound(c.tab, w0, c.t - w0, first_v));
+        const uint32_t wb = rtgd::uniform_u32(lds_lower_bound(c.tab, wa, c.t - wa, last_v + 1u));
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t q = lds_lower_bound(c.tab, wa, wb - wa, x[u]);
-                h += (jb + u * 64 + lane < l0 && q < wb && c.tab[min(q, c.t - 1)] == x[u]) ? 1u : 0u;
-            }
-            w0 = wb;
-        } else {
-#pragma unroll
-            for (int u = 0; u < U; ++u) h += probe<kMode>(c, x[u]);
+        for (int u = 0; u < U; ++u) {
+            const uint32_t q = lds_lower_bound(c.tab, wa, wb - wa, x[u]);
+            h += (jb + u * 64 + lane < l0 && q < wb && c.tab[min(q, c.t - 1)] == x[u]) ? 1u : 0u;
         }
+        w0 = wb;
     }
-    *jb_end = jb;
     return h;
 }
 
@@ -492,21 +601,21 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             const int32_t s0 = s0r & 0x7FFFFFFF;
             const int32_t l0 = __builtin_amdgcn_readlane(sp.y, p);
             uint32_t sh = 0;  // this span's hits (per-edge counts only)
-            int32_t jb = 0;
             uint32_t h;
-            if (kP16 && s0r < 0) h = long_rounds<kMode, U>(adj16 + s0, l0, c, &jb);
-            else h = long_rounds<kMode, U>(adj + s0, l0, c, &jb);
+            if constexpr (kMode == kSorted) {
+                if (kP16 && s0r < 0) h = merge_rounds<U>(adj16 + s0, l0, c);
+                else h = merge_rounds<U>(adj + s0, l0, c);
+            } else {
+                int32_t slots = 0;
+                if (kP16 && s0r < 0) h = long_rounds<kMode, U>(adj16 + s0, l0, c, &slots);
+                else h = long_rounds<kMode, U>(adj + s0, l0, c, &slots);
+                // This lane's loads past the end each probed 0 and hit iff the owner's list holds
+                // 0 (has0): take them back. The lane's elements: positions lane, lane + 64, ...
+                const int32_t in = lane < l0 ? (l0 - lane + 63) >> 6 : 0;
+                h -= has0 * (uint32_t)(slots - in);
+            }
             if constexpr (kPerEdge) sh += h;
             else hits += h;
-            if constexpr (kMode != kSorted) {
-                // This lane's loads past the end in the last round each probed 0 and hit iff
-                // the owner's list holds 0 (has0): take them back.
-                const int32_t r = l0 - (jb - 64 * U) - lane;
-                const int32_t in = r <= 0 ? 0 : min(U, (r + 63) >> 6);
-                const uint32_t pad = has0 * (uint32_t)(U - in);
-                if constexpr (kPerEdge) sh -= pad;
-                else hits -= pad;
-            }
             if constexpr (kPerEdge) {
                 hits += sh;
                 sh = rtgd::wave_sum_u32(sh);
@@ -598,8 +707,8 @@ __device__ __forceinline__ uint32_t stream_table(const int32_t *__restrict__ adj
     const uint32_t mode = rtgd::uniform_u32(c.mode);
     if constexpr (kP16) {
         static_assert(kAll, "16-bit spans come with every strategy");
-        if (mode == kBitmap)
-            return stream_spans<kBitmap, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+        if (mode == kBitmap16)
+            return stream_spans<kBitmap16, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
         if (mode == kCuckoo)
             return stream_spans<kCuckoo, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
         return stream_spans_ool<kP16, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16, mode);
@@ -677,7 +786,9 @@ __global__ __launch_bounds__(kThreads, kP16 ? 6 : (kPerEdge || kAll) ? 8 : 1) vo
             // The planner only gives these strategies to owners whose bitmap or list fits the
             // table; anything else is searched in HBM rather than written past it.
             const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
-            const uint32_t need = kind == kItemSorted ? (uint32_t)d : bitmap_words((uint32_t)a.adj[xb], (uint32_t)a.adj[xb + d - 1]);
+            const uint32_t need = kind == kItemSorted ? (uint32_t)d
+                                  : kP16 ? bitmap16_words((uint32_t)a.adj[xb + d - 1])
+                                         : bitmap_words((uint32_t)a.adj[xb], (uint32_t)a.adj[xb + d - 1]);
             if (!kAll || rtgd::uniform_u32(need) > (uint32_t)a.max_table) kind = kItemSearch;
         }
         if (kind == kItemHash || kind == kItemBitmap || kind == kItemSorted) {
@@ -685,7 +796,8 @@ __global__ __launch_bounds__(kThreads, kP16 ? 6 : (kPerEdge || kAll) ? 8 : 1) vo
             const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
             Table c;
             if (!kAll || kind == kItemHash) c = build_block_table<kThreads>(table, a.adj, xb, d, &fail_flag);
-            else if (kind == kItemBitmap) c = build_block_bitmap<kThreads>(table, a.adj, xb, d);
+            else if (kind == kItemBitmap) c = kP16 ? build_block_bitmap16<kThreads>(table, a.adj, xb, d)
+                                                   : build_block_bitmap<kThreads>(table, a.adj, xb, d);
             else c = build_block_sorted<kThreads>(table, a.adj, xb, d);
             while (true) {
                 int32_t g = 0;
@@ -719,7 +831,7 @@ __global__ __launch_bounds__(kThreads, kP16 ? 6 : (kPerEdge || kAll) ? 8 : 1) vo
                 const int32_t xs = a.small_list[s];
                 const int32_t xb = a.off[xs], d = a.off[xs + 1] - xb;
                 const Table c = mode == kCuckoo ? build_wave_table(tab, a.adj, xb, d)
-                                                : build_wave_other(tab, a.adj, xb, d, mode, (uint32_t)a.wave_table);
+                                                : build_wave_other<kP16>(tab, a.adj, xb, d, mode, (uint32_t)a.wave_table);
                 hits += stream_table<U, kPerEdge, kAll, kP16>(a.adj, a.wspan, a.wofs[xs], a.wofs[xs + 1], c, a.long_min,
                                                               ex, st, a.wedge, a.eout, pe, a.adj16);
                 rtgd::wave_sync_lds();
@@ -1250,6 +1362,7 @@ struct QueueParams {
     int32_t n;
     int32_t wave_owner_max, fit_block, max_table, wave_table;
     int32_t only_bucket;             // -1: every owner; else only this bucket's (autotuning)
+    int32_t bitmap16;                // degree-ordered copy: bitmaps run from 0 (kBitmap16)
     int64_t item_work;
     int64_t wave_item_work;          // a small owner is heavy when its work reaches this
     int32_t strat[kBuckets];
@@ -1281,7 +1394,8 @@ __device__ __forceinline__ OwnerPlan owner_plan(const QueueParams &q, int32_t x)
         return o;
     }
     const int32_t s = q.strat[o.bucket];
-    const uint32_t words = bitmap_words((uint32_t)q.adj[q.off[x]], (uint32_t)q.adj[q.off[x] + o.d - 1]);
+    const uint32_t hi = (uint32_t)q.adj[q.off[x] + o.d - 1];
+    const uint32_t words = q.bitmap16 ? bitmap16_words(hi) : bitmap_words((uint32_t)q.adj[q.off[x]], hi);
     if (s == kStratBitmap && words <= (uint32_t)q.max_table) o.kind = kItemBitmap;
     else if (s == kStratMerge && o.d <= q.max_table) o.kind = kItemSorted;
     else if (s == kStratSearch || o.d > q.fit_block) o.kind = kItemSearch;
@@ -1461,7 +1575,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     const int waves = threads / 64;
     const TableSizes ts = table_sizes(tune, g->max_row);
     QueueParams q{p->v_off, p->v_adj, p->d_wofs, p->d_work, n, ts.wave_owner_max, ts.fit_block, ts.max_table,
-                  ts.wave_table, only_bucket, tune.item_work, tune.wave_item_work, {}};
+                  ts.wave_table, only_bucket, p->renumbered ? 1 : 0, tune.item_work, tune.wave_item_work, {}};
     for (int b = 0; b < kBuckets; ++b) q.strat[b] = tune.strat[b];
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->d_small) (void)hipFree(p->d_small);
@@ -1622,7 +1736,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 9;
+constexpr int kTuneRevision = 10;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
