@@ -743,8 +743,10 @@ struct BatchArgs {
     const uint16_t *adj16;      // 16-bit copy of adj (degree-ordered plans, else null)
 };
 
-// The 16-bit instantiation (degree-ordered copies) is held to 6 waves per SIMD: at 8 it spills
-// about 60 VGPRs and takes 14.5 ms on R-MAT scale 21, at 6 13.7 ms, at 7 13.9, at 5 14.9.
+// The 16-bit instantiation (degree-ordered copies) is held to 7 waves per SIMD (72 VGPRs, a few
+// dozen spilled outside the streaming loops): on R-MAT scale 21 it takes 10.07 ms, at 6 waves
+// 10.39, at 8 10.52. The launch waits on memory (two thirds of wave time), so the seventh wave's
+// loads in flight pay for its spills; at 8 the spills cost more.
 // The per-edge instantiation is held to 8 waves per SIMD (at most 64 VGPRs, a few spilled to
 // scratch outside the streaming loops): left alone it takes 74 and runs 6 waves, which costs
 // more than the spills (R-MAT scale 21: 23.7 against 25.3 ms).
@@ -753,7 +755,7 @@ struct BatchArgs {
 // hold more registers across the hash path and cost it occupancy); there, bitmap and sorted
 // items would be searched and wave items hashed, which gives the same counts.
 template <int kThreads, int U, bool kPerEdge, bool kAll, bool kP16 = false>
-__global__ __launch_bounds__(kThreads, kP16 ? 6 : (kPerEdge || kAll) ? 8 : 1) void tc_batched(BatchArgs a) {
+__global__ __launch_bounds__(kThreads, kP16 ? 7 : (kPerEdge || kAll) ? 8 : 1) void tc_batched(BatchArgs a) {
     constexpr int kWaves = kThreads / 64;
     extern __shared__ uint32_t table[];
     __shared__ int32_t ex_all[kWaves][65];
@@ -1736,7 +1738,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 10;
+constexpr int kTuneRevision = 11;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
