@@ -101,6 +101,13 @@ int rtg_graph_from_csr(rtg_ctx *ctx, const int32_t *offsets, const int32_t *adjs
                        int64_t m, rtg_graph **out);
 int rtg_graph_info(const rtg_graph *g, int32_t *n, int64_t *m, int32_t *max_row);
 int rtg_graph_download(const rtg_graph *g, int32_t *offsets /* n+1 */, int32_t *adjs /* m */);
+/* The degree-ordered copy that batched counting plans over the whole graph count on (no
+   counterpart in the reference, which counts on the CSR itself): vertices renamed in
+   descending order of total degree (out + in, ties by id), rows renamed and sorted;
+   inv[i] = the graph's id of copy vertex i. Built on first use and kept with the graph.
+   Any output may be NULL. Non-empty oriented graphs only.                                 */
+int rtg_graph_ordered_csr(rtg_ctx *ctx, const rtg_graph *g, int32_t *offsets /* n+1 */, int32_t *adjs /* m */,
+                          int32_t *inv /* n */);
 int rtg_graph_device_ptrs(const rtg_graph *g, const int32_t **d_offsets, const int32_t **d_adjs);
 int rtg_graph_destroy(rtg_graph *g);
 

@@ -272,6 +272,15 @@ class DeviceGraph:
         _check(_lib.rtg_graph_download(self._h, _ptr(off), _ptr(adj)))
         return off, adj
 
+    def ordered_csr(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """rtg_graph_ordered_csr: the degree-ordered copy batched counting plans count on (offsets,
+        adjs) and inv (the graph's id of each copy vertex); built on first use."""
+        off = np.empty(self.n + 1, dtype=np.int32)
+        adj = np.empty(self.m, dtype=np.int32)
+        inv = np.empty(self.n, dtype=np.int32)
+        _check(_lib.rtg_graph_ordered_csr(self.ctx.handle, self._h, _ptr(off), _ptr(adj), _ptr(inv)))
+        return off, adj, inv
+
     def device_ptrs(self) -> tuple[int, int]:
         o, a = C.c_void_p(), C.c_void_p()
         _check(_lib.rtg_graph_device_ptrs(self._h, C.byref(o), C.byref(a)))

@@ -895,3 +895,11 @@ RTG_API int rtg_graph_destroy(rtg_graph *g) {
 }
 
 }  // extern "C"
+
+namespace rtg {
+// Loads this file's kernels onto the current device (an attribute query of one of them).
+int preload_csr_build() {
+    hipFuncAttributes attr;
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&minmax_kernel)) == hipSuccess ? RTG_OK : RTG_EHIP;
+}
+}  // namespace rtg

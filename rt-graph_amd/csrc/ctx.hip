@@ -80,6 +80,14 @@ RTG_API int rtg_create(int device, rtg_ctx **out) {
             if (hipEventCreate(&e) != hipSuccess) { st = rtg_fail(RTG_EHIP, "hipEventCreate failed"); break; }
         if (st != RTG_OK) break;
         (void)rtg::si_noalloc_init(device);  // status kept per device; rtg_si_pairs_noalloc reports it
+        // every other source file's kernels (si.hip's are loaded by the symbol query above)
+        for (int (*load)() : {rtg::preload_csr_build, rtg::preload_gen_device, rtg::preload_lookup, rtg::preload_radix_sort,
+                              rtg::preload_renumber, rtg::preload_scan, rtg::preload_serve, rtg::preload_tc,
+                              rtg::preload_tc_2a1})
+            if (load() != RTG_OK) {
+                st = rtg_fail(RTG_EHIP, "rtg_create: loading the kernels failed");
+                break;
+            }
     } while (0);
     if (st != RTG_OK) {
         rtg_destroy(ctx);

@@ -54,6 +54,19 @@ namespace rtg {
 // Makes the context's device current on this thread.
 int ctx_enter(rtg_ctx *ctx);
 
+// One per HIP source file: loads that file's kernels onto the current device (HIP loads a code
+// object on the first launch of any of its kernels, a few milliseconds for the largest), so
+// that rtg_create pays it once instead of the first build, plan or query.
+int preload_csr_build();
+int preload_gen_device();
+int preload_lookup();
+int preload_radix_sort();
+int preload_renumber();
+int preload_scan();
+int preload_serve();
+int preload_tc();
+int preload_tc_2a1();
+
 // Returns at least `bytes` of device scratch owned by the context (contents undefined).
 int ctx_scratch(rtg_ctx *ctx, size_t bytes, void **out);
 

@@ -200,3 +200,11 @@ RTG_API int rtg_gen_uniform_u32_device(rtg_ctx *ctx, int64_t n, uint64_t seed, u
 }
 
 }  // extern "C"
+
+namespace rtg {
+// Loads this file's kernels onto the current device (an attribute query of one of them).
+int preload_gen_device() {
+    hipFuncAttributes attr;
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&rmat_kernel)) == hipSuccess ? RTG_OK : RTG_EHIP;
+}
+}  // namespace rtg

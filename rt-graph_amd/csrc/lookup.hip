@@ -532,3 +532,11 @@ RTG_API int rtg_lookup_index_destroy(rtg_lookup_index *ix) {
 }
 
 }  // extern "C"
+
+namespace rtg {
+// Loads this file's kernels onto the current device (an attribute query of one of them).
+int preload_lookup() {
+    hipFuncAttributes attr;
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&lookup_sampled<uint32_t>)) == hipSuccess ? RTG_OK : RTG_EHIP;
+}
+}  // namespace rtg

@@ -234,3 +234,11 @@ int radix_sort_32(rtg_ctx *ctx, uint32_t *keys[2], int64_t n, bool is_signed, in
 }
 
 }  // namespace rtg
+
+namespace rtg {
+// Loads this file's kernels onto the current device (an attribute query of one of them).
+int preload_radix_sort() {
+    hipFuncAttributes attr;
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&digit_histogram<uint64_t, 8>)) == hipSuccess ? RTG_OK : RTG_EHIP;
+}
+}  // namespace rtg

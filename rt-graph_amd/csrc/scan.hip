@@ -172,3 +172,11 @@ int scan_exclusive_u64_w(rtg_ctx *ctx, const uint64_t *in, uint64_t *out, int64_
 }
 
 }  // namespace rtg
+
+namespace rtg {
+// Loads this file's kernels onto the current device (an attribute query of one of them).
+int preload_scan() {
+    hipFuncAttributes attr;
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&tile_reduce<uint32_t>)) == hipSuccess ? RTG_OK : RTG_EHIP;
+}
+}  // namespace rtg

@@ -872,3 +872,11 @@ RTG_API int rtg_stream_queries(const rtg_graph *g, const rtg_stream_spec *spec, 
 }
 
 }  // extern "C"
+
+namespace rtg {
+// Loads this file's kernels onto the current device (an attribute query of one of them).
+int preload_serve() {
+    hipFuncAttributes attr;
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&fill_zero_u32)) == hipSuccess ? RTG_OK : RTG_EHIP;
+}
+}  // namespace rtg

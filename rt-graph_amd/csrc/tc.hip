@@ -2211,3 +2211,11 @@ RTG_API int rtg_tc_count(rtg_ctx *ctx, const rtg_graph *g, const rtg_tc_opts *op
 }
 
 }  // extern "C"
+
+namespace rtg {
+// Loads this file's kernels onto the current device (an attribute query of one of them).
+int preload_tc() {
+    hipFuncAttributes attr;
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&edge_rows_kernel)) == hipSuccess ? RTG_OK : RTG_EHIP;
+}
+}  // namespace rtg

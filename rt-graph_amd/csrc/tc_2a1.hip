@@ -159,3 +159,11 @@ int rtg_tc_count_2a1_impl(rtg_ctx *ctx, const rtg_graph *g, uint64_t *count, rtg
     }
     return RTG_OK;
 }
+
+namespace rtg {
+// Loads this file's kernels onto the current device (an attribute query of one of them).
+int preload_tc_2a1() {
+    hipFuncAttributes attr;
+    return hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&wedge_counts)) == hipSuccess ? RTG_OK : RTG_EHIP;
+}
+}  // namespace rtg

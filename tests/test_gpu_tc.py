@@ -546,18 +546,39 @@ def test_renumbering_default_and_opt_out(ctx, monkeypatch):
 
 def _degree_ordered(off, adj):
     """The degree-ordered copy by its definition: vertices renamed by descending out + in degree
-    (ties by id), each row's entries renamed and sorted."""
+    (ties by id), each row's entries renamed and sorted; inv[i] = the old id of new vertex i."""
     n = off.size - 1
     dplus = np.diff(off).astype(np.int64)
     row = np.repeat(np.arange(n), dplus)
     tot = dplus + np.bincount(adj, minlength=n)
+    inv = np.lexsort((np.arange(n), -tot))
     rank = np.empty(n, np.int64)
-    rank[np.lexsort((np.arange(n), -tot))] = np.arange(n)
+    rank[inv] = np.arange(n)
     nrow, nadj = rank[row], rank[adj]
     o = np.lexsort((nadj, nrow))
     noff = np.zeros(n + 1, np.int64)
     np.add.at(noff, nrow + 1, 1)
-    return np.cumsum(noff), nadj[o]
+    return np.cumsum(noff), nadj[o], inv
+
+
+@pytest.mark.parametrize("name", ["hand", "k40", "wheel", "gnm_dups", "ba_100000", "rmat_s12", "rmat_s16", "plcm",
+                                  "hub", "gnp_dense"])
+def test_ordered_csr_matches_definition(ctx, monkeypatch, name):
+    """The degree-ordered copy, built by sorting every row where its new row starts (rows of up
+    to 64 entries across one wave's lanes, up to 512 in a wave's LDS buffer, longer ones in a
+    block's: "hub" has rows of 1,199 and 2,500 entries) and by the key sort over all edges
+    (RTG_RENUMBER_KEY_SORT=1, the path of graphs with rows beyond 4,096), against its
+    definition restated in numpy."""
+    off, adj, _ = oracle_csr(name)
+    eoff, eadj, einv = _degree_ordered(off, adj)
+    for key_sort in ("0", "1"):
+        monkeypatch.setenv("RTG_RENUMBER_KEY_SORT", key_sort)
+        g = rtg.DeviceGraph.from_csr(ctx, off, adj)
+        o, a, inv = g.ordered_csr()
+        assert np.array_equal(o, eoff), (name, key_sort)
+        assert np.array_equal(a, eadj), (name, key_sort)
+        assert np.array_equal(inv, einv), (name, key_sort)
+        g.close()
 
 
 def test_span_cuts_streamed(ctx, monkeypatch):
@@ -569,7 +590,7 @@ def test_span_cuts_streamed(ctx, monkeypatch):
     exactly, and the count the oracle's."""
     monkeypatch.setenv("RTG_TC_RENUMBER", "1")
     off, adj, t = oracle_csr("gnp_dense")
-    noff, nadj = _degree_ordered(off, adj)
+    noff, nadj, _ = _degree_ordered(off, adj)
     n = noff.size - 1
     nd = np.diff(noff)
     vmax = np.full(n, -1, np.int64)
