@@ -651,8 +651,9 @@ def main() -> int:
         "config": {"workload": args.workload, "generator": f"R-MAT scale {scale}, edge factor {ef}, "
                    "(0.57,0.19,0.19,0.05), seed 1", "n": g.n, "m_oriented": g.m, "triangles": cnt,
                    "method": "batched on the degree-ordered copy: owners' lists as LDS bitmaps (block owners) or "
-                             "two-choice cuckoo tables (small owners), 16-bit partner lists where they fit, "
-                             "persistent work queue, hipGraph",
+                             "two-choice cuckoo tables (small owners), each kind in its own persistent kernel, "
+                             "partner lists cut past the owner's largest entry and streamed at 16 bits where they "
+                             "fit, work queues, hipGraph",
                    "plan": desc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,

@@ -59,8 +59,9 @@ __global__ __launch_bounds__(256) void edge_rows_kernel(const int32_t *__restric
     }
 }
 
+// [0] count, [1] the general queue's head, [2] the bitmap queue's, [3] the wave items' queue's.
 __global__ void reset_counters(unsigned long long *c) {
-    if (threadIdx.x < 2) c[threadIdx.x] = 0ull;
+    if (threadIdx.x < 4) c[threadIdx.x] = 0ull;
 }
 
 // BinaryCheck, tc/bs_tc/bs_tc.cu:14-25, on data[0 .. len-1].
@@ -851,6 +852,62 @@ __global__ __launch_bounds__(kThreads, kP16 ? 7 : (kPerEdge || kAll) ? 8 : 1) vo
     if (lane == 0 && hits) atomicAdd(&a.count[0], (unsigned long long)hits);
 }
 
+// Block bitmap items of a degree-ordered copy only (kItemBitmap, kBitmap16 tables), from the
+// queue a.items[0 .. n_items) with its own head a.count[2]: the bitmap path without the
+// registers the other strategies hold across it.
+template <int kThreads, int U, bool kPerEdge>
+__global__ __launch_bounds__(kThreads, 8) void tc_bitmap_items(BatchArgs a) {
+    constexpr int kWaves = kThreads / 64;
+    extern __shared__ uint32_t table[];
+    __shared__ int32_t ex_all[kWaves][65];
+    __shared__ int32_t st_all[kWaves][64];
+    __shared__ uint32_t pe_all[kPerEdge ? kWaves : 1][kPerEdge ? 64 : 1];
+    __shared__ uint64_t item_slot;
+    __shared__ int32_t next_batch;
+    const int lane = __lane_id();
+    const int wave = (int)rtgd::uniform_u32(threadIdx.x >> 6);
+    int32_t *ex = ex_all[wave];
+    int32_t *st = st_all[wave];
+    uint32_t *pe = pe_all[kPerEdge ? wave : 0];
+    uint64_t hits = 0;
+    if (threadIdx.x == 0) {
+        item_slot = atomicAdd(&a.count[2], 1ull);
+        next_batch = 0;
+    }
+    __syncthreads();
+    uint64_t it_idx = rtgd::uniform_u64(item_slot);
+    while (it_idx < (uint64_t)a.n_items) {
+        const TcItem it = a.items[it_idx];
+        const int32_t x = it.a;
+        const int32_t xb = a.off[x], d = a.off[x + 1] - xb;
+        const bool fits = rtgd::uniform_u32(bitmap16_words((uint32_t)a.adj[xb + d - 1])) <= (uint32_t)a.max_table;
+        Table c{};
+        if (fits) c = build_block_bitmap16<kThreads>(table, a.adj, xb, d);
+        while (true) {
+            int32_t g = 0;
+            if (lane == 0) g = atomicAdd(&next_batch, a.grab);
+            const int32_t b = (int32_t)rtgd::uniform_u32((uint32_t)g);
+            const int64_t p0 = (int64_t)it.b + b;
+            if (p0 >= it.c) break;
+            const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
+            if (fits)
+                hits += stream_spans<kBitmap16, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
+                                                                   a.eout, pe, a.adj16);
+            else
+                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d, a.wedge, a.eout);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            item_slot = atomicAdd(&a.count[2], 1ull);
+            next_batch = 0;
+        }
+        __syncthreads();
+        it_idx = rtgd::uniform_u64(item_slot);
+    }
+    hits = rtgd::wave_sum_u64(hits);
+    if (lane == 0 && hits) atomicAdd(&a.count[0], (unsigned long long)hits);
+}
+
 using BatchKernel = void (*)(BatchArgs);
 
 // Instantiated (block size, unroll) pairs of the batched kernel, counting or per-edge, with the
@@ -876,6 +933,82 @@ BatchKernel batched_kernel_p16(int threads, int unroll) {
     if (threads == 512 && unroll == 4) return tc_batched<512, 4, kPerEdge, true, true>;
     if (threads == 512 && unroll == 8) return tc_batched<512, 8, kPerEdge, true, true>;
     return nullptr;
+}
+
+// Wave items only (small owners in wave-private cuckoo tables, linear probing as their
+// fallback) of a degree-ordered plan whose small owners are hashed, from the queue a.items[0 ..
+// n_items) with its own head a.count[3].
+template <int kThreads, int U, bool kPerEdge>
+__global__ __launch_bounds__(kThreads, 8) void tc_wave_items(BatchArgs a) {
+    constexpr int kWaves = kThreads / 64;
+    extern __shared__ uint32_t table[];
+    __shared__ int32_t ex_all[kWaves][65];
+    __shared__ int32_t st_all[kWaves][64];
+    __shared__ uint32_t pe_all[kPerEdge ? kWaves : 1][kPerEdge ? 64 : 1];
+    __shared__ uint64_t item_slot;
+    const int wave = (int)rtgd::uniform_u32(threadIdx.x >> 6);
+    int32_t *ex = ex_all[wave];
+    int32_t *st = st_all[wave];
+    uint32_t *pe = pe_all[kPerEdge ? wave : 0];
+    uint32_t *tab = table + wave * a.wave_table;
+    uint64_t hits = 0;
+    if (threadIdx.x == 0) item_slot = atomicAdd(&a.count[3], 1ull);
+    __syncthreads();
+    uint64_t it_idx = rtgd::uniform_u64(item_slot);
+    while (it_idx < (uint64_t)a.n_items) {
+        const TcItem it = a.items[it_idx];
+        for (int32_t s = it.b + wave; s < it.c; s += kWaves) {
+            const int32_t xs = a.small_list[s];
+            const int32_t xb = a.off[xs], d = a.off[xs + 1] - xb;
+            const Table c = build_wave_table(tab, a.adj, xb, d);
+            const int64_t p0 = a.wofs[xs], p1 = a.wofs[xs + 1];
+            if (rtgd::uniform_u32(c.mode) == kCuckoo)
+                hits += stream_spans<kCuckoo, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
+                                                                 a.eout, pe, a.adj16);
+            else
+                hits += stream_spans<kLinearProbe, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st,
+                                                                      a.wedge, a.eout, pe, a.adj16);
+            rtgd::wave_sync_lds();
+        }
+        __syncthreads();  // every wave is done with item_slot
+        if (threadIdx.x == 0) item_slot = atomicAdd(&a.count[3], 1ull);
+        __syncthreads();
+        it_idx = rtgd::uniform_u64(item_slot);
+    }
+    hits = rtgd::wave_sum_u64(hits);
+    if (__lane_id() == 0 && hits) atomicAdd(&a.count[0], (unsigned long long)hits);
+}
+
+// The block-bitmap-only kernel of a degree-ordered plan, counting or per-edge.
+template <bool kPerEdge>
+BatchKernel bitmap_kernel_t(int threads, int unroll) {
+    if (threads == 256 && unroll == 2) return tc_bitmap_items<256, 2, kPerEdge>;
+    if (threads == 256 && unroll == 4) return tc_bitmap_items<256, 4, kPerEdge>;
+    if (threads == 256 && unroll == 8) return tc_bitmap_items<256, 8, kPerEdge>;
+    if (threads == 512 && unroll == 2) return tc_bitmap_items<512, 2, kPerEdge>;
+    if (threads == 512 && unroll == 4) return tc_bitmap_items<512, 4, kPerEdge>;
+    if (threads == 512 && unroll == 8) return tc_bitmap_items<512, 8, kPerEdge>;
+    return nullptr;
+}
+
+BatchKernel bitmap_kernel(int threads, int unroll, bool per_edge) {
+    return per_edge ? bitmap_kernel_t<true>(threads, unroll) : bitmap_kernel_t<false>(threads, unroll);
+}
+
+// The wave-items-only kernel of a degree-ordered plan, counting or per-edge.
+template <bool kPerEdge>
+BatchKernel wave_kernel_t(int threads, int unroll) {
+    if (threads == 256 && unroll == 2) return tc_wave_items<256, 2, kPerEdge>;
+    if (threads == 256 && unroll == 4) return tc_wave_items<256, 4, kPerEdge>;
+    if (threads == 256 && unroll == 8) return tc_wave_items<256, 8, kPerEdge>;
+    if (threads == 512 && unroll == 2) return tc_wave_items<512, 2, kPerEdge>;
+    if (threads == 512 && unroll == 4) return tc_wave_items<512, 4, kPerEdge>;
+    if (threads == 512 && unroll == 8) return tc_wave_items<512, 8, kPerEdge>;
+    return nullptr;
+}
+
+BatchKernel wave_kernel(int threads, int unroll, bool per_edge) {
+    return per_edge ? wave_kernel_t<true>(threads, unroll) : wave_kernel_t<false>(threads, unroll);
 }
 
 BatchKernel batched_kernel(int threads, int unroll, bool per_edge, bool all_strategies, bool p16 = false) {
@@ -1375,6 +1508,7 @@ struct QueueParams {
 struct OwnerPlan {
     int32_t bucket, d, c;
     uint32_t kind;
+    uint32_t words;  // the owner's bitmap words (its table if a bitmap item)
     int64_t chunks;
     uint64_t w;
     bool small;
@@ -1398,6 +1532,7 @@ __device__ __forceinline__ OwnerPlan owner_plan(const QueueParams &q, int32_t x)
     const int32_t s = q.strat[o.bucket];
     const uint32_t hi = (uint32_t)q.adj[q.off[x] + o.d - 1];
     const uint32_t words = q.bitmap16 ? bitmap16_words(hi) : bitmap_words((uint32_t)q.adj[q.off[x]], hi);
+    o.words = words;
     if (s == kStratBitmap && words <= (uint32_t)q.max_table) o.kind = kItemBitmap;
     else if (s == kStratMerge && o.d <= q.max_table) o.kind = kItemSorted;
     else if (s == kStratSearch || o.d > q.fit_block) o.kind = kItemSearch;
@@ -1411,10 +1546,12 @@ __device__ __forceinline__ OwnerPlan owner_plan(const QueueParams &q, int32_t x)
 // acc[1 + b] items of bucket b (owners for bucket 0), acc[5] search items, acc[6] items with a
 // bitmap or sorted-list table (including small owners whose wave mode is one of them), acc[7]
 // heavy small owners (order key at or above that of wave_item_work: the first acc[7] of the
-// sorted small owners).
+// sorted small owners), acc[9] block bitmap items (acc[8] holds the scans' totals), acc[10]
+// the most bitmap words of one (atomic max).
 __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *__restrict__ nblk,
                                                       uint32_t *__restrict__ nsmall, unsigned long long *acc) {
-    uint64_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t words_max = 0;
     const uint32_t heavy_key = cost_key((uint64_t)q.wave_item_work);
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < q.n; x += (int64_t)gridDim.x * blockDim.x) {
         const OwnerPlan o = owner_plan(q, (int32_t)x);
@@ -1433,27 +1570,33 @@ __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *_
                 v[1 + o.bucket] += (uint64_t)o.chunks;
                 v[5] += o.kind == kItemSearch ? (uint64_t)o.chunks : 0;
                 v[6] += (o.kind == kItemBitmap || o.kind == kItemSorted) ? (uint64_t)o.chunks : 0;
+                v[8] += o.kind == kItemBitmap ? (uint64_t)o.chunks : 0;
+                if (o.kind == kItemBitmap) words_max = max(words_max, o.words);
             }
         }
         nblk[x] = nb;
         nsmall[x] = ns;
     }
-    __shared__ uint64_t part[4][8];
+    words_max = rtgd::wave_max_u32(words_max);
+    if (__lane_id() == 0 && words_max) atomicMax(&acc[10], (unsigned long long)words_max);
+    __shared__ uint64_t part[4][9];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 9; ++k) {
         const uint64_t t = rtgd::wave_sum_u64(v[k]);
         if (__lane_id() == 0) part[threadIdx.x >> 6][k] = t;
     }
     __syncthreads();
-    if (threadIdx.x < 8) {
+    if (threadIdx.x < 9) {
         uint64_t t = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w][threadIdx.x];
-        if (t) atomicAdd(&acc[threadIdx.x], (unsigned long long)t);
+        if (t) atomicAdd(&acc[threadIdx.x < 8 ? threadIdx.x : 9], (unsigned long long)t);
     }
 }
 
 // Writes each owner's block items (partners split evenly into o.chunks runs) and small-owner
-// entry, with their order keys (position << 16 | heaviness key).
+// entry, with their order keys: a small owner's (position << 16 | heaviness key), a block item's
+// (position << 17 | other << 16 | heaviness key), other = 1 for the items that are not block
+// bitmaps on a degree-ordered copy, so that those bitmaps come first (their own kernel's queue).
 __global__ __launch_bounds__(256) void queue_emit(QueueParams q, const uint32_t *__restrict__ blk_off,
                                                   const uint32_t *__restrict__ small_off, TcItem *__restrict__ items,
                                                   uint64_t *__restrict__ ikeys, int32_t *__restrict__ small,
@@ -1475,17 +1618,18 @@ __global__ __launch_bounds__(256) void queue_emit(QueueParams q, const uint32_t 
             const int32_t b = base + (int32_t)((int64_t)o.c * k / o.chunks);
             const int32_t e = base + (int32_t)((int64_t)o.c * (k + 1) / o.chunks);
             items[idx] = TcItem{(int32_t)o.kind, (int32_t)x, b, e};
-            ikeys[idx] = ((uint64_t)idx << 16) | cost_key(o.w / (uint64_t)o.chunks * per + (uint64_t)o.d);
+            const uint64_t other = q.bitmap16 && o.kind != kItemBitmap ? 1ull : 0ull;
+            ikeys[idx] = ((uint64_t)idx << 17) | (other << 16) | cost_key(o.w / (uint64_t)o.chunks * per + (uint64_t)o.d);
         }
     }
 }
 
-// Heaviest first: dst[i] = src[key_i >> 16] over the keys sorted on their low 16 bits (stable,
+// Heaviest first: dst[i] = src[key_i >> 17] over the keys sorted on their low 17 bits (stable,
 // so equal heaviness keeps owner order).
 __global__ void queue_gather_items(const uint64_t *__restrict__ keys, int64_t n, const TcItem *__restrict__ src,
                                    TcItem *__restrict__ dst) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[keys[i] >> 16];
+        dst[i] = src[keys[i] >> 17];
 }
 
 __global__ void queue_gather_small(const uint64_t *__restrict__ keys, int64_t n, const int32_t *__restrict__ src,
@@ -1605,7 +1749,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         RTG_TRY(scan_exclusive_u32_w(ctx, nblk, blk_off, n, totals, scan1));
         RTG_TRY(scan_exclusive_u32_w(ctx, nsmall, small_off, n, totals + 1, scan1));
     }
-    unsigned long long h[10] = {};
+    unsigned long long h[11] = {};
     HIP_TRY(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     const int64_t nb = (int64_t)(uint32_t)(h[8] & 0xFFFFFFFFull), ns = (int64_t)(uint32_t)(h[8] >> 32);
@@ -1617,6 +1761,8 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     p->all_strategies = h[6] > 0;
     p->n_block_items = nb;
     p->n_small = ns;
+    p->n_bitmap_items = p->renumbered ? (int64_t)h[9] : 0;
+    p->bitmap_table = (int32_t)std::min<uint64_t>(h[10], (uint64_t)ts.max_table);  // the bitmap kernel's table
 
     // pass 2: items, small owners, order, wave groups
     RTG_TRY(dev_alloc(&p->d_items, (size_t)std::max<int64_t>(nb + ns, 1)));
@@ -1648,7 +1794,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     if (nb > 0) {
         uint64_t *kk[2] = {k0, k1};
         int which = 0, passes = 0;
-        RTG_TRY(radix_sort_u64(ctx, kk, nb, 16, &which, &passes, sort2));
+        RTG_TRY(radix_sort_u64(ctx, kk, nb, 17, &which, &passes, sort2));
         queue_gather_items<<<grid_cap(ctx, nb, 256, 8), 256, 0, st>>>(kk[which], nb, items_tmp, p->d_items);
         HIP_TRY(hipGetLastError());
     }
@@ -1690,19 +1836,50 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     p->wave_owner_max = ts.wave_owner_max;
     p->fit_block = ts.fit_block;
     p->kernel = (void *)k;
+    // items for the general kernel: after the block bitmaps, and before the wave items when those
+    // have their own kernel
+    const int64_t n_general = p->n_items - p->n_bitmap_items - (p->renumbered && tune.strat[0] == kStratHash ? (int64_t)ng : 0);
+    // occupancy-sized persistent grid of kernel kk for `items` queue entries
+    const auto grid_of_kernel = [&](BatchKernel kk, int64_t items, int32_t *grid, size_t kl = 0) -> int {
+        int pc = 0;
+        if (!kl) kl = lds;
+        HIP_TRY(hipFuncSetAttribute((const void *)kk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kl));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kk, threads, kl));
+        *grid = (int32_t)std::min<int64_t>((int64_t)std::max(pc, 1) * ctx->cu_count, std::max<int64_t>(items, 1));
+        return RTG_OK;
+    };
+    // degree-ordered plans whose small owners are hashed run the wave items in their own kernel
+    p->n_wave_items = p->renumbered && tune.strat[0] == kStratHash ? (int64_t)ng : 0;
+    if (p->n_wave_items > 0) {
+        BatchKernel kw = wave_kernel(threads, tune.unroll, false);
+        p->kernel_wv = (void *)kw;
+        RTG_TRY(grid_of_kernel(kw, p->n_wave_items, &p->grid_wv));
+    }
+    if (p->n_bitmap_items > 0) {
+        BatchKernel kb = bitmap_kernel(threads, tune.unroll, false);
+        p->kernel_bm = (void *)kb;
+        RTG_TRY(grid_of_kernel(kb, p->n_bitmap_items, &p->grid_bm, sizeof(uint32_t) * (size_t)p->bitmap_table));
+    }
     if (p->opts.per_edge) {
         BatchKernel ke = batched_kernel(threads, tune.unroll, true, p->all_strategies, p->renumbered);
-        int per_cu_e = 0;
-        HIP_TRY(hipFuncSetAttribute((const void *)ke, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_e, ke, threads, lds));
         p->kernel_pe = (void *)ke;
-        p->grid_pe = (int32_t)std::min<int64_t>((int64_t)std::max(per_cu_e, 1) * ctx->cu_count, std::max<int64_t>(p->n_items, 1));
+        RTG_TRY(grid_of_kernel(ke, n_general, &p->grid_pe));
+        if (p->n_bitmap_items > 0) {
+            BatchKernel kbe = bitmap_kernel(threads, tune.unroll, true);
+            p->kernel_pe_bm = (void *)kbe;
+            RTG_TRY(grid_of_kernel(kbe, p->n_bitmap_items, &p->grid_pe_bm, sizeof(uint32_t) * (size_t)p->bitmap_table));
+        }
+        if (p->n_wave_items > 0) {
+            BatchKernel kwe = wave_kernel(threads, tune.unroll, true);
+            p->kernel_pe_wv = (void *)kwe;
+            RTG_TRY(grid_of_kernel(kwe, p->n_wave_items, &p->grid_pe_wv));
+        }
     }
     p->unroll = tune.unroll;
     p->long_min = tune.long_min;
     p->grab = tune.grab;
     p->tune = tune_string(tune);
-    p->grid = (int32_t)std::min<int64_t>((int64_t)per_cu * ctx->cu_count, std::max<int64_t>(p->n_items, 1));
+    p->grid = (int32_t)std::min<int64_t>((int64_t)per_cu * ctx->cu_count, std::max<int64_t>(n_general, 1));
     return RTG_OK;
 }
 
@@ -1738,7 +1915,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 11;
+constexpr int kTuneRevision = 12;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
@@ -1915,6 +2092,32 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p) {
     return rc;
 }
 
+// The batched queue, in up to three kernels one after another: the block bitmap items of a
+// degree-ordered plan (items [0, n_bitmap_items)) by kb, the other block items (and the wave
+// items, unless they have their own kernel) by the general kernel kg, and the wave items (the
+// queue's last n_wave_items) by kw. eout: per-edge counts.
+void launch_items(rtg_tc_plan *p, hipStream_t st, BatchKernel kb, int32_t grid_b, BatchKernel kg, int32_t grid_g,
+                  BatchKernel kw, int32_t grid_w, uint32_t *eout) {
+    BatchArgs a{p->v_off, p->v_adj, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
+                p->n_bitmap_items, p->d_count, p->bitmap_table, 0,
+                p->long_min, p->grab, eout ? p->d_wedge : nullptr, eout, p->v_adj16};
+    if (p->n_bitmap_items > 0) launch_batched(kb, grid_b, p->block_threads, sizeof(uint32_t) * (size_t)p->bitmap_table, st, a);
+    const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
+    a.max_table = p->max_table;
+    a.wave_table = p->max_table / (p->block_threads / 64);
+    const int64_t general_end = p->n_items - p->n_wave_items;
+    if (general_end > p->n_bitmap_items) {
+        a.items = p->d_items + p->n_bitmap_items;
+        a.n_items = general_end - p->n_bitmap_items;
+        launch_batched(kg, grid_g, p->block_threads, lds, st, a);
+    }
+    if (p->n_wave_items > 0) {
+        a.items = p->d_items + general_end;
+        a.n_items = p->n_wave_items;
+        launch_batched(kw, grid_w, p->block_threads, lds, st, a);
+    }
+}
+
 int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
     const rtg_graph *g = p->g;
     hipStream_t st = ctx->stream;
@@ -1934,11 +2137,8 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
             break;
         case RTG_TC_BATCHED: {
             if (p->n_items == 0) break;
-            BatchArgs a{p->v_off, p->v_adj, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
-                        p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
-                        p->long_min, p->grab, nullptr, nullptr, p->v_adj16};
-            const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
-            launch_batched((BatchKernel)p->kernel, p->grid, p->block_threads, lds, st, a);
+            launch_items(p, st, (BatchKernel)p->kernel_bm, p->grid_bm, (BatchKernel)p->kernel, p->grid,
+                         (BatchKernel)p->kernel_wv, p->grid_wv, nullptr);
             break;
         }
         default:
@@ -1958,11 +2158,8 @@ int launch_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *eout) {
     reset_counters<<<1, 64, 0, st>>>(p->d_count);
     HIP_TRY(hipGetLastError());
     if (p->n_items == 0) return RTG_OK;
-    BatchArgs a{p->v_off, p->v_adj, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
-                p->n_items, p->d_count, p->max_table, p->max_table / (p->block_threads / 64),
-                p->long_min, p->grab, p->d_wedge, eout, p->v_adj16};
-    const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
-    launch_batched((BatchKernel)p->kernel_pe, p->grid_pe, p->block_threads, lds, st, a);
+    launch_items(p, st, (BatchKernel)p->kernel_pe_bm, p->grid_pe_bm, (BatchKernel)p->kernel_pe, p->grid_pe,
+                 (BatchKernel)p->kernel_pe_wv, p->grid_pe_wv, eout);
     HIP_TRY(hipGetLastError());
     return RTG_OK;
 }
@@ -2139,7 +2336,9 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
              kStratName[p->strat[3]] + ",all_strategies=" + (p->all_strategies ? "1" : "0") +
              ",plan_graph_ms=" + std::to_string(p->t_plan_graph_ms) + ",plan_tune_ms=" + std::to_string(p->t_plan_tune_ms) +
              ",plan_items_ms=" + std::to_string(p->t_plan_items_ms) + ",renumbered=" + (p->renumbered ? "1" : "0") +
-             ",renumber_ms=" + std::to_string(p->t_renum_ms) + ",streamed=" + std::to_string(p->streamed) + ",streamed16=" + std::to_string(p->streamed16);
+             ",renumber_ms=" + std::to_string(p->t_renum_ms) + ",streamed=" + std::to_string(p->streamed) + ",streamed16=" + std::to_string(p->streamed16) +
+             ",bitmap_items=" + std::to_string(p->n_bitmap_items) + ",grid_bitmap=" + std::to_string(p->grid_bm) +
+             ",wave_items=" + std::to_string(p->n_wave_items) + ",grid_wave=" + std::to_string(p->grid_wv);
     std::snprintf(buf, len, "%s", d.c_str());
     return RTG_OK;
 }

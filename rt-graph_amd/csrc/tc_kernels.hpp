@@ -52,6 +52,16 @@ struct rtg_tc_plan {
     void *kernel = nullptr;          // batched: selected kernel instantiation
     void *kernel_pe = nullptr;       // batched, per-edge plans: the per-edge instantiation
     int32_t grid_pe = 0;
+    // degree-ordered plans: the block bitmap items (the first n_bitmap_items of the queue) run in
+    // their own kernel, counting and per-edge, before the general kernel takes the rest
+    int64_t n_bitmap_items = 0;
+    void *kernel_bm = nullptr, *kernel_pe_bm = nullptr;
+    int32_t grid_bm = 0, grid_pe_bm = 0;
+    int32_t bitmap_table = 0;        // the bitmap kernel's table words: the largest of its bitmaps
+    // and, when their small owners are hashed, the wave items (the queue's last n_wave_items)
+    int64_t n_wave_items = 0;
+    void *kernel_wv = nullptr, *kernel_pe_wv = nullptr;
+    int32_t grid_wv = 0, grid_pe_wv = 0;
     // byte models: per-edge (no reuse), merge-only, and the owner-staged launch's reads
     uint64_t alg_bytes = 0, merge_bytes = 0, staged_bytes = 0;
     // captured launch sequence
