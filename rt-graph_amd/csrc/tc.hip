@@ -995,7 +995,8 @@ BatchKernel bitmap_kernel(int threads, int unroll, bool per_edge) {
     return per_edge ? bitmap_kernel_t<true>(threads, unroll) : bitmap_kernel_t<false>(threads, unroll);
 }
 
-// The wave-items-only kernel of a degree-ordered plan, counting or per-edge.
+// The wave-items-only kernel of a degree-ordered plan whose small owners are hashed, counting or
+// per-edge.
 template <bool kPerEdge>
 BatchKernel wave_kernel_t(int threads, int unroll) {
     if (threads == 256 && unroll == 2) return tc_wave_items<256, 2, kPerEdge>;
@@ -1498,6 +1499,7 @@ struct QueueParams {
     int32_t wave_owner_max, fit_block, max_table, wave_table;
     int32_t only_bucket;             // -1: every owner; else only this bucket's (autotuning)
     int32_t bitmap16;                // degree-ordered copy: bitmaps run from 0 (kBitmap16)
+    int32_t small_tables;            // small owners get bitmap or sorted-list wave tables
     int64_t item_work;
     int64_t wave_item_work;          // a small owner is heavy when its work reaches this
     int32_t strat[kBuckets];
@@ -1562,7 +1564,7 @@ __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *_
                 ns = 1;
                 v[0] += 4ull + 8ull + 4ull * (uint64_t)o.d;  // small-list entry, partner offsets, owner list
                 v[1] += 1;
-                v[6] += q.strat[0] == kStratBitmap || q.strat[0] == kStratMerge ? 1 : 0;
+                v[6] += q.small_tables ? 1 : 0;
                 v[7] += cost_key(o.w + (uint64_t)o.d) <= heavy_key ? 1 : 0;
             } else {
                 nb = (uint32_t)o.chunks;
@@ -1720,8 +1722,15 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     if (threads != 256 && threads != 512) return set_error(RTG_EINVAL, "batched TC: threads must be 256 or 512");
     const int waves = threads / 64;
     const TableSizes ts = table_sizes(tune, g->max_row);
+    // Small owners get hash tables: by their strategy, or on a degree-ordered copy whose wave
+    // share cannot hold a bitmap from 0 (bitmap16_words is at least 2,049), where a bitmap
+    // would fall back to the hash table anyway.
+    const bool small_hashed = tune.strat[0] == kStratHash ||
+                              (p->renumbered && tune.strat[0] == kStratBitmap && (uint32_t)ts.wave_table < bitmap16_words(0));
     QueueParams q{p->v_off, p->v_adj, p->d_wofs, p->d_work, n, ts.wave_owner_max, ts.fit_block, ts.max_table,
-                  ts.wave_table, only_bucket, p->renumbered ? 1 : 0, tune.item_work, tune.wave_item_work, {}};
+                  ts.wave_table, only_bucket, p->renumbered ? 1 : 0,
+                  !small_hashed && (tune.strat[0] == kStratBitmap || tune.strat[0] == kStratMerge) ? 1 : 0,
+                  tune.item_work, tune.wave_item_work, {}};
     for (int b = 0; b < kBuckets; ++b) q.strat[b] = tune.strat[b];
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->d_small) (void)hipFree(p->d_small);
@@ -1811,8 +1820,10 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         RTG_TRY(scan_exclusive_u32_w(ctx, flag, gid, ns, ngroups, scan2));
         queue_group_starts<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(flag, gid, ns, starts);
         HIP_TRY(hipGetLastError());
-        const int32_t mode = tune.strat[0] == kStratBitmap ? (int32_t)kBitmap : tune.strat[0] == kStratMerge ? (int32_t)kSorted
-                                                                                                         : (int32_t)kCuckoo;
+        const int32_t mode = small_hashed ? (int32_t)kCuckoo
+                             : tune.strat[0] == kStratBitmap ? (int32_t)kBitmap
+                             : tune.strat[0] == kStratMerge  ? (int32_t)kSorted
+                                                             : (int32_t)kCuckoo;
         queue_wave_items<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(starts, ngroups, ns, mode, p->d_items + nb);
         HIP_TRY(hipGetLastError());
     }
@@ -1838,7 +1849,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     p->kernel = (void *)k;
     // items for the general kernel: after the block bitmaps, and before the wave items when those
     // have their own kernel
-    const int64_t n_general = p->n_items - p->n_bitmap_items - (p->renumbered && tune.strat[0] == kStratHash ? (int64_t)ng : 0);
+    const int64_t n_general = p->n_items - p->n_bitmap_items - (p->renumbered && small_hashed ? (int64_t)ng : 0);
     // occupancy-sized persistent grid of kernel kk for `items` queue entries
     const auto grid_of_kernel = [&](BatchKernel kk, int64_t items, int32_t *grid, size_t kl = 0) -> int {
         int pc = 0;
@@ -1849,11 +1860,14 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         return RTG_OK;
     };
     // degree-ordered plans whose small owners are hashed run the wave items in their own kernel
-    p->n_wave_items = p->renumbered && tune.strat[0] == kStratHash ? (int64_t)ng : 0;
+    p->n_wave_items = p->renumbered && small_hashed ? (int64_t)ng : 0;
+    // its wave tables: both cuckoo halves of the largest small owner (not the block table's share)
+    p->wave_table_wv = std::min<int32_t>(ts.wave_table, 2 * (int32_t)cuckoo_half(ts.wave_owner_max));
+    const size_t lds_wv = sizeof(uint32_t) * (size_t)p->wave_table_wv * (size_t)waves;
     if (p->n_wave_items > 0) {
         BatchKernel kw = wave_kernel(threads, tune.unroll, false);
         p->kernel_wv = (void *)kw;
-        RTG_TRY(grid_of_kernel(kw, p->n_wave_items, &p->grid_wv));
+        RTG_TRY(grid_of_kernel(kw, p->n_wave_items, &p->grid_wv, lds_wv));
     }
     if (p->n_bitmap_items > 0) {
         BatchKernel kb = bitmap_kernel(threads, tune.unroll, false);
@@ -1872,7 +1886,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         if (p->n_wave_items > 0) {
             BatchKernel kwe = wave_kernel(threads, tune.unroll, true);
             p->kernel_pe_wv = (void *)kwe;
-            RTG_TRY(grid_of_kernel(kwe, p->n_wave_items, &p->grid_pe_wv));
+            RTG_TRY(grid_of_kernel(kwe, p->n_wave_items, &p->grid_pe_wv, lds_wv));
         }
     }
     p->unroll = tune.unroll;
@@ -1902,6 +1916,11 @@ const char *kCandidates[] = {
     "threads=256,unroll=4,item=2097152,wave_item=2048,wave_max=128,long=128,grab=32",
     "threads=256,unroll=4,item=2097152,wave_item=1024,wave_max=64,long=128,grab=32",
     "threads=256,unroll=8,item=2097152,wave_item=2048,wave_max=64,long=128,grab=32",
+    // small owners up to out-degree 256 (degree-ordered copies: the owners above it are block
+    // bitmaps, and the wave-items kernel's smaller tables fit more blocks per CU)
+    "threads=256,unroll=8,item=2097152,wave_item=2048,wave_max=256,long=128,grab=32",
+    "threads=256,unroll=8,item=2097152,wave_item=1024,wave_max=256,long=128,grab=32",
+    "threads=256,unroll=8,item=2097152,wave_item=2048,wave_max=256,long=128,grab=64",
 };
 
 // Strategies the autotuner times per bucket (the first is the default).
@@ -1915,7 +1934,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 12;
+constexpr int kTuneRevision = 14;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
@@ -2114,7 +2133,9 @@ void launch_items(rtg_tc_plan *p, hipStream_t st, BatchKernel kb, int32_t grid_b
     if (p->n_wave_items > 0) {
         a.items = p->d_items + general_end;
         a.n_items = p->n_wave_items;
-        launch_batched(kw, grid_w, p->block_threads, lds, st, a);
+        a.wave_table = p->wave_table_wv;
+        launch_batched(kw, grid_w, p->block_threads, sizeof(uint32_t) * (size_t)p->wave_table_wv * (p->block_threads / 64),
+                       st, a);
     }
 }
 
