@@ -488,8 +488,12 @@ def test_autotuned_bucket_strategies(ctx, tmp_path):
     assert st["nbuckets"] == 4 and len(st["strategy"]) == 4
 
 
+# On the copy a count runs up to three kernels: block bitmaps, small owners' wave items (hashed,
+# also under a bitmap strategy that cannot fit a wave's share) and the general kernel for the
+# rest; "bhbb" and "bmbs" put items in all three.
 RENUMBER_TUNES = ["strat=hhhs", "strat=bbbs", "strat=hbbs", "strat=mmms", "strat=ssss", "strat=bhbb",
-                  "strat=hbbs,unroll=8", "strat=bbbs,threads=512,unroll=2"]
+                  "strat=hbbs,unroll=8", "strat=bbbs,threads=512,unroll=2", "strat=hbbs,wave_max=256,unroll=8,grab=32",
+                  "strat=bhbb,threads=512,unroll=8", "strat=bmbs,wave_max=16"]
 
 
 @pytest.mark.parametrize("tune", RENUMBER_TUNES)
@@ -624,7 +628,8 @@ def test_span_cuts_streamed(ctx, monkeypatch):
     g.close()
 
 
-@pytest.mark.parametrize("tune", ["strat=hbbs", "strat=hhhs", "strat=bmbs", "strat=hbbs,unroll=4,long=1"])
+@pytest.mark.parametrize("tune", ["strat=hbbs", "strat=hhhs", "strat=bmbs", "strat=hbbs,unroll=4,long=1",
+                                  "strat=bhbb,wave_max=256,unroll=8,grab=32"])
 def test_renumbered_per_edge(ctx, monkeypatch, tune):
     """Per-edge plans on the degree-ordered copy: every span's count is written at the graph's
     own edge through the copy's edge map; every fixture graph against the oracle's per-edge
