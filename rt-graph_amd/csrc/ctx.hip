@@ -76,6 +76,9 @@ RTG_API int rtg_create(int device, rtg_ctx **out) {
         if (hipGetDeviceProperties(&ctx->prop, device) != hipSuccess) { st = rtg_fail(RTG_EHIP, "hipGetDeviceProperties failed"); break; }
         ctx->cu_count = ctx->prop.multiProcessorCount;
         if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { st = rtg_fail(RTG_EHIP, "hipStreamCreate failed"); break; }
+        if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) { st = rtg_fail(RTG_EHIP, "hipStreamCreate failed"); break; }
+        if (hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess) { st = rtg_fail(RTG_EHIP, "hipEventCreate failed"); break; }
         for (auto &e : ctx->ev)
             if (hipEventCreate(&e) != hipSuccess) { st = rtg_fail(RTG_EHIP, "hipEventCreate failed"); break; }
         if (st != RTG_OK) break;
@@ -104,6 +107,10 @@ RTG_API int rtg_destroy(rtg_ctx *ctx) {
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RTG_OK;

@@ -38,6 +38,10 @@ struct rtg_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     hipEvent_t ev[8] = {};
+    // A second stream (default priority, like the first) for kernels that run beside the main
+    // stream's, forked and joined with these events (also under stream capture).
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     // Long-pair strategy the set-intersection autotuner chose, per argument set (most recent
     // last). Every strategy returns the same counts; a stale entry only costs speed.
     struct SiTuned {

@@ -1557,10 +1557,10 @@ __device__ __forceinline__ OwnerPlan owner_plan(const QueueParams &q, int32_t x)
 // bitmap or sorted-list table (including small owners whose wave mode is one of them), acc[7]
 // heavy small owners (order key at or above that of wave_item_work: the first acc[7] of the
 // sorted small owners), acc[9] block bitmap items (acc[8] holds the scans' totals), acc[10]
-// the most bitmap words of one (atomic max).
+// the most bitmap words of one (atomic max), acc[11] the small owners' streamed elements.
 __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *__restrict__ nblk,
                                                       uint32_t *__restrict__ nsmall, unsigned long long *acc) {
-    uint64_t v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t words_max = 0;
     const uint32_t heavy_key = cost_key((uint64_t)q.wave_item_work);
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < q.n; x += (int64_t)gridDim.x * blockDim.x) {
@@ -1574,6 +1574,7 @@ __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *_
                 v[1] += 1;
                 v[6] += q.small_tables ? 1 : 0;
                 v[7] += cost_key(o.w + (uint64_t)o.d) <= heavy_key ? 1 : 0;
+                v[9] += o.w;
             } else {
                 nb = (uint32_t)o.chunks;
                 v[0] += (uint64_t)o.chunks * (16ull + 4ull * (uint64_t)o.d);  // queue entry, owner list per item
@@ -1589,17 +1590,19 @@ __global__ __launch_bounds__(256) void queue_classify(QueueParams q, uint32_t *_
     }
     words_max = rtgd::wave_max_u32(words_max);
     if (__lane_id() == 0 && words_max) atomicMax(&acc[10], (unsigned long long)words_max);
-    __shared__ uint64_t part[4][9];
+    __shared__ uint64_t part[4][10];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < 10; ++k) {
         const uint64_t t = rtgd::wave_sum_u64(v[k]);
         if (__lane_id() == 0) part[threadIdx.x >> 6][k] = t;
     }
     __syncthreads();
-    if (threadIdx.x < 9) {
+    if (threadIdx.x < 10) {
         uint64_t t = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w][threadIdx.x];
-        if (t) atomicAdd(&acc[threadIdx.x < 8 ? threadIdx.x : 9], (unsigned long long)t);
+        // v[0..7] -> acc[0..7], v[8] -> acc[9], v[9] -> acc[11] (acc[8] scan totals, acc[10] a max)
+        const int slot = threadIdx.x < 8 ? (int)threadIdx.x : threadIdx.x == 8 ? 9 : 11;
+        if (t) atomicAdd(&acc[slot], (unsigned long long)t);
     }
 }
 
@@ -1766,7 +1769,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         RTG_TRY(scan_exclusive_u32_w(ctx, nblk, blk_off, n, totals, scan1));
         RTG_TRY(scan_exclusive_u32_w(ctx, nsmall, small_off, n, totals + 1, scan1));
     }
-    unsigned long long h[11] = {};
+    unsigned long long h[12] = {};
     HIP_TRY(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     const int64_t nb = (int64_t)(uint32_t)(h[8] & 0xFFFFFFFFull), ns = (int64_t)(uint32_t)(h[8] >> 32);
@@ -1780,6 +1783,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     p->n_small = ns;
     p->n_bitmap_items = p->renumbered ? (int64_t)h[9] : 0;
     p->bitmap_table = (int32_t)std::min<uint64_t>(h[10], (uint64_t)ts.max_table);  // the bitmap kernel's table
+    p->small_work = (int64_t)h[11];
 
     // pass 2: items, small owners, order, wave groups
     RTG_TRY(dev_alloc(&p->d_items, (size_t)std::max<int64_t>(nb + ns, 1)));
@@ -1858,29 +1862,37 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     // items for the general kernel: after the block bitmaps, and before the wave items when those
     // have their own kernel
     const int64_t n_general = p->n_items - p->n_bitmap_items - (p->renumbered && small_hashed ? (int64_t)ng : 0);
-    // occupancy-sized persistent grid of kernel kk for `items` queue entries
-    const auto grid_of_kernel = [&](BatchKernel kk, int64_t items, int32_t *grid, size_t kl = 0) -> int {
+    // occupancy-sized persistent grid of kernel kk for `items` queue entries; `share` (beside the
+    // other kernel): -1 one block per CU fewer than fit, +1 one block per CU
+    const auto grid_of_kernel = [&](BatchKernel kk, int64_t items, int32_t *grid, size_t kl = 0, int share = 0) -> int {
         int pc = 0;
         if (!kl) kl = lds;
         HIP_TRY(hipFuncSetAttribute((const void *)kk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kl));
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kk, threads, kl));
+        if (share < 0) pc -= 1;
+        if (share > 0) pc = 1;
         *grid = (int32_t)std::min<int64_t>((int64_t)std::max(pc, 1) * ctx->cu_count, std::max<int64_t>(items, 1));
         return RTG_OK;
     };
     // degree-ordered plans whose small owners are hashed run the wave items in their own kernel
     p->n_wave_items = p->renumbered && small_hashed ? (int64_t)ng : 0;
+    // The wave items run beside the block bitmaps (one block per CU) when they hold at most 1/16 of
+    // the streamed elements (1.6% on R-MAT scale 21; the configuration model's 16% would leave
+    // the bitmap kernel waiting for them), else after them.
+    p->side = p->n_bitmap_items > 0 && p->n_wave_items > 0 && 16 * p->small_work <= p->streamed;
     // its wave tables: both cuckoo halves of the largest small owner (not the block table's share)
     p->wave_table_wv = std::min<int32_t>(ts.wave_table, 2 * (int32_t)cuckoo_half(ts.wave_owner_max));
     const size_t lds_wv = sizeof(uint32_t) * (size_t)p->wave_table_wv * (size_t)waves;
     if (p->n_wave_items > 0) {
         BatchKernel kw = wave_kernel(threads, tune.unroll, false);
         p->kernel_wv = (void *)kw;
-        RTG_TRY(grid_of_kernel(kw, p->n_wave_items, &p->grid_wv, lds_wv));
+        RTG_TRY(grid_of_kernel(kw, p->n_wave_items, &p->grid_wv, lds_wv, p->side ? 1 : 0));
     }
     if (p->n_bitmap_items > 0) {
         BatchKernel kb = bitmap_kernel(threads, tune.unroll, false);
         p->kernel_bm = (void *)kb;
-        RTG_TRY(grid_of_kernel(kb, p->n_bitmap_items, &p->grid_bm, sizeof(uint32_t) * (size_t)p->bitmap_table));
+        RTG_TRY(grid_of_kernel(kb, p->n_bitmap_items, &p->grid_bm, sizeof(uint32_t) * (size_t)p->bitmap_table,
+                               p->side ? -1 : 0));
     }
     if (p->opts.per_edge) {
         BatchKernel ke = batched_kernel(threads, tune.unroll, true, p->all_strategies, p->renumbered);
@@ -1889,12 +1901,13 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         if (p->n_bitmap_items > 0) {
             BatchKernel kbe = bitmap_kernel(threads, tune.unroll, true);
             p->kernel_pe_bm = (void *)kbe;
-            RTG_TRY(grid_of_kernel(kbe, p->n_bitmap_items, &p->grid_pe_bm, sizeof(uint32_t) * (size_t)p->bitmap_table));
+            RTG_TRY(grid_of_kernel(kbe, p->n_bitmap_items, &p->grid_pe_bm, sizeof(uint32_t) * (size_t)p->bitmap_table,
+                                   p->side ? -1 : 0));
         }
         if (p->n_wave_items > 0) {
             BatchKernel kwe = wave_kernel(threads, tune.unroll, true);
             p->kernel_pe_wv = (void *)kwe;
-            RTG_TRY(grid_of_kernel(kwe, p->n_wave_items, &p->grid_pe_wv, lds_wv));
+            RTG_TRY(grid_of_kernel(kwe, p->n_wave_items, &p->grid_pe_wv, lds_wv, p->side ? 1 : 0));
         }
     }
     p->unroll = tune.unroll;
@@ -1942,7 +1955,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 15;
+constexpr int kTuneRevision = 17;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
@@ -2119,15 +2132,33 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p) {
     return rc;
 }
 
-// The batched queue, in up to three kernels one after another: the block bitmap items of a
-// degree-ordered plan (items [0, n_bitmap_items)) by kb, the other block items (and the wave
-// items, unless they have their own kernel) by the general kernel kg, and the wave items (the
-// queue's last n_wave_items) by kw. eout: per-edge counts.
-void launch_items(rtg_tc_plan *p, hipStream_t st, BatchKernel kb, int32_t grid_b, BatchKernel kg, int32_t grid_g,
-                  BatchKernel kw, int32_t grid_w, uint32_t *eout) {
+// The batched queue, in up to three kernels: the block bitmap items of a degree-ordered plan
+// (items [0, n_bitmap_items)) by kb, the other block items (and the wave items, unless they have
+// their own kernel) by the general kernel kg, and the wave items (the queue's last n_wave_items)
+// by kw. With few wave items (p->side), the wave kernel runs on the context's side stream beside
+// the bitmap kernel (its grid and the bitmap kernel's are sized to fit the device together): most
+// wave items are short, and the long tail of the few heavy ones no longer follows the bitmap
+// kernel. eout: per-edge counts.
+int launch_items(rtg_tc_plan *p, hipStream_t st, BatchKernel kb, int32_t grid_b, BatchKernel kg, int32_t grid_g,
+                 BatchKernel kw, int32_t grid_w, uint32_t *eout) {
+    rtg_ctx *ctx = p->ctx;
+    const bool beside = p->side;
     BatchArgs a{p->v_off, p->v_adj, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
                 p->n_bitmap_items, p->d_count, p->bitmap_table, 0,
                 p->long_min, p->grab, eout ? p->d_wedge : nullptr, eout, p->v_adj16};
+    BatchArgs aw = a;  // the wave items'
+    aw.items = p->d_items + (p->n_items - p->n_wave_items);
+    aw.n_items = p->n_wave_items;
+    aw.max_table = p->max_table;
+    aw.wave_table = p->wave_table_wv;
+    const size_t lds_w = sizeof(uint32_t) * (size_t)p->wave_table_wv * (p->block_threads / 64);
+    if (beside) {
+        HIP_TRY(hipEventRecord(ctx->fork, st));
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+        launch_batched(kw, grid_w, p->block_threads, lds_w, ctx->side, aw);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ctx->join, ctx->side));
+    }
     if (p->n_bitmap_items > 0) launch_batched(kb, grid_b, p->block_threads, sizeof(uint32_t) * (size_t)p->bitmap_table, st, a);
     const size_t lds = sizeof(uint32_t) * (size_t)p->max_table;
     a.max_table = p->max_table;
@@ -2138,13 +2169,10 @@ void launch_items(rtg_tc_plan *p, hipStream_t st, BatchKernel kb, int32_t grid_b
         a.n_items = general_end - p->n_bitmap_items;
         launch_batched(kg, grid_g, p->block_threads, lds, st, a);
     }
-    if (p->n_wave_items > 0) {
-        a.items = p->d_items + general_end;
-        a.n_items = p->n_wave_items;
-        a.wave_table = p->wave_table_wv;
-        launch_batched(kw, grid_w, p->block_threads, sizeof(uint32_t) * (size_t)p->wave_table_wv * (p->block_threads / 64),
-                       st, a);
-    }
+    if (p->n_wave_items > 0 && !beside) launch_batched(kw, grid_w, p->block_threads, lds_w, st, aw);
+    HIP_TRY(hipGetLastError());
+    if (beside) HIP_TRY(hipStreamWaitEvent(st, ctx->join, 0));
+    return RTG_OK;
 }
 
 int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
@@ -2166,8 +2194,8 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
             break;
         case RTG_TC_BATCHED: {
             if (p->n_items == 0) break;
-            launch_items(p, st, (BatchKernel)p->kernel_bm, p->grid_bm, (BatchKernel)p->kernel, p->grid,
-                         (BatchKernel)p->kernel_wv, p->grid_wv, nullptr);
+            RTG_TRY(launch_items(p, st, (BatchKernel)p->kernel_bm, p->grid_bm, (BatchKernel)p->kernel, p->grid,
+                                 (BatchKernel)p->kernel_wv, p->grid_wv, nullptr));
             break;
         }
         default:
@@ -2187,8 +2215,8 @@ int launch_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *eout) {
     reset_counters<<<1, 64, 0, st>>>(p->d_count);
     HIP_TRY(hipGetLastError());
     if (p->n_items == 0) return RTG_OK;
-    launch_items(p, st, (BatchKernel)p->kernel_pe_bm, p->grid_pe_bm, (BatchKernel)p->kernel_pe, p->grid_pe,
-                 (BatchKernel)p->kernel_pe_wv, p->grid_pe_wv, eout);
+    RTG_TRY(launch_items(p, st, (BatchKernel)p->kernel_pe_bm, p->grid_pe_bm, (BatchKernel)p->kernel_pe, p->grid_pe,
+                         (BatchKernel)p->kernel_pe_wv, p->grid_pe_wv, eout));
     HIP_TRY(hipGetLastError());
     return RTG_OK;
 }
@@ -2367,7 +2395,8 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
              ",plan_items_ms=" + std::to_string(p->t_plan_items_ms) + ",renumbered=" + (p->renumbered ? "1" : "0") +
              ",renumber_ms=" + std::to_string(p->t_renum_ms) + ",streamed=" + std::to_string(p->streamed) + ",streamed16=" + std::to_string(p->streamed16) +
              ",bitmap_items=" + std::to_string(p->n_bitmap_items) + ",grid_bitmap=" + std::to_string(p->grid_bm) +
-             ",wave_items=" + std::to_string(p->n_wave_items) + ",grid_wave=" + std::to_string(p->grid_wv);
+             ",wave_items=" + std::to_string(p->n_wave_items) + ",grid_wave=" + std::to_string(p->grid_wv) +
+             ",small_work=" + std::to_string(p->small_work) + ",side=" + (p->side ? "1" : "0");
     std::snprintf(buf, len, "%s", d.c_str());
     return RTG_OK;
 }

@@ -63,6 +63,8 @@ struct rtg_tc_plan {
     void *kernel_wv = nullptr, *kernel_pe_wv = nullptr;
     int32_t grid_wv = 0, grid_pe_wv = 0;
     int32_t wave_table_wv = 0;       // the wave-items kernel's slots per wave
+    int64_t small_work = 0;          // streamed elements of the small owners
+    bool side = false;               // the wave kernel runs beside the bitmap kernel
     // byte models: per-edge (no reuse), merge-only, and the owner-staged launch's reads
     uint64_t alg_bytes = 0, merge_bytes = 0, staged_bytes = 0;
     // captured launch sequence
