@@ -22,7 +22,7 @@ struct rtg_graph {
 // ties by id): a triangle count does not depend on vertex ids, and in this order every
 // out-neighbour (an endpoint of higher degree) gets a small id, so an out-list holds only ids
 // below its own vertex's id in most rows. Rows are sorted. adj16 holds the low 16 bits of every
-// entry, exact for a row whose largest entry vmax[v] is below 2^16. s16 and s256 sample every
+// entry, exact for a row whose largest entry (vinfo[v].z) is below 2^16. s16 and s256 sample every
 // 16th and every 256th entry, for searches inside a row: s256 is small enough to stay in L2, and
 // the 16 entries of s16 between two s256 samples fill one aligned 64-byte line.
 struct rtg_renum {
@@ -31,7 +31,7 @@ struct rtg_renum {
     int32_t *d_off = nullptr;     // n + 1
     int32_t *d_adj = nullptr;     // m
     uint16_t *d_adj16 = nullptr;  // m
-    int32_t *d_vmax = nullptr;    // n: row v's largest entry (-1 for an empty row)
+    int32_t *d_vinfo = nullptr;   // 4 n: row v's {start, end, largest entry (-1 when empty), 0}
     int32_t *d_inv = nullptr;     // n: new id -> the graph's id
     int32_t *d_s16 = nullptr;     // 16 * ceil(m / 256) + 16: d_adj[16 i], INT32_MAX past m
     int32_t *d_s256 = nullptr;    // ceil(m / 256): d_adj[256 j]

@@ -332,16 +332,17 @@ __global__ void sample_rows(const int32_t *__restrict__ adj, int64_t m, int32_t 
     }
 }
 
-// vmax[v] = row v's largest entry (-1 when empty); counts[0] = non-empty rows whose entries all
-// lie below 2^16, counts[1] = their entries (one device atomic per block).
+// vinfo[v] = {row v's start, its end, its largest entry (-1 when empty), 0}: what planning
+// gathers for a random row, in one 16-byte load; counts[0] = non-empty rows whose entries all lie
+// below 2^16, counts[1] = their entries (one device atomic per block).
 __global__ __launch_bounds__(kT) void row_max(const int32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                              int32_t n, int32_t *__restrict__ vmax,
+                                              int32_t n, int4 *__restrict__ vinfo,
                                               unsigned long long *__restrict__ counts) {
     unsigned long long r = 0, e = 0;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         const int32_t b = off[v], d = off[v + 1] - b;
         const int32_t x = d > 0 ? adj[b + d - 1] : -1;
-        vmax[v] = x;
+        vinfo[v] = make_int4(b, b + d, x, 0);
         const bool f = d > 0 && x < 65536;
         r += f ? 1 : 0;
         e += f ? (unsigned long long)d : 0ull;
@@ -379,7 +380,7 @@ int build(rtg_ctx *ctx, const rtg_graph *g, rtg_renum *r) {
     HIP_TRY(hipMalloc(&r->d_off, sizeof(int32_t) * ((size_t)n + 1)));
     HIP_TRY(hipMalloc(&r->d_adj, sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1)));
     HIP_TRY(hipMalloc(&r->d_adj16, sizeof(uint16_t) * (size_t)std::max<int64_t>(m, 1)));
-    HIP_TRY(hipMalloc(&r->d_vmax, sizeof(int32_t) * (size_t)std::max<int32_t>(n, 1)));
+    HIP_TRY(hipMalloc(&r->d_vinfo, 4 * sizeof(int32_t) * (size_t)std::max<int32_t>(n, 1)));
     HIP_TRY(hipMalloc(&r->d_inv, sizeof(int32_t) * (size_t)std::max<int32_t>(n, 1)));
     const int64_t n256 = (m + 255) / 256;
     HIP_TRY(hipMalloc(&r->d_s16, sizeof(int32_t) * (size_t)(16 * n256 + 16)));
@@ -451,7 +452,7 @@ int build(rtg_ctx *ctx, const rtg_graph *g, rtg_renum *r) {
     sample_rows<<<grid_of(ctx, 16 * n256 + 16), kT, 0, st>>>(r->d_adj, m, r->d_s16, r->d_s256);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemsetAsync(counts, 0, sizeof(unsigned long long) * 2, st));
-    row_max<<<grid_of(ctx, n, 4), kT, 0, st>>>(r->d_off, r->d_adj, n, r->d_vmax, counts);
+    row_max<<<grid_of(ctx, n, 4), kT, 0, st>>>(r->d_off, r->d_adj, n, reinterpret_cast<int4 *>(r->d_vinfo), counts);
     HIP_TRY(hipGetLastError());
     unsigned long long hc[2] = {0, 0};
     HIP_TRY(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, st));
@@ -468,7 +469,7 @@ namespace rtg {
 
 void renum_free(rtg_renum *r) {
     if (!r) return;
-    void *p[] = {r->d_off, r->d_adj, r->d_adj16, r->d_vmax, r->d_inv, r->d_emap, r->d_s16, r->d_s256};
+    void *p[] = {r->d_off, r->d_adj, r->d_adj16, r->d_vinfo, r->d_inv, r->d_emap, r->d_s16, r->d_s256};
     for (void *x : p)
         if (x) (void)hipFree(x);
     delete r;

@@ -85,7 +85,7 @@ struct rtg_tc_plan {
     const int32_t *v_off = nullptr;
     const int32_t *v_adj = nullptr;
     const uint16_t *v_adj16 = nullptr;
-    const int32_t *v_vmax = nullptr;   // the copy's row maxima
+    const int32_t *v_vinfo = nullptr;  // the copy's rows: {start, end, largest entry, 0} each
     const int32_t *v_s16 = nullptr;    // the copy's every 16th entry
     const int32_t *v_s256 = nullptr;   // the copy's every 256th entry
     const int32_t *v_emap = nullptr;   // per-edge plans on the copy: its edge -> the graph's edge
