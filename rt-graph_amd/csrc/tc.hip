@@ -462,15 +462,17 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
 // reading element j (coalesced buffer loads bounded by the list); shorter ones are
 // concatenated: their lengths are scanned into the wave's LDS slots and each lane walks a
 // cursor, so that lane j of a round handles element j of the concatenation.
-// kPerEdge: every span's count is also stored at eout[wedge[pi]] — a long span's by a wave
-// sum, a concatenated short span's through its LDS counter pe[ci].
+// kPerEdge: every span's count is also stored at eout[wedge[pi]] — a long span's counted with
+// ballots (a wave sum for kSorted), a concatenated short span's through its LDS counter pe[ci].
 // kSorted (the owner's sorted list in LDS) merges a long span with the owner's list: both
 // ascend, so a round of 64*U consecutive elements can only match the owner-list window between
 // the lower bound of its first element and the upper bound of its last (two searches the wave
 // shares), each lane searches only that window, and the next round starts where it ended.
 // One group of W loads of kW-byte elements per lane from buffer resource rs (lane offsets vo + u
-// * 64 * kW), then their W probes: the loads all issue before the first probe waits.
-template <uint32_t kMode, int W, typename T>
+// * 64 * kW), then their W probes: the loads all issue before the first probe waits. Returns the
+// lane's hits, or with kWave the whole wave's (uniform: a probe is 0 or 1, so each probe's hits
+// are the population count of a ballot, scalar work instead of a cross-lane sum per span).
+template <uint32_t kMode, int W, typename T, bool kWave = false>
 __device__ __forceinline__ uint32_t probe_group(const __amdgpu_buffer_rsrc_t &rs, int vo, const Table &c) {
     constexpr int kW = (int)sizeof(T);
     uint32_t x[W];
@@ -482,22 +484,25 @@ __device__ __forceinline__ uint32_t probe_group(const __amdgpu_buffer_rsrc_t &rs
     uint32_t h = 0;
 #pragma unroll
     for (int u = 0; u < W; ++u) {
-        if constexpr (kMode == kBitmap16 && kW == 2) h += bitmap16_has16(c, x[u]);
-        else h += probe<kMode>(c, x[u]);
+        uint32_t hu;
+        if constexpr (kMode == kBitmap16 && kW == 2) hu = bitmap16_has16(c, x[u]);
+        else hu = probe<kMode>(c, x[u]);
+        if constexpr (kWave) h += (uint32_t)__popcll(__ballot(hu != 0u));
+        else h += hu;
     }
     return h;
 }
 
 // The last, partial round of a span: nu (1 .. W) groups of 64 elements are left; the group of
 // the smallest power-of-two width >= nu runs. Returns its width in *w.
-template <uint32_t kMode, int W, typename T>
+template <uint32_t kMode, int W, typename T, bool kWave = false>
 __device__ __forceinline__ uint32_t tail_group(const __amdgpu_buffer_rsrc_t &rs, int vo, const Table &c, int32_t nu,
                                                int32_t *w) {
     if constexpr (W > 1) {
-        if (nu <= W / 2) return tail_group<kMode, W / 2, T>(rs, vo, c, nu, w);
+        if (nu <= W / 2) return tail_group<kMode, W / 2, T, kWave>(rs, vo, c, nu, w);
     }
     *w = W;
-    return probe_group<kMode, W, T>(rs, vo, c);
+    return probe_group<kMode, W, T, kWave>(rs, vo, c);
 }
 
 // The rounds of one long span base[0 .. l0): U loads of kW-byte elements per lane through a raw
@@ -507,8 +512,9 @@ __device__ __forceinline__ uint32_t tail_group(const __amdgpu_buffer_rsrc_t &rs,
 // Full rounds of 64*U elements come first; the last, partial round runs a group of 1, 2, 4 ..
 // U loads, the fewest that cover it (a span of 150 elements takes 256 probes per wave instead
 // of 512 at U = 8). Returns the span's hits, including the probes of 0 of the loads past the
-// end; *slots is the number of probes each lane made.
-template <uint32_t kMode, int U, typename T>
+// end; *slots is the number of probes each lane made. kWave: the wave's hits (uniform), as in
+// probe_group.
+template <uint32_t kMode, int U, typename T, bool kWave = false>
 __device__ __forceinline__ uint32_t long_rounds(const T *base, int32_t l0, const Table &c, int32_t *slots) {
     constexpr int kW = (int)sizeof(T);
     int vo = __lane_id() * kW;
@@ -521,14 +527,14 @@ __device__ __forceinline__ uint32_t long_rounds(const T *base, int32_t l0, const
         asm volatile("" : "+v"(vo));
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base + jb), (short)0,
                                                                             (l0 - jb) * kW, kBufferWord3);
-        h += probe_group<kMode, U, T>(rs, vo, c);
+        h += probe_group<kMode, U, T, kWave>(rs, vo, c);
     }
     int32_t w = 0;
     if (jb < l0) {
         asm volatile("" : "+v"(vo));
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base + jb), (short)0,
                                                                             (l0 - jb) * kW, kBufferWord3);
-        h += tail_group<kMode, U, T>(rs, vo, c, (l0 - jb + 63) >> 6, &w);
+        h += tail_group<kMode, U, T, kWave>(rs, vo, c, (l0 - jb + 63) >> 6, &w);
     }
     *slots = (jb >> 6) + w;
     return h;
@@ -601,11 +607,18 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             const int32_t s0r = __builtin_amdgcn_readlane(sp.x, p);
             const int32_t s0 = s0r & 0x7FFFFFFF;
             const int32_t l0 = __builtin_amdgcn_readlane(sp.y, p);
-            uint32_t sh = 0;  // this span's hits (per-edge counts only)
             uint32_t h;
             if constexpr (kMode == kSorted) {
                 if (kP16 && s0r < 0) h = merge_rounds<U>(adj16 + s0, l0, c);
                 else h = merge_rounds<U>(adj + s0, l0, c);
+                if constexpr (kPerEdge) h = rtgd::wave_sum_u32(h);
+            } else if constexpr (kPerEdge) {
+                // the span's hits over the wave, uniform; every lane made `slots` probes, the
+                // 64 * slots - l0 past the end probed 0 and hit iff the owner's list holds 0
+                int32_t slots = 0;
+                if (kP16 && s0r < 0) h = long_rounds<kMode, U, uint16_t, true>(adj16 + s0, l0, c, &slots);
+                else h = long_rounds<kMode, U, int32_t, true>(adj + s0, l0, c, &slots);
+                h -= has0 * (uint32_t)(64 * slots - l0);
             } else {
                 int32_t slots = 0;
                 if (kP16 && s0r < 0) h = long_rounds<kMode, U>(adj16 + s0, l0, c, &slots);
@@ -615,12 +628,13 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
                 const int32_t in = lane < l0 ? (l0 - lane + 63) >> 6 : 0;
                 h -= has0 * (uint32_t)(slots - in);
             }
-            if constexpr (kPerEdge) sh += h;
-            else hits += h;
             if constexpr (kPerEdge) {
-                hits += sh;
-                sh = rtgd::wave_sum_u32(sh);
-                if (lane == 0) eout[wedge[bs + p]] = sh;
+                if (lane == 0) {  // the span's count, once per wave
+                    hits += h;
+                    eout[wedge[bs + p]] = h;
+                }
+            } else {
+                hits += h;
             }
         }
         const bool is_short = !is_long && sp.y > 0;
