@@ -155,7 +155,7 @@ typedef struct rtg_tc_opts {
     int32_t autotune;        /* batched: time the candidate configurations when the cache has
                                 no entry (RTG_TC_TUNE in the environment overrides both)     */
     int32_t per_edge;        /* batched: also record each span's edge, so that
-                                rtg_tc_plan_run_per_edge can run (4 more bytes per edge)      */
+                                rtg_tc_plan_run_per_edge can run (8 more bytes per edge)      */
 } rtg_tc_opts;
 
 #define RTG_TC_MAX_BUCKETS 8

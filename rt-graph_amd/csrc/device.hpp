@@ -147,6 +147,20 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
+// The same sum through DPP lane moves (VALU only, no LDS permutes), returned uniform: row_shr 1,
+// 2, 4, 8 leave each row of 16 lanes its sum in its lane 15 (lanes shifted in from outside the
+// row read 0), row_bcast 15 adds lane 15's value to row 1 and lane 47's to row 3, row_bcast 31
+// lane 31's to rows 2 and 3, so lane 63 ends with the wave's sum.
+__device__ __forceinline__ uint32_t wave_sum_dpp(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // Inclusive scan across the 64 lanes.
 template <class T>
 __device__ __forceinline__ T wave_inclusive_scan(T v) {

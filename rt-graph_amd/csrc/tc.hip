@@ -428,11 +428,11 @@ __device__ __forceinline__ uint32_t probe(const Table &c, uint32_t x) {
 // binary-searched in the owner's sorted list own[0 .. d) in HBM. Kept out of line and compact
 // (no unrolling): it is rare, and a second copy of the streaming loop would only crowd the
 // instruction cache of the common path. With eout set, each span's count is also stored at
-// eout[wedge[pi]] (per-edge counts).
+// eout[pi] (per-edge counts, by span position).
 template <bool kPerEdge>
 __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                int64_t p0, int64_t p1, const int32_t *__restrict__ own, int32_t d,
-                                               const uint32_t *__restrict__ wedge, uint32_t *__restrict__ eout) {
+                                               uint32_t *__restrict__ eout) {
     uint32_t hits = 0;
     for (int64_t pi = p0; pi < p1; ++pi) {
         const int2 sp = wspan[pi];
@@ -450,7 +450,7 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
         hits += sh;
         if constexpr (kPerEdge) {
             sh = rtgd::wave_sum_u32(sh);
-            if (__lane_id() == 0) eout[wedge[pi]] = sh;
+            if (__lane_id() == 0) eout[pi] = sh;
         }
     }
     return hits;
@@ -462,17 +462,17 @@ __device__ __noinline__ uint32_t stream_search(const int32_t *__restrict__ adj, 
 // reading element j (coalesced buffer loads bounded by the list); shorter ones are
 // concatenated: their lengths are scanned into the wave's LDS slots and each lane walks a
 // cursor, so that lane j of a round handles element j of the concatenation.
-// kPerEdge: every span's count is also stored at eout[wedge[pi]] — a long span's counted with
-// ballots (a wave sum for kSorted), a concatenated short span's through its LDS counter pe[ci].
+// kPerEdge: every span's count is also stored at eout[pi], by span position (coalesced; a gather
+// puts them in edge order afterwards) — a long span's by a DPP wave
+// sum (the lanes' hits and pads summed once per span), a concatenated short span's through its
+// LDS counter pe[ci].
 // kSorted (the owner's sorted list in LDS) merges a long span with the owner's list: both
 // ascend, so a round of 64*U consecutive elements can only match the owner-list window between
 // the lower bound of its first element and the upper bound of its last (two searches the wave
 // shares), each lane searches only that window, and the next round starts where it ended.
 // One group of W loads of kW-byte elements per lane from buffer resource rs (lane offsets vo + u
-// * 64 * kW), then their W probes: the loads all issue before the first probe waits. Returns the
-// lane's hits, or with kWave the whole wave's (uniform: a probe is 0 or 1, so each probe's hits
-// are the population count of a ballot, scalar work instead of a cross-lane sum per span).
-template <uint32_t kMode, int W, typename T, bool kWave = false>
+// * 64 * kW), then their W probes: the loads all issue before the first probe waits.
+template <uint32_t kMode, int W, typename T>
 __device__ __forceinline__ uint32_t probe_group(const __amdgpu_buffer_rsrc_t &rs, int vo, const Table &c) {
     constexpr int kW = (int)sizeof(T);
     uint32_t x[W];
@@ -484,25 +484,22 @@ __device__ __forceinline__ uint32_t probe_group(const __amdgpu_buffer_rsrc_t &rs
     uint32_t h = 0;
 #pragma unroll
     for (int u = 0; u < W; ++u) {
-        uint32_t hu;
-        if constexpr (kMode == kBitmap16 && kW == 2) hu = bitmap16_has16(c, x[u]);
-        else hu = probe<kMode>(c, x[u]);
-        if constexpr (kWave) h += (uint32_t)__popcll(__ballot(hu != 0u));
-        else h += hu;
+        if constexpr (kMode == kBitmap16 && kW == 2) h += bitmap16_has16(c, x[u]);
+        else h += probe<kMode>(c, x[u]);
     }
     return h;
 }
 
 // The last, partial round of a span: nu (1 .. W) groups of 64 elements are left; the group of
 // the smallest power-of-two width >= nu runs. Returns its width in *w.
-template <uint32_t kMode, int W, typename T, bool kWave = false>
+template <uint32_t kMode, int W, typename T>
 __device__ __forceinline__ uint32_t tail_group(const __amdgpu_buffer_rsrc_t &rs, int vo, const Table &c, int32_t nu,
                                                int32_t *w) {
     if constexpr (W > 1) {
-        if (nu <= W / 2) return tail_group<kMode, W / 2, T, kWave>(rs, vo, c, nu, w);
+        if (nu <= W / 2) return tail_group<kMode, W / 2, T>(rs, vo, c, nu, w);
     }
     *w = W;
-    return probe_group<kMode, W, T, kWave>(rs, vo, c);
+    return probe_group<kMode, W, T>(rs, vo, c);
 }
 
 // The rounds of one long span base[0 .. l0): U loads of kW-byte elements per lane through a raw
@@ -512,9 +509,8 @@ __device__ __forceinline__ uint32_t tail_group(const __amdgpu_buffer_rsrc_t &rs,
 // Full rounds of 64*U elements come first; the last, partial round runs a group of 1, 2, 4 ..
 // U loads, the fewest that cover it (a span of 150 elements takes 256 probes per wave instead
 // of 512 at U = 8). Returns the span's hits, including the probes of 0 of the loads past the
-// end; *slots is the number of probes each lane made. kWave: the wave's hits (uniform), as in
-// probe_group.
-template <uint32_t kMode, int U, typename T, bool kWave = false>
+// end; *slots is the number of probes each lane made.
+template <uint32_t kMode, int U, typename T>
 __device__ __forceinline__ uint32_t long_rounds(const T *base, int32_t l0, const Table &c, int32_t *slots) {
     constexpr int kW = (int)sizeof(T);
     int vo = __lane_id() * kW;
@@ -527,14 +523,14 @@ __device__ __forceinline__ uint32_t long_rounds(const T *base, int32_t l0, const
         asm volatile("" : "+v"(vo));
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base + jb), (short)0,
                                                                             (l0 - jb) * kW, kBufferWord3);
-        h += probe_group<kMode, U, T, kWave>(rs, vo, c);
+        h += probe_group<kMode, U, T>(rs, vo, c);
     }
     int32_t w = 0;
     if (jb < l0) {
         asm volatile("" : "+v"(vo));
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base + jb), (short)0,
                                                                             (l0 - jb) * kW, kBufferWord3);
-        h += tail_group<kMode, U, T, kWave>(rs, vo, c, (l0 - jb + 63) >> 6, &w);
+        h += tail_group<kMode, U, T>(rs, vo, c, (l0 - jb + 63) >> 6, &w);
     }
     *slots = (jb >> 6) + w;
     return h;
@@ -588,8 +584,7 @@ __device__ __forceinline__ uint32_t merge_rounds(const T *base, int32_t l0, cons
 template <uint32_t kMode, int U, bool kPerEdge, bool kP16 = false>
 __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                  int64_t p0, int64_t p1, const Table &c, int32_t long_min,
-                                                 int32_t *ex, int32_t *st, const uint32_t *__restrict__ wedge,
-                                                 uint32_t *__restrict__ eout, uint32_t *pe,
+                                                 int32_t *ex, int32_t *st, uint32_t *__restrict__ eout, uint32_t *pe,
                                                  const uint16_t *__restrict__ adj16 = nullptr) {
     const int lane = __lane_id();
     const uint64_t lt = rtgd::lanemask_lt();
@@ -611,14 +606,14 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             if constexpr (kMode == kSorted) {
                 if (kP16 && s0r < 0) h = merge_rounds<U>(adj16 + s0, l0, c);
                 else h = merge_rounds<U>(adj + s0, l0, c);
-                if constexpr (kPerEdge) h = rtgd::wave_sum_u32(h);
+                if constexpr (kPerEdge) h = rtgd::wave_sum_dpp(h);
             } else if constexpr (kPerEdge) {
                 // the span's hits over the wave, uniform; every lane made `slots` probes, the
                 // 64 * slots - l0 past the end probed 0 and hit iff the owner's list holds 0
                 int32_t slots = 0;
-                if (kP16 && s0r < 0) h = long_rounds<kMode, U, uint16_t, true>(adj16 + s0, l0, c, &slots);
-                else h = long_rounds<kMode, U, int32_t, true>(adj + s0, l0, c, &slots);
-                h -= has0 * (uint32_t)(64 * slots - l0);
+                if (kP16 && s0r < 0) h = long_rounds<kMode, U>(adj16 + s0, l0, c, &slots);
+                else h = long_rounds<kMode, U>(adj + s0, l0, c, &slots);
+                h = rtgd::wave_sum_dpp(h) - has0 * (uint32_t)(64 * slots - l0);
             } else {
                 int32_t slots = 0;
                 if (kP16 && s0r < 0) h = long_rounds<kMode, U>(adj16 + s0, l0, c, &slots);
@@ -631,7 +626,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             if constexpr (kPerEdge) {
                 if (lane == 0) {  // the span's count, once per wave
                     hits += h;
-                    eout[wedge[bs + p]] = h;
+                    eout[bs + p] = h;
                 }
             } else {
                 hits += h;
@@ -678,7 +673,7 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
             }
             rtgd::wave_sync_lds();
             if constexpr (kPerEdge) {
-                if (is_short) eout[wedge[pi]] = pe[__popcll(sm & lt)];
+                if (is_short) eout[pi] = pe[__popcll(sm & lt)];
                 rtgd::wave_sync_lds();  // pe is rewritten by the next group of partners
             }
         }
@@ -691,22 +686,20 @@ __device__ __forceinline__ uint32_t stream_spans(const int32_t *__restrict__ adj
 template <uint32_t kMode, int U, bool kPerEdge>
 __device__ __noinline__ uint32_t stream_spans_ool(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                   int64_t p0, int64_t p1, Table c, int32_t long_min, int32_t *ex,
-                                                  int32_t *st, const uint32_t *__restrict__ wedge,
-                                                  uint32_t *__restrict__ eout, uint32_t *pe) {
-    return stream_spans<kMode, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+                                                  int32_t *st, uint32_t *__restrict__ eout, uint32_t *pe) {
+    return stream_spans<kMode, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe);
 }
 
 // With 16-bit spans: linear probing (the hash's fallback) and the sorted list, out of line.
 template <bool kP16, int U, bool kPerEdge>
 __device__ __noinline__ uint32_t stream_spans_ool(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                   int64_t p0, int64_t p1, Table c, int32_t long_min, int32_t *ex,
-                                                  int32_t *st, const uint32_t *__restrict__ wedge,
-                                                  uint32_t *__restrict__ eout, uint32_t *pe,
+                                                  int32_t *st, uint32_t *__restrict__ eout, uint32_t *pe,
                                                   const uint16_t *__restrict__ adj16, uint32_t mode) {
     if (mode == kLinearProbe)
-        return stream_spans<kLinearProbe, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe,
+        return stream_spans<kLinearProbe, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe,
                                                               adj16);
-    return stream_spans<kSorted, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+    return stream_spans<kSorted, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe, adj16);
 }
 
 // Partners [p0, p1) against the owner table c, in the probe mode the table was built in. kAll:
@@ -716,25 +709,24 @@ __device__ __noinline__ uint32_t stream_spans_ool(const int32_t *__restrict__ ad
 template <int U, bool kPerEdge, bool kAll, bool kP16 = false>
 __device__ __forceinline__ uint32_t stream_table(const int32_t *__restrict__ adj, const int2 *__restrict__ wspan,
                                                  int64_t p0, int64_t p1, const Table &c, int32_t long_min, int32_t *ex,
-                                                 int32_t *st, const uint32_t *__restrict__ wedge,
-                                                 uint32_t *__restrict__ eout, uint32_t *pe,
+                                                 int32_t *st, uint32_t *__restrict__ eout, uint32_t *pe,
                                                  const uint16_t *__restrict__ adj16 = nullptr) {
     const uint32_t mode = rtgd::uniform_u32(c.mode);
     if constexpr (kP16) {
         static_assert(kAll, "16-bit spans come with every strategy");
         if (mode == kBitmap16)
-            return stream_spans<kBitmap16, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
+            return stream_spans<kBitmap16, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe, adj16);
         if (mode == kCuckoo)
-            return stream_spans<kCuckoo, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16);
-        return stream_spans_ool<kP16, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe, adj16, mode);
+            return stream_spans<kCuckoo, U, kPerEdge, true>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe, adj16);
+        return stream_spans_ool<kP16, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe, adj16, mode);
     } else {
-        if (mode == kCuckoo) return stream_spans<kCuckoo, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+        if (mode == kCuckoo) return stream_spans<kCuckoo, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe);
         if (!kAll || mode == kLinearProbe)
-            return stream_spans<kLinearProbe, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+            return stream_spans<kLinearProbe, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe);
         if constexpr (kAll) {
             if (mode == kBitmap)
-                return stream_spans_ool<kBitmap, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
-            return stream_spans_ool<kSorted, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, wedge, eout, pe);
+                return stream_spans_ool<kBitmap, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe);
+            return stream_spans_ool<kSorted, U, kPerEdge>(adj, wspan, p0, p1, c, long_min, ex, st, eout, pe);
         }
         return 0u;
     }
@@ -753,8 +745,7 @@ struct BatchArgs {
     int32_t wave_table;         // wave-private table slots
     int32_t long_min;           // direct-stream threshold
     int32_t grab;               // partners a wave takes per grab in a block item (<= 64)
-    const uint32_t *wedge;      // per-edge counts: oriented edge of each span (else null)
-    uint32_t *eout;             // per-edge counts: output by oriented edge (else null)
+    uint32_t *eout;             // per-edge counts: each span's count by position (else null)
     const uint16_t *adj16;      // 16-bit copy of adj (degree-ordered plans, else null)
 };
 
@@ -823,7 +814,7 @@ __global__ __launch_bounds__(kThreads, kP16 ? 7 : (kPerEdge || kAll) ? 8 : 1) vo
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
-                hits += stream_table<U, kPerEdge, kAll, kP16>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
+                hits += stream_table<U, kPerEdge, kAll, kP16>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, 
                                                               a.eout, pe, a.adj16);
             }
         } else if (kind == kItemSearch) {
@@ -837,7 +828,7 @@ __global__ __launch_bounds__(kThreads, kP16 ? 7 : (kPerEdge || kAll) ? 8 : 1) vo
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
-                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d, a.wedge, a.eout);
+                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d,  a.eout);
             }
         } else {
             // Wave items: small owners small_list[it.b, it.c), one per wave at a time, each in a
@@ -850,7 +841,7 @@ __global__ __launch_bounds__(kThreads, kP16 ? 7 : (kPerEdge || kAll) ? 8 : 1) vo
                 const Table c = mode == kCuckoo ? build_wave_table(tab, a.adj, xb, d)
                                                 : build_wave_other<kP16>(tab, a.adj, xb, d, mode, (uint32_t)a.wave_table);
                 hits += stream_table<U, kPerEdge, kAll, kP16>(a.adj, a.wspan, a.wofs[xs], a.wofs[xs + 1], c, a.long_min,
-                                                              ex, st, a.wedge, a.eout, pe, a.adj16);
+                                                              ex, st,  a.eout, pe, a.adj16);
                 rtgd::wave_sync_lds();
             }
         }
@@ -905,10 +896,10 @@ __global__ __launch_bounds__(kThreads, 8) void tc_bitmap_items(BatchArgs a) {
             if (p0 >= it.c) break;
             const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
             if (fits)
-                hits += stream_spans<kBitmap16, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
+                hits += stream_spans<kBitmap16, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, 
                                                                    a.eout, pe, a.adj16);
             else
-                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d, a.wedge, a.eout);
+                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d,  a.eout);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -977,11 +968,11 @@ __global__ __launch_bounds__(kThreads, 8) void tc_wave_items(BatchArgs a) {
             const Table c = build_wave_table(tab, a.adj, xb, d);
             const int64_t p0 = a.wofs[xs], p1 = a.wofs[xs + 1];
             if (rtgd::uniform_u32(c.mode) == kCuckoo)
-                hits += stream_spans<kCuckoo, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, a.wedge,
+                hits += stream_spans<kCuckoo, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, 
                                                                  a.eout, pe, a.adj16);
             else
                 hits += stream_spans<kLinearProbe, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st,
-                                                                      a.wedge, a.eout, pe, a.adj16);
+                                                                       a.eout, pe, a.adj16);
             rtgd::wave_sync_lds();
         }
         __syncthreads();  // every wave is done with item_slot
@@ -1271,6 +1262,42 @@ __global__ void plan_spans(const int32_t *__restrict__ off, const int32_t *__res
     }
 }
 
+// Per-edge plans: epos[e - e_begin] = i for span i of positive length and its oriented edge e =
+// wedge[i] (the fill leaves kNoSpan for edges without one, whose count is 0): where each edge's
+// count lies among the spans' counts, which the counting kernels store by span position.
+constexpr uint32_t kNoSpan = 0xFFFFFFFFu;
+
+__global__ void edge_positions(const uint32_t *__restrict__ wedge, const int2 *__restrict__ wspan, int64_t nw,
+                               int64_t e_begin, uint32_t *__restrict__ epos) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x)
+        if (wspan[i].y > 0) epos[(int64_t)wedge[i] - e_begin] = (uint32_t)i;
+}
+
+// out[j] = the count of the range's edge j: ecnt at its span position, 0 without one. The
+// per-edge output is written in edge order with whole-line stores; storing each span's count
+// straight at its edge (random 4-byte stores, a partial line each) cost R-MAT scale 21 2.4 ms
+// of a 12.3 ms per-edge run. kPlanU edges per thread and round keep the random reads in flight.
+__global__ __launch_bounds__(256) void gather_edge_counts(const uint32_t *__restrict__ epos, int64_t ne,
+                                                          const uint32_t *__restrict__ ecnt,
+                                                          uint32_t *__restrict__ out) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j0 < ne; j0 += stride * kPlanU) {
+        uint32_t q[kPlanU], v[kPlanU];
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            const int64_t j = j0 + k * stride;
+            q[k] = j < ne ? epos[j] : kNoSpan;
+        }
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) v[k] = q[k] != kNoSpan ? ecnt[q[k]] : 0u;
+#pragma unroll
+        for (int k = 0; k < kPlanU; ++k) {
+            const int64_t j = j0 + k * stride;
+            if (j < ne) out[j] = v[k];
+        }
+    }
+}
+
 // At each owner's last position: its partner count and its streamed elements (from the
 // exclusive prefix sums of the span lengths).
 __global__ void plan_runs(const uint64_t *__restrict__ sorted, int64_t nw, int bits, const uint32_t *__restrict__ first,
@@ -1426,7 +1453,7 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     if (ne >= ((int64_t)1 << (63 - bits)) || bits > 32)
         return set_error(RTG_EINVAL, "batched TC: edge range too large for the plan keys");
     const size_t kn = (size_t)std::max<int64_t>(ne, 1);
-    const size_t bytes = Arena::round(sizeof(int32_t) * (size_t)std::max<int64_t>(g->m, 1)) +
+    const size_t bytes = Arena::round(sizeof(int32_t) * (size_t)std::max<int64_t>(g->m, 1)) * (p->opts.per_edge ? 2 : 1) +
                          Arena::round(sizeof(uint32_t) * ((size_t)n + 1)) * 2 + Arena::round(64) +
                          Arena::round(sizeof(uint64_t) * kn) * 4 + Arena::round(radix_sort_work_bytes((int64_t)kn)) +
                          Arena::round(std::max(scan_work_bytes((int64_t)kn), scan_work_bytes(n + 1))) + 4096;
@@ -1437,6 +1464,8 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     RTG_TRY(ctx_scratch(ctx, bytes, &sbuf));
     Arena ar{static_cast<char *>(sbuf), 0, bytes};
     int32_t *row = ar.take<int32_t>((size_t)std::max<int64_t>(g->m, 1));
+    // per-edge plans: each span's oriented edge (at most m spans)
+    uint32_t *wedge = p->opts.per_edge ? ar.take<uint32_t>((size_t)std::max<int64_t>(g->m, 1)) : nullptr;
     uint32_t *d_cnt = ar.take<uint32_t>((size_t)n + 1);
     uint32_t *d_first = ar.take<uint32_t>((size_t)n + 1);
     unsigned long long *d_bytes = ar.take<unsigned long long>(8);  // alg, merge, edges, flag, fingerprint
@@ -1472,10 +1501,14 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
     p->offsets_fingerprint = hb[4];
     const int64_t nw = p->n_work_edges;
     RTG_TRY(dev_alloc(&p->d_wspan, (size_t)std::max<int64_t>(nw, 1)));
-    if (p->opts.per_edge) RTG_TRY(dev_alloc(&p->d_wedge, (size_t)std::max<int64_t>(nw, 1)));
+    if (p->opts.per_edge) {
+        RTG_TRY(dev_alloc(&p->d_epos, (size_t)std::max<int64_t>(ne, 1)));
+        RTG_TRY(dev_alloc(&p->d_ecnt, (size_t)std::max<int64_t>(nw, 1)));
+        HIP_TRY(hipMemsetAsync(p->d_epos, 0xFF, sizeof(uint32_t) * (size_t)std::max<int64_t>(ne, 1), st));
+    }
     if (nw > 0) {
         plan_spans<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(p->v_off, p->v_adj, row, p->e_begin, keys[which],
-                                                              nw, bits, p->d_wspan, p->d_wedge, lens, d_first,
+                                                              nw, bits, p->d_wspan, wedge, lens, d_first,
                                                               reinterpret_cast<const int4 *>(p->v_vinfo),
                                                               p->v_s16, p->v_s256, d_bytes + 5, p->v_emap);
         HIP_TRY(hipGetLastError());
@@ -1483,6 +1516,10 @@ int plan_graph(rtg_ctx *ctx, rtg_tc_plan *p) {
         plan_runs<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(keys[which], nw, bits, d_first, lens, lens_ex, d_cnt,
                                                              p->d_work);
         HIP_TRY(hipGetLastError());
+        if (wedge) {
+            edge_positions<<<grid_cap(ctx, nw, 256, 16), 256, 0, st>>>(wedge, p->d_wspan, nw, p->e_begin, p->d_epos);
+            HIP_TRY(hipGetLastError());
+        }
     }
     RTG_TRY(dev_alloc(&p->d_wofs, (size_t)n + 1));
     RTG_TRY(scan_exclusive_u32_w(ctx, d_cnt, (uint32_t *)p->d_wofs, n + 1, nullptr, scan_work));
@@ -2183,7 +2220,7 @@ int launch_items(rtg_tc_plan *p, hipStream_t st, BatchKernel kb, int32_t grid_b,
     const bool beside = p->side;
     BatchArgs a{p->v_off, p->v_adj, p->d_wofs, p->d_wspan, p->d_small, p->d_items,
                 p->n_bitmap_items, p->d_count, p->bitmap_table, 0,
-                p->long_min, p->grab, eout ? p->d_wedge : nullptr, eout, p->v_adj16};
+                p->long_min, p->grab, eout, p->v_adj16};
     BatchArgs aw = a;  // the wave items'
     aw.items = p->d_items + (p->n_items - p->n_wave_items);
     aw.n_items = p->n_wave_items;
@@ -2246,16 +2283,20 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
 // Per-edge counts of the plan's edge range: the range of eout is cleared, then the per-edge
 // instantiation of the batched kernel writes every span's count at its edge (edges whose
 // shorter list is empty keep 0). The total is accumulated as in a count.
+// Per-edge counts: the counting kernels store each span's count by position (p->d_ecnt), then
+// the gather writes the range's counts into eout in edge order.
 int launch_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *eout) {
     hipStream_t st = ctx->stream;
     const int64_t ne = p->e_end - p->e_begin;
-    if (ne > 0) HIP_TRY(hipMemsetAsync(eout + p->e_begin, 0, sizeof(uint32_t) * (size_t)ne, st));
     reset_counters<<<1, 64, 0, st>>>(p->d_count);
     HIP_TRY(hipGetLastError());
-    if (p->n_items == 0) return RTG_OK;
-    RTG_TRY(launch_items(p, st, (BatchKernel)p->kernel_pe_bm, p->grid_pe_bm, (BatchKernel)p->kernel_pe, p->grid_pe,
-                         (BatchKernel)p->kernel_pe_wv, p->grid_pe_wv, eout));
-    HIP_TRY(hipGetLastError());
+    if (p->n_items > 0)
+        RTG_TRY(launch_items(p, st, (BatchKernel)p->kernel_pe_bm, p->grid_pe_bm, (BatchKernel)p->kernel_pe, p->grid_pe,
+                             (BatchKernel)p->kernel_pe_wv, p->grid_pe_wv, p->d_ecnt));
+    if (ne > 0) {
+        gather_edge_counts<<<grid_cap(ctx, ne, 256, 16), 256, 0, st>>>(p->d_epos, ne, p->d_ecnt, eout + p->e_begin);
+        HIP_TRY(hipGetLastError());
+    }
     return RTG_OK;
 }
 
@@ -2447,7 +2488,7 @@ RTG_API int rtg_tc_plan_destroy(rtg_tc_plan *p) {
     if (p->ctx) (void)hipSetDevice(p->ctx->device);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    void *bufs[] = {p->d_row, p->d_wofs, p->d_wspan, p->d_wedge, p->d_small, p->d_items, p->d_count, p->d_work};
+    void *bufs[] = {p->d_row, p->d_wofs, p->d_wspan, p->d_epos, p->d_ecnt, p->d_small, p->d_items, p->d_count, p->d_work};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete p;

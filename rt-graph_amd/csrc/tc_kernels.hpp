@@ -29,10 +29,11 @@ struct rtg_tc_plan {
     rtg_tc_opts opts{};
     int64_t e_begin = 0, e_end = 0;
     // device arrays
-    int32_t *d_row = nullptr;        // edge -> source row (all methods)
+    int32_t *d_row = nullptr;        // edge -> source row (plain methods)
     int32_t *d_wofs = nullptr;       // batched: partner-list offsets per owner (n+1)
     int2 *d_wspan = nullptr;         // batched: partner out-list spans {start, length} by owner
-    uint32_t *d_wedge = nullptr;     // batched, per-edge plans: oriented edge of each span
+    uint32_t *d_epos = nullptr;      // batched, per-edge plans: each edge's span position (or kNoSpan)
+    uint32_t *d_ecnt = nullptr;      // batched, per-edge plans: each span's count, by position
     int32_t *d_small = nullptr;      // batched: small owners, heaviest first
     TcItem *d_items = nullptr;       // batched: work queue, heaviest first
     unsigned long long *d_work = nullptr;  // batched: streamed partner elements per owner (n+1)
