@@ -828,7 +828,7 @@ __global__ __launch_bounds__(kThreads, kP16 ? 7 : (kPerEdge || kAll) ? 8 : 1) vo
                 const int64_t p0 = (int64_t)it.b + b;
                 if (p0 >= it.c) break;
                 const int64_t p1 = min((int64_t)it.c, p0 + a.grab);
-                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d,  a.eout);
+                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d, a.eout);
             }
         } else {
             // Wave items: small owners small_list[it.b, it.c), one per wave at a time, each in a
@@ -841,7 +841,7 @@ __global__ __launch_bounds__(kThreads, kP16 ? 7 : (kPerEdge || kAll) ? 8 : 1) vo
                 const Table c = mode == kCuckoo ? build_wave_table(tab, a.adj, xb, d)
                                                 : build_wave_other<kP16>(tab, a.adj, xb, d, mode, (uint32_t)a.wave_table);
                 hits += stream_table<U, kPerEdge, kAll, kP16>(a.adj, a.wspan, a.wofs[xs], a.wofs[xs + 1], c, a.long_min,
-                                                              ex, st,  a.eout, pe, a.adj16);
+                                                              ex, st, a.eout, pe, a.adj16);
                 rtgd::wave_sync_lds();
             }
         }
@@ -899,7 +899,7 @@ __global__ __launch_bounds__(kThreads, 8) void tc_bitmap_items(BatchArgs a) {
                 hits += stream_spans<kBitmap16, U, kPerEdge, true>(a.adj, a.wspan, p0, p1, c, a.long_min, ex, st, 
                                                                    a.eout, pe, a.adj16);
             else
-                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d,  a.eout);
+                hits += stream_search<kPerEdge>(a.adj, a.wspan, p0, p1, a.adj + xb, d, a.eout);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -2030,7 +2030,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 17;
+constexpr int kTuneRevision = 18;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
