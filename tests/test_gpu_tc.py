@@ -119,8 +119,9 @@ def test_per_edge_counts(ctx, name):
 
 @pytest.mark.parametrize("name", list(GRAPHS))
 def test_per_edge_plan(ctx, name):
-    """Batched per-edge plans (every span's count stored at its edge) on every test graph,
-    with and without an edge range; run twice to check that the output range is cleared."""
+    """Batched per-edge plans (every span's count stored by position, then gathered in edge
+    order) on every test graph, with and without an edge range (entries outside it untouched);
+    run twice to check that every run rewrites the output range."""
     off, adj, t = oracle_csr(name)
     g = rtg.DeviceGraph.from_csr(ctx, off, adj)
     exp = orc.tc_per_edge(off, adj, 0)
