@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -1937,37 +1938,50 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     // items for the general kernel: after the block bitmaps, and before the wave items when those
     // have their own kernel
     const int64_t n_general = p->n_items - p->n_bitmap_items - (p->renumbered && small_hashed ? (int64_t)ng : 0);
-    // occupancy-sized persistent grid of kernel kk for `items` queue entries; `share` (beside the
-    // other kernel): -1 one block per CU fewer than fit, +1 one block per CU
-    const auto grid_of_kernel = [&](BatchKernel kk, int64_t items, int32_t *grid, size_t kl = 0, int share = 0) -> int {
+    // occupancy-sized persistent grid of kernel kk for `items` queue entries; `side` (the wave
+    // kernel beside the bitmap kernel): blocks per CU in proportion to the wave items' share of
+    // the count's time
+    const auto grid_of_kernel = [&](BatchKernel kk, int64_t items, int32_t *grid, size_t kl = 0, bool side = false) -> int {
         int pc = 0;
         if (!kl) kl = lds;
         HIP_TRY(hipFuncSetAttribute((const void *)kk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kl));
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kk, threads, kl));
-        if (share < 0) pc -= 1;
-        if (share > 0) pc = 1;
+        if (side) {
+            pc = std::max(1, std::min(pc, (int)std::ceil(2.0 * pc * p->wave_share)));
+            p->side_blocks = pc;
+        }
         *grid = (int32_t)std::min<int64_t>((int64_t)std::max(pc, 1) * ctx->cu_count, std::max<int64_t>(items, 1));
         return RTG_OK;
     };
     // degree-ordered plans whose small owners are hashed run the wave items in their own kernel
     p->n_wave_items = p->renumbered && small_hashed ? (int64_t)ng : 0;
-    // The wave items run beside the block bitmaps (one block per CU) when they hold at most 1/16 of
-    // the streamed elements (1.6% on R-MAT scale 21; the configuration model's 16% would leave
-    // the bitmap kernel waiting for them), else after them.
-    p->side = p->n_bitmap_items > 0 && p->n_wave_items > 0 && 16 * p->small_work <= p->streamed;
+    // The wave items run beside the block bitmaps, on the context's side stream. The bitmap
+    // kernel keeps its full grid: its blocks beyond what fits start as the wave kernel's retire.
+    // The wave kernel gets blocks per CU in proportion to its share of the count's time (a wave
+    // item's element costs about 4 block bitmap elements), doubled so that it does not become the
+    // tail (the bitmap kernel, bound by HBM, loses little to it). Measured on R-MAT scale 21 (the
+    // small owners 1.6% of the streamed elements: 1 block per CU) 7.16 ms, 7.20-7.40 with 2-8
+    // blocks; on the n = 3M configuration model (16%: 7 blocks) 6.11-6.15 ms with 4-8 blocks,
+    // 6.43 with 3, 15.7 with 1, 6.47 with the two kernels one after the other.
+    p->side = p->n_bitmap_items > 0 && p->n_wave_items > 0;
+    p->side_blocks = 0;
+    {
+        const double sw = 4.0 * (double)p->small_work;
+        const double rest = (double)std::max<int64_t>(p->streamed - p->small_work, 0);
+        p->wave_share = sw / std::max(sw + rest, 1.0);
+    }
     // its wave tables: both cuckoo halves of the largest small owner (not the block table's share)
     p->wave_table_wv = std::min<int32_t>(ts.wave_table, 2 * (int32_t)cuckoo_half(ts.wave_owner_max));
     const size_t lds_wv = sizeof(uint32_t) * (size_t)p->wave_table_wv * (size_t)waves;
     if (p->n_wave_items > 0) {
         BatchKernel kw = wave_kernel(threads, tune.unroll, false);
         p->kernel_wv = (void *)kw;
-        RTG_TRY(grid_of_kernel(kw, p->n_wave_items, &p->grid_wv, lds_wv, p->side ? 1 : 0));
+        RTG_TRY(grid_of_kernel(kw, p->n_wave_items, &p->grid_wv, lds_wv, p->side));
     }
     if (p->n_bitmap_items > 0) {
         BatchKernel kb = bitmap_kernel(threads, tune.unroll, false);
         p->kernel_bm = (void *)kb;
-        RTG_TRY(grid_of_kernel(kb, p->n_bitmap_items, &p->grid_bm, sizeof(uint32_t) * (size_t)p->bitmap_table,
-                               p->side ? -1 : 0));
+        RTG_TRY(grid_of_kernel(kb, p->n_bitmap_items, &p->grid_bm, sizeof(uint32_t) * (size_t)p->bitmap_table));
     }
     if (p->opts.per_edge) {
         BatchKernel ke = batched_kernel(threads, tune.unroll, true, p->all_strategies, p->renumbered);
@@ -1976,13 +1990,12 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         if (p->n_bitmap_items > 0) {
             BatchKernel kbe = bitmap_kernel(threads, tune.unroll, true);
             p->kernel_pe_bm = (void *)kbe;
-            RTG_TRY(grid_of_kernel(kbe, p->n_bitmap_items, &p->grid_pe_bm, sizeof(uint32_t) * (size_t)p->bitmap_table,
-                                   p->side ? -1 : 0));
+            RTG_TRY(grid_of_kernel(kbe, p->n_bitmap_items, &p->grid_pe_bm, sizeof(uint32_t) * (size_t)p->bitmap_table));
         }
         if (p->n_wave_items > 0) {
             BatchKernel kwe = wave_kernel(threads, tune.unroll, true);
             p->kernel_pe_wv = (void *)kwe;
-            RTG_TRY(grid_of_kernel(kwe, p->n_wave_items, &p->grid_pe_wv, lds_wv, p->side ? 1 : 0));
+            RTG_TRY(grid_of_kernel(kwe, p->n_wave_items, &p->grid_pe_wv, lds_wv, p->side));
         }
     }
     p->unroll = tune.unroll;
@@ -2030,7 +2043,7 @@ const int32_t kBucketCandidates[kBuckets][4] = {
 // Revision of the batched kernel's code and candidates, part of the cache key: raised when a
 // change to the kernel or to the candidate list can change which candidate the autotuner keeps,
 // so that entries tuned earlier are re-tuned.
-constexpr int kTuneRevision = 18;
+constexpr int kTuneRevision = 19;
 
 // Cache key: device architecture and CU count, kernel revision, graph shape, the row offsets'
 // fingerprint, the edge range and whether the plan is for per-edge counts or counts on the
@@ -2210,10 +2223,10 @@ int plan_batched(rtg_ctx *ctx, rtg_tc_plan *p) {
 // The batched queue, in up to three kernels: the block bitmap items of a degree-ordered plan
 // (items [0, n_bitmap_items)) by kb, the other block items (and the wave items, unless they have
 // their own kernel) by the general kernel kg, and the wave items (the queue's last n_wave_items)
-// by kw. With few wave items (p->side), the wave kernel runs on the context's side stream beside
-// the bitmap kernel (its grid and the bitmap kernel's are sized to fit the device together): most
-// wave items are short, and the long tail of the few heavy ones no longer follows the bitmap
-// kernel. eout: per-edge counts.
+// by kw. With both bitmap and wave items (p->side), the wave kernel runs on the context's side
+// stream beside the bitmap kernel (its blocks per CU sized by its share of the time, the bitmap
+// kernel's grid full): the tail of the heavy wave items no longer follows the bitmap kernel.
+// eout: per-edge counts.
 int launch_items(rtg_tc_plan *p, hipStream_t st, BatchKernel kb, int32_t grid_b, BatchKernel kg, int32_t grid_g,
                  BatchKernel kw, int32_t grid_w, uint32_t *eout) {
     rtg_ctx *ctx = p->ctx;
@@ -2478,7 +2491,8 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
              ",renumber_ms=" + std::to_string(p->t_renum_ms) + ",streamed=" + std::to_string(p->streamed) + ",streamed16=" + std::to_string(p->streamed16) +
              ",bitmap_items=" + std::to_string(p->n_bitmap_items) + ",grid_bitmap=" + std::to_string(p->grid_bm) +
              ",wave_items=" + std::to_string(p->n_wave_items) + ",grid_wave=" + std::to_string(p->grid_wv) +
-             ",small_work=" + std::to_string(p->small_work) + ",side=" + (p->side ? "1" : "0");
+             ",small_work=" + std::to_string(p->small_work) + ",side=" + (p->side ? "1" : "0") +
+             ",side_blocks=" + std::to_string(p->side_blocks);
     std::snprintf(buf, len, "%s", d.c_str());
     return RTG_OK;
 }

@@ -66,6 +66,8 @@ struct rtg_tc_plan {
     int32_t wave_table_wv = 0;       // the wave-items kernel's slots per wave
     int64_t small_work = 0;          // streamed elements of the small owners
     bool side = false;               // the wave kernel runs beside the bitmap kernel
+    double wave_share = 0.0;         // the wave items' estimated share of the count's time
+    int32_t side_blocks = 0;         // the wave kernel's blocks per CU there (the last grid sized)
     // byte models: per-edge (no reuse), merge-only, and the owner-staged launch's reads
     uint64_t alg_bytes = 0, merge_bytes = 0, staged_bytes = 0;
     // captured launch sequence
