@@ -1804,6 +1804,13 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     if (threads != 256 && threads != 512) return set_error(RTG_EINVAL, "batched TC: threads must be 256 or 512");
     const int waves = threads / 64;
     const TableSizes ts = table_sizes(tune, g->max_row);
+    // Wave items of at most 1/4 per CU of all the streamed elements: on a small graph the tuned
+    // size leaves most CUs idle (BA(100,000, 5): all 91,184 owners small, 156 wave items of
+    // 4,096 elements per wave took 1.23 ms, 1,226 items of 512 0.23 ms). Graphs of more than
+    // 2^23 streamed elements keep the tuned size at 2,048.
+    const int64_t wave_item = std::min<int64_t>(
+        tune.wave_item_work, std::max<int64_t>(256, p->streamed / ((int64_t)waves * ctx->cu_count * 4)));
+    p->wave_item_used = wave_item;
     // Small owners get hash tables: by their strategy, or on a degree-ordered copy whose wave
     // share cannot hold a bitmap from 0 (bitmap16_words is at least 2,049), where a bitmap
     // would fall back to the hash table anyway.
@@ -1812,7 +1819,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     QueueParams q{p->v_off, p->v_adj, p->d_wofs, p->d_work, n, ts.wave_owner_max, ts.fit_block, ts.max_table,
                   ts.wave_table, only_bucket, p->renumbered ? 1 : 0,
                   !small_hashed && (tune.strat[0] == kStratBitmap || tune.strat[0] == kStratMerge) ? 1 : 0,
-                  tune.item_work, tune.wave_item_work, {}};
+                  tune.item_work, wave_item, {}};
     for (int b = 0; b < kBuckets; ++b) q.strat[b] = tune.strat[b];
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->d_small) (void)hipFree(p->d_small);
@@ -1902,7 +1909,7 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
         queue_gather_small<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(kk[which], ns, small_tmp, swork, p->d_small, sw);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u64_w(ctx, sw, swork, ns, nullptr, scan2));  // exclusive work prefix, into swork
-        const uint64_t target = (uint64_t)std::max<int64_t>(1, tune.wave_item_work * waves);
+        const uint64_t target = (uint64_t)std::max<int64_t>(1, wave_item * waves);
         queue_group_flags<<<grid_cap(ctx, ns, 256, 8), 256, 0, st>>>(swork, ns, heavy_small, target, waves, flag);
         HIP_TRY(hipGetLastError());
         RTG_TRY(scan_exclusive_u32_w(ctx, flag, gid, ns, ngroups, scan2));
@@ -2502,7 +2509,8 @@ RTG_API int rtg_tc_plan_describe(const rtg_tc_plan *p, char *buf, size_t len) {
              ",renumber_ms=" + std::to_string(p->t_renum_ms) + ",streamed=" + std::to_string(p->streamed) + ",streamed16=" + std::to_string(p->streamed16) +
              ",bitmap_items=" + std::to_string(p->n_bitmap_items) + ",grid_bitmap=" + std::to_string(p->grid_bm) +
              ",wave_items=" + std::to_string(p->n_wave_items) + ",grid_wave=" + std::to_string(p->grid_wv) +
-             ",small_work=" + std::to_string(p->small_work) + ",side=" + (p->side ? "1" : "0") +
+             ",small_work=" + std::to_string(p->small_work) + ",wave_item_used=" + std::to_string(p->wave_item_used) +
+             ",side=" + (p->side ? "1" : "0") +
              ",side_blocks=" + std::to_string(p->side_blocks);
     std::snprintf(buf, len, "%s", d.c_str());
     return RTG_OK;

@@ -67,6 +67,7 @@ struct rtg_tc_plan {
     int64_t small_work = 0;          // streamed elements of the small owners
     bool side = false;               // the wave kernel runs beside the bitmap kernel
     double wave_share = 0.0;         // the wave items' estimated share of the count's time
+    int64_t wave_item_used = 0;      // the wave item size planned (the tune's, capped on small graphs)
     int32_t side_blocks = 0;         // the wave kernel's blocks per CU there (the last grid sized)
     // byte models: per-edge (no reuse), merge-only, and the owner-staged launch's reads
     uint64_t alg_bytes = 0, merge_bytes = 0, staged_bytes = 0;
