@@ -1765,11 +1765,14 @@ struct TableSizes {
     int32_t max_table, wave_table, fit_block, wave_owner_max;
 };
 
-TableSizes table_sizes(const BatchTune &tune, int32_t max_d) {
+TableSizes table_sizes(const BatchTune &tune, int32_t max_d, bool bitmap16) {
     const int waves = tune.block_threads / 64;
-    // Block table: both cuckoo halves of the largest owner that fits, at least 2048 slots, a
-    // multiple of 64 * waves so that the wave-private tables split it evenly, at most the cap.
-    // Owners whose table would exceed the cap are searched (or given a bitmap or list).
+    // Block table: both cuckoo halves of the largest owner that fits, at least 2048 slots (with
+    // bitmap16, a degree-ordered plan with bitmap owners: at least one bitmap from 0, 2,049
+    // words — at 2,048 slots no owner of R-MAT scale 17 or 18 got a bitmap, and the block
+    // owners ran as hash items in the general kernel), a multiple of 64 * waves so that the
+    // wave-private tables split it evenly, at most the cap. Owners whose table would exceed the
+    // cap are searched (or given a bitmap or list).
     const int32_t quantum = 64 * waves;
     const int32_t cap = (tune.table_cap > 0 ? tune.table_cap : kMaxTableSlots) / quantum * quantum;
     int32_t fit_block = 0;
@@ -1782,7 +1785,8 @@ TableSizes table_sizes(const BatchTune &tune, int32_t max_d) {
         }
         fit_block = lo;
     }
-    int32_t max_table = std::max<int32_t>(std::min<int32_t>(2048, cap), 2 * (int32_t)cuckoo_half(fit_block));
+    const int32_t floor = bitmap16 ? std::max<int32_t>(2048, (int32_t)bitmap16_words(0)) : 2048;
+    int32_t max_table = std::max<int32_t>(std::min<int32_t>(floor, cap), 2 * (int32_t)cuckoo_half(fit_block));
     max_table = std::min<int32_t>(cap, (max_table + quantum - 1) / quantum * quantum);
     const int32_t wave_table = max_table / waves;
     int32_t fit = 0;  // largest out-degree whose table fits a wave's share
@@ -1803,7 +1807,8 @@ int plan_items(rtg_ctx *ctx, rtg_tc_plan *p, const BatchTune &tune, int only_buc
     const int threads = tune.block_threads;
     if (threads != 256 && threads != 512) return set_error(RTG_EINVAL, "batched TC: threads must be 256 or 512");
     const int waves = threads / 64;
-    const TableSizes ts = table_sizes(tune, g->max_row);
+    const TableSizes ts = table_sizes(tune, g->max_row,
+                                      p->renumbered && (tune.strat[1] == kStratBitmap || tune.strat[2] == kStratBitmap));
     // Wave items of at most 1/4 per CU of all the streamed elements: on a small graph the tuned
     // size leaves most CUs idle (BA(100,000, 5): all 91,184 owners small, 156 wave items of
     // 4,096 elements per wave took 1.23 ms, 1,226 items of 512 0.23 ms). Graphs of more than
