@@ -2316,11 +2316,9 @@ int launch_count(rtg_ctx *ctx, rtg_tc_plan *p) {
     return RTG_OK;
 }
 
-// Per-edge counts of the plan's edge range: the range of eout is cleared, then the per-edge
-// instantiation of the batched kernel writes every span's count at its edge (edges whose
-// shorter list is empty keep 0). The total is accumulated as in a count.
-// Per-edge counts: the counting kernels store each span's count by position (p->d_ecnt), then
-// the gather writes the range's counts into eout in edge order.
+// Per-edge counts of the plan's edge range: the per-edge instantiations of the batched kernels
+// store each span's count by position (p->d_ecnt; the total is accumulated as in a count), then
+// the gather writes the range's counts into eout in edge order (0 for edges without a span).
 int launch_per_edge(rtg_ctx *ctx, rtg_tc_plan *p, uint32_t *eout) {
     hipStream_t st = ctx->stream;
     const int64_t ne = p->e_end - p->e_begin;
